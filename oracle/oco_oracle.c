@@ -1,0 +1,456 @@
+/*
+ * oco_oracle.c -- TEST INFRASTRUCTURE ONLY (see oco_oracle.h).
+ *
+ * FP64 CPU restatement of the ocotillo reference hot path.  Each function
+ * cites the reference file:line it restates.  Fortran evaluates binary
+ * operators of equal precedence left to right and unary minus after
+ * multiplication; every expression below is parenthesised to that order so
+ * that, with -ffp-contract=off and the same libm, results are bit-identical
+ * to the reference built with -fdefault-real-8 -fdefault-double-8.
+ *
+ * Constants are the values the reference's compiler folds
+ * (source/common.f90:23-81, source/continuous_opacity.f90:20-33); they were
+ * read back from the compiled reference and are pinned by the golden test.
+ */
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#include "oco_oracle.h"
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#ifdef OO_PORTABLE_MATH
+#include "../ocotillo_amd/csrc/oco_math.h"
+#define OO_EXP(x)      oco_exp(x)
+#define OO_LOG(x)      oco_log(x)
+#define OO_LOG10(x)    oco_log10(x)
+#define OO_POW(x, y)   oco_pow((x), (y))
+#define OO_POW10(y)    oco_exp10(y)
+#define OO_SQRT(x)     oco_sqrt(x)
+#else
+#define OO_EXP(x)      exp(x)
+#define OO_LOG(x)      log(x)
+#define OO_LOG10(x)    log10(x)
+#define OO_POW(x, y)   pow((x), (y))
+#define OO_POW10(y)    pow(10.0, (y))
+#define OO_SQRT(x)     sqrt(x)
+#endif
+/* Wavelength tables are set up once per run on the host with glibc in both
+ * builds (the product does the same), so they use libm directly. */
+
+/* source/common.f90 (values as folded by the compiler) */
+#define OO_SIGMA_SB             0x1.dbaa5d09e7374p-15      /* 5.6703744191844314e-05 */
+#define OO_LOG_OVERFLOW_LIMIT   0x1.3400000000000p+8       /* 308.0 */
+#define OO_MP                   0x1.02d3617bf22f4p-79      /* 1.67262192369e-24 */
+#define OO_MP1                  0x1.fa690885f1ef2p+78      /* 5.978637406556784e+23 */
+#define OO_H_PLANCK             0x1.067c559cb516cp-87      /* 6.62607015e-27 */
+#define OO_PI1                  0x1.45f306dc9c883p-2       /* 0.3183098861837907 */
+#define OO_SQRT3                0x1.bb67ae8584caap+0       /* 1.7320508075688772 */
+#define OO_EXP1                 0x1.5bf0a8b145769p+1       /* 2.718281828459045 */
+#define OO_K_CGS                0x1.3e5b21cc8f74fp-53      /* 1.380649e-16 */
+#define OO_K1_CGS               0x1.9bb717cba28f0p+52      /* 7242970516039920.0 */
+#define OO_K1_EV                0x1.6aa4251ce947ap+13      /* 11604.518121550082 */
+#define OO_HION_EV              0x1.b333333333333p+3       /* 13.6 */
+#define OO_C_LIGHT              0x1.beb9bf3a00000p+34      /* 29979245800.0 */
+#define OO_R                    0x1.7f7ccd45b2805p-36      /* 2.1798741e-11 */
+#define OO_R1                   0x1.55ca1c19b9fa9p+35      /* 45874208973.81184 */
+#define OO_LOG10E               0x1.bcb7b1526e50ep-2       /* 0.4342944819032518 */
+#define OO_IEV                  0x1.b369160866805p+3       /* 13.606577888876936 */
+#define OO_IEV1                 0x1.2d07e731cc2b6p-4       /* 0.07349386511192332 */
+#define OO_C_B                  0x1.60213dead66d7p+28      /* 369234910.67735046 */
+#define OO_HCR1                 0x1.31c52e6096baep-17      /* 9.112663236601273e-06 */
+#define OO_ONE_THIRD            0x1.5555555555555p-2       /* 1/3 */
+#define OO_TWO_THIRDS           0x1.5555555555555p-1       /* 2/3 */
+/* source/continuous_opacity.f90:20-25 */
+#define OO_SWITCH_IONFRACTION   1e-2
+#define OO_AHBF                 1.0449e-26
+#define OO_RANGSTROM            1.0968e-3
+#define OO_ALPHA0               1.0443e-26
+
+/* Fortran x**n for a run-time integer n (binary powering, low bit first). */
+static double powi_rt(double x, int n)
+{
+    double r = 1.0, b = x;
+    unsigned m = (unsigned)(n < 0 ? -n : n);
+    while (m) {
+        if (m & 1u) r = r * b;
+        m >>= 1;
+        if (m) b = b * b;
+    }
+    return n < 0 ? 1.0 / r : r;
+}
+
+/* source/continuous_opacity.f90:37-62 */
+void oo_calc_wavelength(double w1, double w0, int nw, double* wa)
+{
+    if (nw > 1) {
+        double dw = (w1 - w0) / (double)(nw - 1);
+        for (int i = 1; i <= nw; i++) wa[i - 1] = w0 + (double)(i - 1) * dw;
+    } else {
+        wa[0] = w0;
+    }
+}
+
+/* gaunt: source/continuous_opacity.f90:343-375 */
+static double gaunt(int n, double wave_cm)
+{
+    double lam_n = (double)(n * n) * OO_HCR1;
+    if (wave_cm <= lam_n) {
+        double q = (wave_cm / lam_n) - 1.0;
+        double eps2 = q * q;
+        return 1.0 - (0.17285714285714285 * (1.0 - eps2)) * pow((double)n * (1.0 + eps2), -OO_TWO_THIRDS);
+    }
+    return 0.0;
+}
+
+/* pre_calc_opacity_quantities: source/continuous_opacity.f90:64-141 */
+int oo_tables_init(oo_tables* t, int nw, const double* waves)
+{
+    static const double a_coeff[7] = {+1.99654, -1.18267e-6, +2.62423e-7, -4.40524e-11,
+                                      +3.23992e-15, -1.39568e-19, +2.78701e-24};
+    static const double b_coeff[3][5] = {{-2.276300, -1.685000, +0.766610, -0.053356, +0.000000},
+                                         {+15.28270, -9.284600, +1.993810, -0.142631, +0.000000},
+                                         {-197.789, +190.266, -67.9775, +10.6913, -0.62515}};
+    memset(t, 0, sizeof(*t));
+    t->nw = nw;
+    double* blk = (double*)calloc((size_t)nw * (12 + OO_MDIM), sizeof(double));
+    if (!blk) return -1;
+    double** arrs[12] = {&t->wa, &t->wcm1, &t->nu, &t->chi, &t->chi1, &t->wa3, &t->w1cm5,
+                         &t->gff, &t->s_coeff, &t->f1, &t->f2, &t->f3};
+    for (int k = 0; k < 12; k++) *arrs[k] = blk + (size_t)k * nw;
+    t->g = blk + (size_t)12 * nw;
+
+    t->damping_factor_const = (OO_H_PLANCK * OO_C_LIGHT) * OO_K1_CGS;
+    t->source_function_const = (2.0 * OO_H_PLANCK) * (OO_C_LIGHT * OO_C_LIGHT);
+    for (int i = 0; i < nw; i++) {
+        double wa = waves[i];
+        double wcm = wa * 1e-8;
+        t->wa[i] = wa;
+        t->wcm1[i] = 1.0 / wcm;
+        t->nu[i] = OO_C_LIGHT * t->wcm1[i];
+        t->chi[i] = 1.2398e4 / wa;
+        t->chi1[i] = 1.0 / t->chi[i];
+        t->wa3[i] = (wa * wa) * wa;
+        double w1 = t->wcm1[i];
+        t->w1cm5[i] = (((w1 * w1) * w1) * w1) * w1;   /* flang expands x**5 left to right */
+        t->gff[i] = 0.3456 * pow(wa * OO_RANGSTROM, -OO_ONE_THIRD);
+        double s = 0.0;
+        for (int k = 1; k <= 7; k++) s = s + (1e-17 * a_coeff[k - 1]) * powi_rt(wa, k - 1);
+        t->s_coeff[i] = s;
+        double lgw = log10(wa);
+        double f[3];
+        for (int r = 0; r < 3; r++) {
+            f[r] = 0.0;
+            for (int j = 1; j <= 5; j++) f[r] = f[r] + powi_rt(lgw, j - 1) * b_coeff[r][j - 1];
+        }
+        t->f1[i] = f[0]; t->f2[i] = f[1]; t->f3[i] = f[2];
+        for (int n = 1; n <= OO_MDIM; n++) t->g[(size_t)i * OO_MDIM + n - 1] = gaunt(n, wcm);
+    }
+    for (int n = 1; n <= OO_MDIM; n++) {
+        t->n1[n - 1] = 1.0 / (double)n;
+        t->chi_n[n - 1] = OO_R * (1.0 - 1.0 * (t->n1[n - 1] * t->n1[n - 1]));
+    }
+    t->chi_m = OO_R * (1.0 - t->n1[OO_MDIM - 1] * t->n1[OO_MDIM - 1]);
+    return 0;
+}
+
+void oo_tables_free(oo_tables* t)
+{
+    free(t->wa);
+    memset(t, 0, sizeof(*t));
+}
+
+/* wavelength_independent_pillars and callees: source/gas_state.f90:30-110,
+ * source/continuous_opacity.f90:143-163 */
+void oo_cell_state(double rho, double T, double fully_ionized_T, oo_cell* c)
+{
+    c->rho = rho;
+    c->T = T;
+    c->rho1 = 1.0 / rho;
+    c->T1 = 1.0 / T;
+    /* calc_hydrogen_ion_frac :48-76 */
+    double constants = 2414683039571967.0 * OO_POW(T, 1.5);
+    double exparg = (c->T1 * OO_HION_EV) * OO_K1_EV;
+    double niine_ni = constants * OO_EXP(-exparg);
+    double c_ion = (OO_MP * c->rho1) * niine_ni;
+    double x = 0.5 * (OO_SQRT(c_ion * c_ion + 4.0 * c_ion) - c_ion);
+    if (T > fully_ionized_T) x = 1.0;
+    c->x = x;
+    /* solve_gas_state :78-99 */
+    c->n = rho * OO_MP1;
+    c->ninv = c->rho1 * OO_MP;
+    c->nHII = x * c->n;
+    c->nHI = c->n - c->nHII;
+    c->ne = c->nHII;
+    c->fn = (c->nHI != 0.0) ? c->nHI / (c->nHI + c->nHII) : 0.0;
+    /* get_electron_pressure :101-110 */
+    c->Pe = (c->ne * OO_K_CGS) * T;
+    /* get_electron_thomson_scattering continuous_opacity.f90:143-151 */
+    c->es = (0.6648e-24 * c->nHII) * c->ninv;
+    c->theta = 5040.0 * c->T1;
+    c->theta1 = 1.0 / c->theta;
+    c->lgtheta = OO_LOG10(c->theta);
+    c->lgtheta2 = c->lgtheta * c->lgtheta;
+    /* get_hydrogen_ion_bound_free continuous_opacity.f90:153-163 */
+    c->hmbf = ((4.158e-10 * c->Pe) * OO_POW(c->theta, 2.5)) * OO_POW10(0.754 * c->theta);
+}
+
+/* calc_opacity_and_albedo and callees: source/continuous_opacity.f90:165-341,392-409 */
+void oo_opacity_point(const oo_tables* t, const oo_cell* c, int iw, oo_point* p)
+{
+    /* get_hydrogen_stimulated_emission :165-173 */
+    double stim = 1.0 - OO_POW10(-(t->chi[iw] * c->theta));
+    /* get_kappa_H_bf :203-246 */
+    double ktemp = OO_K_CGS * c->T;
+    double ktemp1 = OO_K1_CGS * c->T1;
+    double sm = 0.0;
+    for (int n = 0; n < OO_MDIM; n++) {
+        double n1 = t->n1[n];
+        sm = sm + (t->g[(size_t)iw * OO_MDIM + n] * ((n1 * n1) * n1)) * OO_EXP(-(t->chi_n[n] * ktemp1));
+    }
+    double unsold = ((0.5 * ktemp) * OO_R1) * (OO_EXP(-(t->chi_m * ktemp1)) - OO_EXP(-(OO_R * ktemp1)));
+    double kHbf = (((OO_AHBF * stim) * c->fn) * t->wa3[iw]) * (unsold + sm);
+    /* get_kappa_Hm_bf :302-324 */
+    double kHmbf = (t->wa[iw] < 16000.0) ? ((c->hmbf * stim) * c->fn) * t->s_coeff[iw] : 0.0;
+    /* get_kappa_Hm_ff :326-341 */
+    double flam = ((t->f1[iw] + t->f2[iw] * c->lgtheta) + t->f3[iw] * c->lgtheta2) - 26.0;
+    double kHmff = (((c->fn * c->ne) * OO_K_CGS) * c->T) * OO_POW10(flam);
+    /* :188-189 */
+    double kappa = (((kHbf + kHmbf) + kHmff) + c->es) * OO_MP1;
+    double alpha = kappa * c->rho;
+    /* calc_kappa_H_ff :248-279 */
+    if ((1.0 - c->x) > OO_SWITCH_IONFRACTION) {
+        double g_ff = 1.0 + t->gff[iw] * (((c->theta1 * OO_LOG10E) * t->chi1[iw]) + 0.5);
+        double ef = (((0.5 * OO_LOG10E) * c->theta1) * OO_IEV1) * OO_POW10(-(c->theta * OO_IEV));
+        double kHff = (((((stim * c->fn) * OO_ALPHA0) * t->wa3[iw]) * g_ff) * ef);
+        kappa = kappa + kHff * OO_MP1;
+        alpha = alpha + kappa * c->rho;
+    } else {
+        /* bremsstrahlung_absorptionCoeff :281-300, zprotons = 1 */
+        double freq = t->nu[iw];
+        double inner = (freq * 1e-9) * OO_POW(c->T * 1e-4, -1.5);
+        double gB = OO_LOG(OO_EXP(5.960 - (OO_SQRT3 * OO_PI1) * OO_LOG(inner)) + OO_EXP1);
+        double lnQ = (((OO_LOG(c->ne * (c->nHI + c->nHII)) + 2.0 * OO_LOG(1.0)) - 0.5 * OO_LOG(c->T))
+                      - 3.0 * OO_LOG(freq))
+                     + OO_LOG(1.0 - OO_EXP(-(((OO_H_PLANCK * freq) * OO_K1_CGS) * c->T1)));
+        double kRho = (OO_C_B * OO_EXP(lnQ)) * gB;
+        alpha = alpha + kRho;
+        kappa = kappa + alpha * c->rho1;
+    }
+    p->albedo = (c->es * OO_MP1) / kappa;
+    p->alpha = alpha;
+    p->alpha1 = 1.0 / alpha;
+    p->alpha2 = alpha * alpha;
+    p->stim = stim;
+    /* get_source_function :392-409 */
+    double damp = (t->damping_factor_const * c->T1) * t->wcm1[iw];
+    if (damp > OO_LOG_OVERFLOW_LIMIT) damp = OO_LOG_OVERFLOW_LIMIT;
+    p->S = (t->source_function_const * t->w1cm5[iw]) / (OO_EXP(damp) - 1.0);
+}
+
+/* grey_parameters: source/continuous_opacity.f90:377-390 */
+void oo_grey_point(const oo_cell* c, double sigma_grey, oo_point* p)
+{
+    double T = c->T;
+    p->S = OO_SIGMA_SB * (((T * T) * T) * T);
+    double ag = (3.68e22 * OO_POW(T, -3.5)) * c->rho;
+    p->alpha = (ag + sigma_grey) * c->rho;
+    p->albedo = sigma_grey / (ag + sigma_grey);
+    p->alpha1 = 1.0 / p->alpha;
+    p->alpha2 = p->alpha * p->alpha;
+    p->stim = 0.0;
+}
+
+/* get_tridag_coefficients: source/auxiliary.f90:111-201 */
+void oo_tridag_coefficients(int nz, const double* al, const double* al1, const double* al2,
+                            const double* om, const double* S, double dz, double dz2,
+                            double* a, double* b, double* c, double* d)
+{
+    for (int i = 1; i < nz - 1; i++) {
+        double kp = 0.5 * (al[i + 1] + al[i]);
+        double km = 0.5 * (al[i] + al[i - 1]);
+        a[i] = al2[i] / km;
+        c[i] = al2[i] / kp;
+        double zeta = (dz2 * ((al[i] * al[i]) * al[i])) * (1.0 - om[i]);
+        b[i] = -((a[i] + c[i]) + zeta);
+        d[i] = -(S[i] * zeta);
+    }
+    double zeta = ((al[0] * dz2) * (1.0 - om[0])) * 0.25;
+    a[0] = 0.0;
+    b[0] = -((al1[0] + dz) + zeta) * al2[0];
+    c[0] = al[0];
+    d[0] = -((S[0] * zeta) * al2[0]);
+    int n = nz - 1;
+    zeta = ((al[n] * dz2) * (1.0 - om[n])) * 0.25;
+    a[n] = al[n];
+    b[n] = -((al1[n] + dz) + zeta) * al2[n];
+    c[n] = 0.0;
+    d[n] = -((S[n] * zeta) * al2[n]);
+}
+
+/* tridag: source/auxiliary.f90:14-47 */
+int oo_tridag(int n, const double* a, const double* b, const double* c, const double* r,
+              double* u, double* gam, int* jfail)
+{
+    double bet = b[0];
+    if (bet == 0.0) { *jfail = 1; return 1; }
+    u[0] = r[0] / bet;
+    for (int j = 1; j < n; j++) {
+        gam[j] = c[j - 1] / bet;
+        bet = b[j] - a[j] * gam[j];
+        if (bet == 0.0) { *jfail = j + 1; return 2; }
+        u[j] = (r[j] - a[j] * u[j - 1]) / bet;
+    }
+    for (int j = n - 2; j >= 0; j--) u[j] = u[j] - gam[j + 1] * u[j + 1];
+    return 0;
+}
+
+/* update_ghosts, der, calc_flux: source/auxiliary.f90:49-78,96-109 */
+void oo_ghosts_flux(int nz, double* U, const double* al1, double dz1, double* V)
+{
+    const int ng = 3, n1 = ng, n2 = ng + nz - 1;   /* 0-based */
+    for (int i = 1; i <= ng; i++) {
+        U[n1 - i] = 2.0 * U[n1] - U[n1 + i];
+        U[n2 + i] = 2.0 * U[n2] - U[n2 - i];
+    }
+    const double c60 = 1.0 / 60.0;
+    for (int i = 0; i < nz; i++) {
+        const double* f = U + n1 + i;
+        double df = c60 * (((45.0 * (f[1] - f[-1])) - (9.0 * (f[2] - f[-2]))) + (f[3] - f[-3]));
+        V[i] = ((-1.0 * al1[i]) * df) * dz1;
+    }
+}
+
+/* One (column, wavelength) system: source/flux_feautrier.f90:183-216. */
+static int solve_system(int nz, const oo_tables* t, const oo_cell* cells, int iw, int lgrey,
+                        double sigma_grey, double dz, double dz1, double dz2, double* work,
+                        double* U, double* V, double* A, int* jfail, double sums[4])
+{
+    double *al = work, *al1 = al + nz, *al2 = al1 + nz, *om = al2 + nz, *S = om + nz;
+    double *a = S + nz, *b = a + nz, *c = b + nz, *d = c + nz, *gam = d + nz;
+    for (int i = 0; i < nz; i++) {
+        oo_point p;
+        if (lgrey) oo_grey_point(&cells[i], sigma_grey, &p);
+        else oo_opacity_point(t, &cells[i], iw, &p);
+        al[i] = p.alpha; al1[i] = p.alpha1; al2[i] = p.alpha2; om[i] = p.albedo; S[i] = p.S;
+    }
+    oo_tridag_coefficients(nz, al, al1, al2, om, S, dz, dz2, a, b, c, d);
+    if (sums) {
+        sums[0] = sums[1] = sums[2] = sums[3] = 0.0;
+        for (int i = 0; i < nz; i++) { sums[0] += a[i]; sums[1] += b[i]; sums[2] += c[i]; sums[3] += d[i]; }
+    }
+    int st = oo_tridag(nz, a, b, c, d, U + 3, gam, jfail);
+    if (st) return st;
+    oo_ghosts_flux(nz, U, al1, dz1, V);
+    memcpy(A, al, sizeof(double) * (size_t)nz);
+    return 0;
+}
+
+/* The column loop: source/flux_feautrier.f90:160-228.  Columns run in
+ * parallel; the first failure is chosen in the reference's (ix, iy, iw) order. */
+int oo_solve_block(int nz, int nyloc, int nxloc, const oo_tables* t,
+                   double dz, double dz1, double dz2,
+                   const double* rho3d, const double* T3d,
+                   double fully_ionized_T, int lgrey, double sigma_grey,
+                   double* U, double* V, double* alpha, oo_diag* diag, int nthreads)
+{
+    const int nw = t->nw, mz = nz + 6;
+    const long ncol = (long)nxloc * nyloc;
+    long first_fail = -1;
+    oo_diag dg;
+    memset(&dg, 0, sizeof(dg));
+#ifdef _OPENMP
+    if (nthreads <= 0) nthreads = omp_get_max_threads();
+#else
+    nthreads = 1;
+#endif
+#pragma omp parallel num_threads(nthreads)
+    {
+        oo_cell* cells = (oo_cell*)malloc(sizeof(oo_cell) * (size_t)nz);
+        double* work = (double*)malloc(sizeof(double) * (size_t)nz * 10);
+        long my_fail = -1; int my_st = 0, my_j = 0;
+#pragma omp for schedule(dynamic, 4)
+        for (long col = 0; col < ncol; col++) {
+            for (int i = 0; i < nz; i++)
+                oo_cell_state(rho3d[col * nz + i], T3d[col * nz + i], fully_ionized_T, &cells[i]);
+            for (int iw = 0; iw < nw; iw++) {
+                long sys = (long)iw * ncol + col;
+                double* Us = U + sys * mz;
+                double* Vs = V + sys * nz;
+                double* As = alpha + sys * nz;
+                int jf = 0;
+                double sums[4];
+                int st = solve_system(nz, t, cells, iw, lgrey, sigma_grey, dz, dz1, dz2, work,
+                                      Us, Vs, As, &jf, (col == 0 && iw == 0) ? sums : NULL);
+                if (col == 0 && iw == 0) {
+                    dg.sum_a = sums[0]; dg.sum_b = sums[1]; dg.sum_c = sums[2]; dg.sum_d = sums[3];
+                }
+                if (st) {
+                    /* reference order: ix outer, iy, iw inner -> key = col*nw + iw */
+                    long key = col * nw + iw;
+                    if (my_fail < 0 || key < my_fail) { my_fail = key; my_st = st; my_j = jf; }
+                    break;
+                }
+            }
+        }
+#pragma omp critical
+        {
+            if (my_fail >= 0 && (first_fail < 0 || my_fail < first_fail)) {
+                first_fail = my_fail; dg.status = my_st; dg.fail_j = my_j;
+            }
+        }
+        free(cells); free(work);
+    }
+    if (first_fail >= 0) {
+        long col = first_fail / nw;
+        dg.fail_iw = (int)(first_fail % nw) + 1;
+        dg.fail_ix = (int)(col / nyloc) + 1;
+        dg.fail_iy = (int)(col % nyloc) + 1;
+    } else {
+        dg.minU = dg.maxU = U[3]; dg.minV = dg.maxV = V[0]; dg.minA = dg.maxA = alpha[0];
+        for (int i = 0; i < nz; i++) {
+            double u = U[3 + i], v = V[i], a = alpha[i];
+            if (u < dg.minU) dg.minU = u; if (u > dg.maxU) dg.maxU = u;
+            if (v < dg.minV) dg.minV = v; if (v > dg.maxV) dg.maxV = v;
+            if (a < dg.minA) dg.minA = a; if (a > dg.maxA) dg.maxA = a;
+        }
+    }
+    if (diag) *diag = dg;
+    return dg.status;
+}
+
+/* calc_grid: source/disk.f90:42-62 */
+void oo_calc_grid(int nz, double z1, double z0, double* z, double* dz)
+{
+    double d = (z1 - z0) / (double)(nz - 1);
+    for (int i = 1; i <= nz; i++) z[3 + i - 1] = z0 + (double)(i - 1) * d;
+    for (int i = 1; i <= 3; i++) {
+        z[3 - i] = z[3] - (double)i * d;
+        z[3 + nz - 1 + i] = z[3 + nz - 1] + (double)i * d;
+    }
+    *dz = d;
+}
+
+/* calc_density: source/disk.f90:111-129 */
+void oo_calc_density(int nz, const double* z, double rho0, double H, double rho_floor, double* rho)
+{
+    for (int i = 0; i < nz; i++) {
+        double zi = z[3 + i];
+        double r = rho0 * exp(-(((0.5 * (zi * zi)) / (H * H))));
+        rho[i] = (r < rho_floor) ? rho_floor : r;
+    }
+}
+
+/* calc_temperature: source/disk.f90:64-109.  Tgauss is built from this z
+ * unless a cached one (first call's) is supplied. */
+void oo_calc_temperature(int nz, const double* z, double midTemp, double sigma, double floorTemp,
+                         double switchTemp, const double* Tgauss_in, int from_athena, double* T)
+{
+    for (int i = 0; i < nz; i++) {
+        double zi = z[3 + i];
+        double tg = Tgauss_in ? Tgauss_in[i] : midTemp * exp(-(((0.5 * (zi * zi)) / (sigma * sigma))));
+        if (!from_athena) T[i] = midTemp;
+        if (T[i] < switchTemp) T[i] = (tg > floorTemp) ? tg : floorTemp;
+    }
+}
