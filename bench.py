@@ -1,0 +1,130 @@
+"""Benchmark of the FP64 drop-in path on one MI355X.
+
+Workload (one step = one pass of the hot path over one block, inputs already
+resident in HBM): the non-grey drop-in block, 64 x 64 columns, nz = 256,
+nw = 128 log-spaced 100 A - 10 um, seeded disk profiles
+(ocotillo_amd.workloads.config2) -> 1.342e8 (mesh cell, wavelength) points.
+
+Prints one JSON line: points/s, the roofline of the dominant kernel (device
+time from HIP events on the library's stream) and a CPU baseline (the FP64
+CPU oracle, OpenMP over the host cores, on the same block).
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, REPO)
+
+HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6
+
+
+def algorithmic_bytes(nz, ncol, nw):
+    """Compulsory HBM bytes of one block: U with ghosts, V and alpha written
+    once (FP64), rho and T read once."""
+    return 8.0 * ((nz + 6) + 2 * nz) * ncol * nw + 16.0 * nz * ncol
+
+
+def sweep_bytes(nz, ncol, nw):
+    """Algorithmic bytes of the sweep kernel per launch: its outputs U, V, alpha."""
+    return 8.0 * ((nz + 6) + 2 * nz) * ncol * nw
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--nx", type=int, default=64)
+    ap.add_argument("--ny", type=int, default=64)
+    ap.add_argument("--nz", type=int, default=256)
+    ap.add_argument("--nw", type=int, default=128)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build measures one GPU; the path shards by columns/wavelengths with no "
+              "exchange (see DESIGN.md), multi-GPU runs are not part of it", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import ocotillo_amd as oco
+    from ocotillo_amd.workloads import config2
+
+    rho, T, dz, waves = config2(args.nx, args.ny, args.nz, args.nw)
+    nx, ny, nz = rho.shape
+    nw = waves.size
+    ncol = nx * ny
+    points = float(nz) * ncol * nw
+
+    dev = torch.device("cuda:0")
+    d_rho = torch.from_numpy(rho).to(dev)
+    d_T = torch.from_numpy(T).to(dev)
+    d_U = torch.empty((nw, nx, ny, nz + 6), dtype=torch.float64, device=dev)
+    d_V = torch.empty((nw, nx, ny, nz), dtype=torch.float64, device=dev)
+    d_A = torch.empty((nw, nx, ny, nz), dtype=torch.float64, device=dev)
+    torch.cuda.synchronize()
+
+    ctx = oco.Context(0)
+    ctx.set_wavelengths(waves)
+    ctx.set_physics(2e4, False, 0.4)
+
+    def step():
+        return ctx.solve_block_device(nz, ny, nx, dz, d_rho.data_ptr(), d_T.data_ptr(), d_U.data_ptr(),
+                                      d_V.data_ptr(), d_A.data_ptr())
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    cells_ms, sweep_ms = [], []
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        d = step()
+        cells_ms.append(d.cells_ms)
+        sweep_ms.append(d.sweep_ms)
+        print("step %d: cells %.3f ms sweep %.3f ms" % (i, d.cells_ms, d.sweep_ms), file=sys.stderr, flush=True)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    ms_per_step = (t1 - t0) * 1e3 / args.steps
+    value = points / (ms_per_step * 1e-3)
+
+    sw = float(np.mean(sweep_ms))
+    achieved_gbs = sweep_bytes(nz, ncol, nw) / (sw * 1e-3) / 1e9
+    roofline = {"kernel": "sweep_fp64", "bound": "hbm", "achieved": round(achieved_gbs, 1),
+                "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved_gbs / HBM_PEAK_GBS, 4),
+                "traffic": None, "kernel_ms": round(sw, 4), "cells_kernel_ms": round(float(np.mean(cells_ms)), 4)}
+
+    cpu = None
+    if not args.no_cpu:
+        from oracle.oracle import Oracle
+        o = Oracle(waves)
+        nthreads = min(os.cpu_count() or 1, 16)
+        tc0 = time.perf_counter()
+        o.solve_block(rho, T, dz, nthreads=nthreads)
+        tc = time.perf_counter() - tc0
+        cpu = {"value": round(points / tc, 1), "unit": "points/s", "cores": nthreads, "kind": "port",
+               "sample": "the full 64x64x256 x 128-wavelength block, one pass (%.2f s wall)" % tc}
+
+    out = {
+        "metric": "mesh*lambda points/s mixed vs FP64 at 1e-6 flux err; serve p50/p99 latency",
+        "value": round(value, 1), "unit": "points/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(ms_per_step, 4), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "FP64 drop-in block: %dx%d columns, nz=%d, nw=%d log-spaced 100A-10um, "
+                               "seeded disk profiles (rho floor 1e-15)" % (nx, ny, nz, nw),
+                   "points_per_step": points, "mode": "fp64"},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+    ctx.close()
+
+
+if __name__ == "__main__":
+    main()

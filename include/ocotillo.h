@@ -1,0 +1,106 @@
+/*
+ * ocotillo.h -- C ABI of the MI355X implementation of ocotillo's
+ * per-column, per-wavelength hot path (Saha state, hydrogen continuous
+ * opacity, Feautrier tridiagonal solve, 6th-order flux).
+ *
+ * The reference has no FFI: its hot path is the Fortran call sequence inside
+ * the column loop of source/flux_feautrier.f90:160-228, with state in module
+ * globals.  Each entry point below names the reference code it replaces.
+ * Arrays are plain pointers in the reference's Fortran (column-major)
+ * layouts; scalars are passed by value, so the Fortran host binds them with
+ * bind(C) / value and the Python tools with ctypes.
+ *
+ * Threading: a context is used by one host thread at a time.  Every call is
+ * synchronous unless its name ends in _async.  No global state.
+ */
+#ifndef OCOTILLO_H
+#define OCOTILLO_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes.  OCO_E_PIVOT0_STAGE1/2 reproduce the reference's two
+ * "ERROR tridiag stage" stops (source/auxiliary.f90:27-30, 36-39). */
+enum {
+    OCO_OK = 0,
+    OCO_E_PIVOT0_STAGE1 = 1,
+    OCO_E_PIVOT0_STAGE2 = 2,
+    OCO_E_ARG = 3,
+    OCO_E_OOM = 4,
+    OCO_E_HIP = 5,
+    OCO_E_STATE = 6,
+    OCO_E_NODEVICE = 7,
+    OCO_E_BUSY = 8
+};
+
+/* Precision modes of oco_solve_block*. */
+enum {
+    OCO_MODE_FP64 = 0,   /* drop-in: the reference's FP64 arithmetic, operation by operation */
+    OCO_MODE_MIXED = 1   /* FP32 opacity with exponent scaling, FP64 Saha and elimination */
+};
+
+typedef struct oco_ctx oco_ctx;
+
+/* Diagnostics of one block solve.  fail_* are the 1-based (ix, iy, iw, row)
+ * of the first zero pivot in the reference's loop order; sum_* and min/max
+ * are the first-pillar canaries the reference prints for (ix,iy,iw)=(1,1,1)
+ * (source/flux_feautrier.f90:198-224); *_ms are device times of the kernels
+ * (HIP events on the context's stream). */
+typedef struct {
+    int status;
+    int fail_ix, fail_iy, fail_iw, fail_j;
+    double sum_a, sum_b, sum_c, sum_d;
+    double minU, maxU, minV, maxV, minA, maxA;
+    double cells_ms, sweep_ms, total_ms;
+} oco_diag;
+
+/* Create a context on HIP device `device`, allowed to hold at most
+ * `hbm_budget` bytes of device memory (0 = no limit).  Replaces nothing in
+ * the reference (its state is module globals, source/common.f90:1-83). */
+int oco_create(int device, size_t hbm_budget, oco_ctx** ctx);
+int oco_destroy(oco_ctx* ctx);
+
+/* Wavelength grid in Angstrom.  Replaces pre_calc_opacity_quantities
+ * (source/continuous_opacity.f90:64-141): builds the per-wavelength tables
+ * and keeps them resident on the device. */
+int oco_set_wavelengths(oco_ctx* ctx, int nw, const double* waves_angstrom);
+
+/* Replaces the &gas_state_input namelist value fully_ionized_T
+ * (source/gas_state.f90:14-28) and the &input values lgrey, sigma_grey
+ * (source/flux_feautrier.f90:51-58). */
+int oco_set_physics(oco_ctx* ctx, double fully_ionized_T, int lgrey, double sigma_grey);
+
+/* One processor block: replaces the column loop body
+ * source/flux_feautrier.f90:160-228 (wavelength_independent_pillars,
+ * calc_opacity_and_albedo / grey_parameters, get_tridag_coefficients,
+ * tridag, update_ghosts, calc_flux and the stores at :215-216).
+ * Host buffers, Fortran layouts as in source/flux_feautrier.f90:36-42:
+ *   rho3d, T3d   (nz, nyloc, nxloc)        read
+ *   U            (nz+6, nyloc, nxloc, nw)  written, 3 ghost cells each side
+ *   V, absorp    (nz, nyloc, nxloc, nw)    written
+ * dz, dz1, dz2 are the grid step and the 1/dz, dz**2 the reference sets on
+ * the root block (:151-154).  Returns OCO_OK or a status code; *diag (may
+ * be NULL) receives the details. */
+int oco_solve_block(oco_ctx* ctx, int nz, int nyloc, int nxloc, double dz, double dz1, double dz2,
+                    const double* rho3d, const double* T3d, double* U, double* V, double* absorp,
+                    int mode, oco_diag* diag);
+
+/* Same as oco_solve_block with every array already in device memory
+ * (same layouts).  No host<->device copies except the diagnostics. */
+int oco_solve_block_device(oco_ctx* ctx, int nz, int nyloc, int nxloc, double dz, double dz1,
+                           double dz2, const double* d_rho3d, const double* d_T3d, double* d_U,
+                           double* d_V, double* d_absorp, int mode, oco_diag* diag);
+
+/* Message text for a status code. */
+const char* oco_strerror(int status);
+
+/* Library version string. */
+const char* oco_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,163 @@
+"""ocotillo_amd -- MI355X implementation of ocotillo's per-column,
+per-wavelength hot path (Saha state, hydrogen continuous opacity, Feautrier
+solve, 6th-order flux), behind the C ABI in include/ocotillo.h.
+
+This module is the ctypes binding of that ABI (the binding the reference's
+Python tools would use).  It never computes on the CPU: if the HIP library
+is missing or no GPU is present, every solve raises.
+
+Layouts are the reference's Fortran ones (source/flux_feautrier.f90:36-42);
+as C-ordered numpy arrays they are indexed
+    rho3d, T3d : [ix, iy, z]        (Fortran (nz, nyloc, nxloc))
+    U          : [iw, ix, iy, z+6]  (Fortran (mz, nyloc, nxloc, nw))
+    V, absorp  : [iw, ix, iy, z]
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+__all__ = ["OcoError", "Context", "Diag", "load_library", "LIB_PATH", "MODE_FP64", "MODE_MIXED",
+           "EXPORTS"]
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG, "libocotillo.so")
+
+OK, E_PIVOT0_STAGE1, E_PIVOT0_STAGE2, E_ARG, E_OOM, E_HIP, E_STATE, E_NODEVICE, E_BUSY = range(9)
+MODE_FP64, MODE_MIXED = 0, 1
+
+# every symbol include/ocotillo.h declares
+EXPORTS = ["oco_create", "oco_destroy", "oco_set_wavelengths", "oco_set_physics", "oco_solve_block",
+           "oco_solve_block_device", "oco_strerror", "oco_version"]
+
+
+class Diag(C.Structure):
+    _fields_ = ([(n, C.c_int) for n in ("status", "fail_ix", "fail_iy", "fail_iw", "fail_j")]
+                + [(n, C.c_double) for n in ("sum_a", "sum_b", "sum_c", "sum_d", "minU", "maxU",
+                                             "minV", "maxV", "minA", "maxA", "cells_ms", "sweep_ms",
+                                             "total_ms")])
+
+
+class OcoError(RuntimeError):
+    def __init__(self, status, msg, diag=None):
+        super().__init__(msg)
+        self.status = status
+        self.diag = diag
+
+
+_lib = None
+_DP = C.POINTER(C.c_double)
+
+
+def load_library(path=None):
+    """Load libocotillo.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None and path is None:
+        return _lib
+    p = path or LIB_PATH
+    if not os.path.exists(p):
+        raise OcoError(E_STATE, f"HIP library {p} is not built (run python -m ocotillo_amd.build)")
+    L = C.CDLL(p)
+    L.oco_create.argtypes = [C.c_int, C.c_size_t, C.POINTER(C.c_void_p)]
+    L.oco_destroy.argtypes = [C.c_void_p]
+    L.oco_set_wavelengths.argtypes = [C.c_void_p, C.c_int, _DP]
+    L.oco_set_physics.argtypes = [C.c_void_p, C.c_double, C.c_int, C.c_double]
+    L.oco_solve_block.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double,
+                                  _DP, _DP, _DP, _DP, _DP, C.c_int, C.POINTER(Diag)]
+    L.oco_solve_block_device.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double,
+                                         C.c_double, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                         C.c_void_p, C.c_int, C.POINTER(Diag)]
+    L.oco_strerror.restype = C.c_char_p
+    L.oco_strerror.argtypes = [C.c_int]
+    L.oco_version.restype = C.c_char_p
+    if path is None:
+        _lib = L
+    return L
+
+
+def strerror(status):
+    return load_library().oco_strerror(status).decode()
+
+
+def _ptr(a):
+    return a.ctypes.data_as(_DP)
+
+
+class Context:
+    """One device context: resident wavelength tables and work buffers.
+
+    Mirrors the reference's run set-up: set_wavelengths replaces
+    pre_calc_opacity_quantities, set_physics the &gas_state_input /
+    lgrey / sigma_grey values, solve_block the column loop of one
+    processor block."""
+
+    def __init__(self, device=0, hbm_budget=0):
+        self.L = load_library()
+        h = C.c_void_p()
+        st = self.L.oco_create(device, hbm_budget, C.byref(h))
+        if st != OK:
+            raise OcoError(st, "oco_create: " + strerror(st))
+        self.h = h
+        self.nw = 0
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.oco_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def set_wavelengths(self, waves_angstrom):
+        w = np.ascontiguousarray(waves_angstrom, dtype=np.float64)
+        st = self.L.oco_set_wavelengths(self.h, w.size, _ptr(w))
+        if st != OK:
+            raise OcoError(st, "oco_set_wavelengths: " + strerror(st))
+        self.nw = w.size
+
+    def set_physics(self, fully_ionized_T=2e4, lgrey=False, sigma_grey=0.4):
+        st = self.L.oco_set_physics(self.h, fully_ionized_T, int(bool(lgrey)), sigma_grey)
+        if st != OK:
+            raise OcoError(st, "oco_set_physics: " + strerror(st))
+
+    def solve_block(self, rho3d, T3d, dz, dz1=None, dz2=None, mode=MODE_FP64, out=None, raise_on_pivot=True):
+        """Host arrays in, host arrays out.  Returns (U, V, absorp, Diag)."""
+        rho3d = np.ascontiguousarray(rho3d, dtype=np.float64)
+        T3d = np.ascontiguousarray(T3d, dtype=np.float64)
+        if rho3d.ndim != 3 or rho3d.shape != T3d.shape:
+            raise OcoError(E_ARG, "rho3d and T3d must both be [nxloc, nyloc, nz]")
+        nxloc, nyloc, nz = rho3d.shape
+        nw = self.nw
+        if out is None:
+            U = np.empty((nw, nxloc, nyloc, nz + 6))
+            V = np.empty((nw, nxloc, nyloc, nz))
+            A = np.empty((nw, nxloc, nyloc, nz))
+        else:
+            U, V, A = out
+        d = Diag()
+        dz1 = 1.0 / dz if dz1 is None else dz1
+        dz2 = dz * dz if dz2 is None else dz2
+        st = self.L.oco_solve_block(self.h, nz, nyloc, nxloc, dz, dz1, dz2, _ptr(rho3d), _ptr(T3d), _ptr(U),
+                                    _ptr(V), _ptr(A), mode, C.byref(d))
+        if st != OK and (raise_on_pivot or st not in (E_PIVOT0_STAGE1, E_PIVOT0_STAGE2)):
+            raise OcoError(st, "oco_solve_block: " + strerror(st), d)
+        return U, V, A, d
+
+    def solve_block_device(self, nz, nyloc, nxloc, dz, d_rho, d_T, d_U, d_V, d_A, dz1=None, dz2=None,
+                           mode=MODE_FP64, raise_on_pivot=True):
+        """Device pointers (ints, e.g. torch tensor.data_ptr()) in and out."""
+        d = Diag()
+        dz1 = 1.0 / dz if dz1 is None else dz1
+        dz2 = dz * dz if dz2 is None else dz2
+        st = self.L.oco_solve_block_device(self.h, nz, nyloc, nxloc, dz, dz1, dz2, C.c_void_p(d_rho),
+                                           C.c_void_p(d_T), C.c_void_p(d_U), C.c_void_p(d_V), C.c_void_p(d_A),
+                                           mode, C.byref(d))
+        if st != OK and (raise_on_pivot or st not in (E_PIVOT0_STAGE1, E_PIVOT0_STAGE2)):
+            raise OcoError(st, "oco_solve_block_device: " + strerror(st), d)
+        return d

@@ -1,0 +1,254 @@
+/*
+ * oco_kernels.hip -- gfx950 kernels of the FP64 drop-in path.
+ *
+ *   cells_fp64  one lane per cell: Saha state, number densities and every
+ *               temperature-only factor of the opacity (the reference's
+ *               wavelength_independent_pillars plus the T-only parts of
+ *               calc_opacity_and_albedo), stored as a 192-byte record.
+ *   sweep_fp64  one lane per (column, wavelength) system; a wavefront takes
+ *               64 consecutive systems (wavelength fastest, so for nw >= 64
+ *               all lanes share one column and its records are wave-uniform
+ *               scalar loads).  Forward pass: opacity, albedo and source
+ *               function recomputed in registers for cell i+1, rows of the
+ *               tridiagonal system, Thomas elimination; gam, u and 1/alpha
+ *               go to a per-wavefront scratch slot laid out [z][3][lane] so
+ *               every access is one coalesced 512-byte line.  Backward pass:
+ *               back-substitution fused with the ghost extrapolation, the
+ *               6th-order derivative (7-point register window) and V.
+ *
+ * All FP64, compiled with -ffp-contract=off; the elementary functions are
+ * oco_math.h's, read from an LDS copy of their tables.
+ */
+#include "oco_internal.h"
+#include "oco_physics.h"
+
+__constant__ oco_mtab oco_mtab_dev = OCO_MTAB_INIT;
+
+#define OCO_WAVE 64
+
+__device__ static inline void load_mtab(oco_mtab* smt)
+{
+    const double* src = (const double*)&oco_mtab_dev;
+    double* dst = (double*)smt;
+    for (int i = threadIdx.x; i < (int)(sizeof(oco_mtab) / sizeof(double)); i += blockDim.x) dst[i] = src[i];
+    __syncthreads();
+}
+
+__global__ __launch_bounds__(256) void cells_fp64(const double* __restrict__ rho, const double* __restrict__ T,
+                                                  long ncells, double fully_ionized_T, double* __restrict__ cells)
+{
+    __shared__ oco_mtab smt;
+    load_mtab(&smt);
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < ncells; i += (long)gridDim.x * blockDim.x) {
+        double rec[CR_NFIELD];
+        oco_cell_record(rho[i], T[i], fully_ionized_T, &smt, rec);
+        double* out = cells + i * OCO_CELL_STRIDE;
+#pragma unroll
+        for (int k = 0; k < CR_NFIELD; k++) out[k] = rec[k];
+    }
+}
+
+/* opacity, albedo, source function of cell z for one lane */
+template <bool GREY>
+__device__ static inline oco_pt point_at(const oco_sweep_args& a, long col, int z, const double* w,
+                                         const oco_mtab* M)
+{
+    if (GREY) {
+        long c = col * a.nz + z;
+        return oco_grey_point(a.rho[c], a.T[c], a.sigma_grey, M);
+    } else {
+        return oco_point(a.cells + (col * a.nz + z) * OCO_CELL_STRIDE, w, M);
+    }
+}
+
+__device__ static inline void record_fail(unsigned long long* fail, long sys, int stage, int row)
+{
+    unsigned long long key = ((unsigned long long)sys << 32) | ((unsigned long long)stage << 28) |
+                             (unsigned long long)row;
+    atomicMin(fail, key);
+}
+
+template <bool GREY, bool UNIFORM>
+__global__ __launch_bounds__(OCO_WAVE) void sweep_fp64(const oco_sweep_args a)
+{
+    __shared__ oco_mtab smt;
+    load_mtab(&smt);
+    const int lane = threadIdx.x;
+    const int nz = a.nz, nw = a.nw;
+    const long nsys = a.ncol * nw;
+    const long ntiles = (nsys + OCO_WAVE - 1) / OCO_WAVE;
+    const double dz = a.dz, dz1 = a.dz1, dz2 = a.dz2;
+    double* scr = a.scratch + (long)blockIdx.x * nz * 3 * OCO_WAVE + lane;
+    const long ostride = a.ncol;   /* systems per wavelength in the output layout */
+
+    for (long tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+        long sys0 = tile * OCO_WAVE;
+        long col;
+        int iw;
+        if (UNIFORM) {            /* nw % 64 == 0: one column per wavefront */
+            col = sys0 / nw;
+            iw = (int)(sys0 % nw) + lane;
+        } else {
+            long s = sys0 + lane;
+            if (s >= nsys) break;
+            col = s / nw;
+            iw = (int)(s % nw);
+        }
+        const long sys = col * nw + iw;
+        const long osys = (long)iw * ostride + col;
+        double* Uo = a.U + osys * (nz + 6);
+        double* Vo = a.V + osys * nz;
+        double* Ao = a.A + osys * nz;
+        double w[OCO_WT_STRIDE];
+        if (!GREY) {
+#pragma unroll
+            for (int k = 0; k < OCO_WT_STRIDE; k++) w[k] = a.wtab[iw * OCO_WT_STRIDE + k];
+        }
+        const bool canary = (sys == 0) && (a.canary != nullptr);
+        double sa = 0.0, sb = 0.0, sc = 0.0, sd = 0.0;
+
+        /* ---- forward elimination (source/auxiliary.f90:25-41, rows from :147-197) ---- */
+        oco_pt p0 = point_at<GREY>(a, col, 0, w, &smt);
+        oco_pt p1 = point_at<GREY>(a, col, 1, w, &smt);
+        double zeta = ((p0.alpha * dz2) * (1.0 - p0.albedo)) * 0.25;
+        double b = -((p0.alpha1 + dz) + zeta) * p0.alpha2;
+        double c = p0.alpha;
+        double d = -((p0.S * zeta) * p0.alpha2);
+        if (canary) { sb = b; sc = c; sd = d; }
+        bool failed = false;
+        if (b == 0.0) {
+            record_fail(a.fail, sys, 1, 1);
+            failed = true;
+        }
+        double bet = b;
+        double u = d / bet;
+        scr[0 * OCO_WAVE] = 0.0;
+        scr[1 * OCO_WAVE] = u;
+        scr[2 * OCO_WAVE] = p0.alpha1;
+        Ao[0] = p0.alpha;
+        double alpha_m = p0.alpha;
+        for (int i = 1; i < nz && !failed; i++) {
+            double cprev = c;
+            oco_pt pc = p1;
+            double av, bv, dv;
+            if (i < nz - 1) {
+                p1 = point_at<GREY>(a, col, i + 1, w, &smt);
+                double kp = 0.5 * (p1.alpha + pc.alpha);
+                double km = 0.5 * (pc.alpha + alpha_m);
+                av = pc.alpha2 / km;
+                c = pc.alpha2 / kp;
+                double ze = (dz2 * ((pc.alpha * pc.alpha) * pc.alpha)) * (1.0 - pc.albedo);
+                bv = -((av + c) + ze);
+                dv = -(pc.S * ze);
+            } else {
+                double ze = ((pc.alpha * dz2) * (1.0 - pc.albedo)) * 0.25;
+                av = pc.alpha;
+                bv = -((pc.alpha1 + dz) + ze) * pc.alpha2;
+                c = 0.0;
+                dv = -((pc.S * ze) * pc.alpha2);
+            }
+            if (canary) { sa = sa + av; sb = sb + bv; sc = sc + c; sd = sd + dv; }
+            double gam = cprev / bet;
+            bet = bv - av * gam;
+            if (bet == 0.0) {
+                record_fail(a.fail, sys, 2, i + 1);
+                failed = true;
+                break;
+            }
+            u = (dv - av * u) / bet;
+            double* s = scr + (long)i * 3 * OCO_WAVE;
+            s[0] = gam;
+            s[OCO_WAVE] = u;
+            s[2 * OCO_WAVE] = pc.alpha1;
+            Ao[i] = pc.alpha;
+            alpha_m = pc.alpha;
+        }
+        if (canary) {
+            a.canary[0] = sa; a.canary[1] = sb; a.canary[2] = sc; a.canary[3] = sd;
+        }
+        if (failed) continue;
+
+        /* ---- back-substitution (:43-45) fused with update_ghosts (:49-62),
+         *      der (:64-78) and calc_flux (:96-109) ---- */
+        const double c60 = 1.0 / 60.0;
+        /* window w0..w6 = extended U at e = j .. j+6 (e = z + 3) */
+        double Un = u;                       /* U(nz-1) */
+        double gnext = scr[(long)(nz - 1) * 3 * OCO_WAVE];
+        double a1q0 = scr[(long)(nz - 1) * 3 * OCO_WAVE + 2 * OCO_WAVE];   /* 1/alpha of the V row pending */
+        double a1q1 = 0.0, a1q2 = 0.0, a1q3 = 0.0;
+        Uo[3 + nz - 1] = Un;
+        double t[4];
+        t[0] = Un;
+        for (int k = 1; k < 4; k++) {
+            int j = nz - 1 - k;
+            const double* s = scr + (long)j * 3 * OCO_WAVE;
+            double Uj = s[OCO_WAVE] - gnext * t[k - 1];
+            gnext = s[0];
+            double a1 = s[2 * OCO_WAVE];
+            a1q3 = a1q2; a1q2 = a1q1; a1q1 = a1q0; a1q0 = a1;
+            t[k] = Uj;
+            Uo[3 + j] = Uj;
+        }
+        /* top ghosts (f(n2+i) = 2 f(n2) - f(n2-i)) */
+        double g1 = 2.0 * t[0] - t[1], g2 = 2.0 * t[0] - t[2], g3 = 2.0 * t[0] - t[3];
+        Uo[3 + nz] = g1;
+        Uo[3 + nz + 1] = g2;
+        Uo[3 + nz + 2] = g3;
+        double w0 = t[3], w1 = t[2], w2 = t[1], w3 = t[0], w4 = g1, w5 = g2, w6 = g3;
+        /* a1q3 holds 1/alpha of row nz-1 */
+        {
+            double df = c60 * (((45.0 * (w4 - w2)) - (9.0 * (w5 - w1))) + (w6 - w0));
+            Vo[nz - 1] = ((-1.0 * a1q3) * df) * dz1;
+        }
+        for (int j = nz - 5; j >= 0; j--) {
+            const double* s = scr + (long)j * 3 * OCO_WAVE;
+            double Uj = s[OCO_WAVE] - gnext * w0;
+            gnext = s[0];
+            double a1 = s[2 * OCO_WAVE];
+            a1q3 = a1q2; a1q2 = a1q1; a1q1 = a1q0; a1q0 = a1;
+            Uo[3 + j] = Uj;
+            w6 = w5; w5 = w4; w4 = w3; w3 = w2; w2 = w1; w1 = w0; w0 = Uj;
+            double df = c60 * (((45.0 * (w4 - w2)) - (9.0 * (w5 - w1))) + (w6 - w0));
+            Vo[j + 3] = ((-1.0 * a1q3) * df) * dz1;
+        }
+        /* bottom ghosts (f(n1-i) = 2 f(n1) - f(n1+i)); window now e = 3..9 */
+        double u0 = w0, G1 = 2.0 * u0 - w1, G2 = 2.0 * u0 - w2, G3 = 2.0 * u0 - w3;
+        Uo[2] = G1;
+        Uo[1] = G2;
+        Uo[0] = G3;
+        const double gh[3] = {G1, G2, G3};
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            w6 = w5; w5 = w4; w4 = w3; w3 = w2; w2 = w1; w1 = w0; w0 = gh[k];
+            a1q3 = a1q2; a1q2 = a1q1; a1q1 = a1q0;
+            double df = c60 * (((45.0 * (w4 - w2)) - (9.0 * (w5 - w1))) + (w6 - w0));
+            Vo[2 - k] = ((-1.0 * a1q3) * df) * dz1;
+        }
+    }
+}
+
+hipError_t oco_launch_cells(const double* rho, const double* T, long ncells, double fully_ionized_T,
+                            double* cells, hipStream_t stream)
+{
+    if (ncells <= 0) return hipSuccess;
+    long blocks = (ncells + 255) / 256;
+    if (blocks > 65536) blocks = 65536;
+    hipLaunchKernelGGL(cells_fp64, dim3((unsigned)blocks), dim3(256), 0, stream, rho, T, ncells,
+                       fully_ionized_T, cells);
+    return hipGetLastError();
+}
+
+int oco_sweep_max_blocks(void) { return 4096; }
+
+hipError_t oco_launch_sweep(const oco_sweep_args& a, int lgrey, int nblocks, hipStream_t stream)
+{
+    const bool uniform = (a.nw % OCO_WAVE) == 0;
+    if (lgrey) {
+        hipLaunchKernelGGL((sweep_fp64<true, false>), dim3(nblocks), dim3(OCO_WAVE), 0, stream, a);
+    } else if (uniform) {
+        hipLaunchKernelGGL((sweep_fp64<false, true>), dim3(nblocks), dim3(OCO_WAVE), 0, stream, a);
+    } else {
+        hipLaunchKernelGGL((sweep_fp64<false, false>), dim3(nblocks), dim3(OCO_WAVE), 0, stream, a);
+    }
+    return hipGetLastError();
+}

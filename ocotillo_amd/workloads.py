@@ -1,0 +1,64 @@
+"""Seeded synthetic workloads (the benchmark configurations and test inputs).
+
+All generators are deterministic (numpy PCG64, base seed 20261203) and
+return C-ordered arrays in the solver's layout: rho, T as [ix, iy, z]
+(Fortran (nz, nyloc, nxloc)).  The z grid is the reference's calc_grid
+(source/disk.f90:42-62): nz points uniformly on [z0, z1].
+"""
+import numpy as np
+
+SEED = 20261203
+AU = 1.496e14              # H unit of the reference's disk samples (cm)
+SIGMA_T = 2.54e14          # Gaussian T fall-off width of samples/NongreyAGN (cm)
+T_FLOOR = 2000.0
+# rho floor of the reference's non-grey samples (samples/NongreyAGN/input.in).
+# A floor of 1e-24 g/cm^3 at 2000 K gives alpha ~ 1e-31 /cm, dz*alpha < 2^-52,
+# and the reference's Thomas pivot becomes exactly zero (it stops).
+RHO_FLOOR = 1e-15
+
+
+def z_grid(nz, z0=-5e15, z1=5e15):
+    dz = (z1 - z0) / (nz - 1)
+    z = z0 + np.arange(nz) * dz
+    return z, dz
+
+
+def log_waves(nw, w0=100.0, w1=1e5):
+    """nw wavelengths log-spaced on [w0, w1] Angstrom (100 A - 10 um by default)."""
+    if nw == 1:
+        return np.array([w0])
+    return np.exp(np.linspace(np.log(w0), np.log(w1), nw))
+
+
+def disk_columns(nx, ny, nz, seed=SEED + 1, rho_floor=RHO_FLOOR, z0=-5e15, z1=5e15, rng=None):
+    """Non-grey disk columns: per column T_mid ~ U(2e4, 1e5) K with a Gaussian
+    fall-off in z to a 2000 K floor, rho0 ~ logU(1e-11, 1e-8) g/cm^3,
+    H ~ U(0.5, 2) * 1.496e14 cm, rho = max(rho0 exp(-z^2/2H^2), floor).
+    Returns rho, T ([nx, ny, nz]) and dz."""
+    rng = np.random.default_rng(seed) if rng is None else rng
+    z, dz = z_grid(nz, z0, z1)
+    ncol = nx * ny
+    Tm = rng.uniform(2e4, 1e5, ncol)[:, None]
+    rho0 = 10.0 ** rng.uniform(-11.0, -8.0, ncol)[:, None]
+    H = rng.uniform(0.5, 2.0, ncol)[:, None] * AU
+    T = np.maximum(Tm * np.exp(-z ** 2 / (2 * SIGMA_T ** 2)), T_FLOOR)
+    rho = np.maximum(rho0 * np.exp(-z ** 2 / (2 * H ** 2)), rho_floor)
+    return rho.reshape(nx, ny, nz), T.reshape(nx, ny, nz), dz
+
+
+def config1():
+    """Grey isothermal column (samples/GreyIsothermal): nz=10 on
+    [-5e15, 5e15], rho = max(1e-9 exp(-z^2/2H^2), 1e-24), H = 1 AU, T = 2000 K
+    (midTemp = isoTemp = 1 K is below the 2000 K floor), one wavelength,
+    sigma_grey = 0.4."""
+    z, dz = z_grid(10)
+    rho = np.maximum(1e-9 * np.exp(-(0.5 * z * z) / (AU * AU)), 1e-24)
+    T = np.full(10, 2000.0)
+    return rho.reshape(1, 1, 10), T.reshape(1, 1, 10), dz, np.array([3000.0])
+
+
+def config2(nx=64, ny=64, nz=256, nw=128):
+    """Non-grey drop-in block: 64x64 columns, nz=256, nw=128 log-spaced
+    100 A - 10 um, disk_columns profiles.  Returns rho, T, dz, waves."""
+    rho, T, dz = disk_columns(nx, ny, nz, seed=SEED + 2)
+    return rho, T, dz, log_waves(nw)

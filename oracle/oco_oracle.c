@@ -28,6 +28,30 @@
 #define OO_POW(x, y)   oco_pow((x), (y))
 #define OO_POW10(y)    oco_exp10(y)
 #define OO_SQRT(x)     oco_sqrt(x)
+#elif defined(OO_PERTURB)
+/* Conditioning probe: every libm result moved by -1, 0 or +1 ulp, chosen by
+ * a hash of (seed, function, argument).  Running the block for a few seeds
+ * measures how far the reference's own answer moves when its elementary
+ * functions change in the last bit (any two faithful libms differ so). */
+static unsigned long long oo_seed = 0;
+void oo_set_perturbation(unsigned long long seed) { oo_seed = seed; }
+static double oo_pert(double y, double x, unsigned fn)
+{
+    if (!oo_seed || !(y == y) || y == 0.0 || y > 1.7976931348623157e308) return y;
+    unsigned long long h;
+    memcpy(&h, &x, 8);
+    h ^= oo_seed * 0x9E3779B97F4A7C15ULL + fn * 0xBF58476D1CE4E5B9ULL;
+    h ^= h >> 31; h *= 0x94D049BB133111EBULL; h ^= h >> 29;
+    unsigned k = (unsigned)(h % 3u);
+    if (k == 0) return y;
+    return nextafter(y, k == 1 ? 2.0 * y : 0.0);
+}
+#define OO_EXP(x)      oo_pert(exp(x), (x), 1)
+#define OO_LOG(x)      oo_pert(log(x), (x), 2)
+#define OO_LOG10(x)    oo_pert(log10(x), (x), 3)
+#define OO_POW(x, y)   oo_pert(pow((x), (y)), (x) * 3.0 + (y), 4)
+#define OO_POW10(y)    oo_pert(pow(10.0, (y)), (y), 5)
+#define OO_SQRT(x)     sqrt(x)
 #else
 #define OO_EXP(x)      exp(x)
 #define OO_LOG(x)      log(x)

@@ -50,7 +50,7 @@ def build():
 
 
 def lib(portable=False):
-    name = "liboco_oracle_pm.so" if portable else "liboco_oracle.so"
+    name = {False: "liboco_oracle.so", True: "liboco_oracle_pm.so", "env": "liboco_oracle_env.so"}[portable]
     if name not in _libs:
         path = os.path.join(LIBDIR, name)
         if not os.path.exists(path):
@@ -71,6 +71,8 @@ def lib(portable=False):
         L.oo_calc_wavelength.argtypes = [C.c_double, C.c_double, C.c_int, DP]
         L.oo_calc_grid.argtypes = [C.c_int, C.c_double, C.c_double, DP, C.POINTER(C.c_double)]
         L.oo_calc_density.argtypes = [C.c_int, DP, C.c_double, C.c_double, C.c_double, DP]
+        if portable == "env":
+            L.oo_set_perturbation.argtypes = [C.c_ulonglong]
         _libs[name] = L
     return _libs[name]
 
@@ -142,3 +144,23 @@ def calc_wavelength(w1, w0, nw, portable=False):
     wa = np.zeros(nw)
     lib(portable).oo_calc_wavelength(w1, w0, nw, wa)
     return wa
+
+
+def envelope(rho3d, T3d, dz, waves, seeds=(1, 2, 3), **kw):
+    """Conditioning envelope of the reference's answer: the largest change of
+    U, V, alpha over runs whose libm results are moved by +-1 ulp.
+    Returns (U0, V0, A0, EU, EV, EA) with *0 the unperturbed glibc result."""
+    o = Oracle(waves, portable="env")
+    o.L.oo_set_perturbation(0)
+    U0, V0, A0, d0 = o.solve_block(rho3d, T3d, dz, **kw)
+    EU = np.zeros_like(U0)
+    EV = np.zeros_like(V0)
+    EA = np.zeros_like(A0)
+    for s in seeds:
+        o.L.oo_set_perturbation(s)
+        U, V, A, d = o.solve_block(rho3d, T3d, dz, **kw)
+        np.maximum(EU, np.abs(U - U0), out=EU)
+        np.maximum(EV, np.abs(V - V0), out=EV)
+        np.maximum(EA, np.abs(A - A0), out=EA)
+    o.L.oo_set_perturbation(0)
+    return U0, V0, A0, EU, EV, EA
