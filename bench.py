@@ -47,6 +47,9 @@ def main():
     ap.add_argument("--ny", type=int, default=64)
     ap.add_argument("--nz", type=int, default=256)
     ap.add_argument("--nw", type=int, default=128)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a plain 1 GiB device-to-device copy (3 times): the reference point "
+                         "for scaling the FETCH_SIZE / WRITE_SIZE counters")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU; the path shards by columns/wavelengths with no "
@@ -54,6 +57,14 @@ def main():
         sys.exit(2)
 
     import torch
+    if args.copy_calibration:
+        src = torch.ones(1 << 27, dtype=torch.float64, device="cuda:0")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_each_way": src.numel() * 8, "copies": 3}))
+        return
     import ocotillo_amd as oco
     from ocotillo_amd.workloads import config2
 

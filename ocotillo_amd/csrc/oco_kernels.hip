@@ -48,8 +48,28 @@ __global__ __launch_bounds__(256) void cells_fp64(const double* __restrict__ rho
     }
 }
 
+/* Cell records are read-only during the sweep.  When a wavefront's lanes
+ * share one column the record address is wave-uniform; reading it through
+ * the constant address space turns the loads into scalar loads (one fetch
+ * per wavefront instead of 64 lanes of vector loads). */
+typedef const __attribute__((address_space(4))) double* oco_cptr;
+
+template <bool UNIFORM>
+__device__ static inline void load_record(const oco_sweep_args& a, long col, int z, double* r)
+{
+    if (UNIFORM) {
+        oco_cptr p = (oco_cptr)(a.cells + (col * a.nz + z) * OCO_CELL_STRIDE);
+#pragma unroll
+        for (int k = 0; k < CR_NFIELD; k++) r[k] = p[k];
+    } else {
+        const double* p = a.cells + (col * a.nz + z) * OCO_CELL_STRIDE;
+#pragma unroll
+        for (int k = 0; k < CR_NFIELD; k++) r[k] = p[k];
+    }
+}
+
 /* opacity, albedo, source function of cell z for one lane */
-template <bool GREY>
+template <bool GREY, bool UNIFORM>
 __device__ static inline oco_pt point_at(const oco_sweep_args& a, long col, int z, const double* w,
                                          const oco_mtab* M)
 {
@@ -57,7 +77,9 @@ __device__ static inline oco_pt point_at(const oco_sweep_args& a, long col, int 
         long c = col * a.nz + z;
         return oco_grey_point(a.rho[c], a.T[c], a.sigma_grey, M);
     } else {
-        return oco_point(a.cells + (col * a.nz + z) * OCO_CELL_STRIDE, w, M);
+        double r[CR_NFIELD];
+        load_record<UNIFORM>(a, col, z, r);
+        return oco_point(r, w, M);
     }
 }
 
@@ -108,8 +130,8 @@ __global__ __launch_bounds__(OCO_WAVE) void sweep_fp64(const oco_sweep_args a)
         double sa = 0.0, sb = 0.0, sc = 0.0, sd = 0.0;
 
         /* ---- forward elimination (source/auxiliary.f90:25-41, rows from :147-197) ---- */
-        oco_pt p0 = point_at<GREY>(a, col, 0, w, &smt);
-        oco_pt p1 = point_at<GREY>(a, col, 1, w, &smt);
+        oco_pt p0 = point_at<GREY, UNIFORM>(a, col, 0, w, &smt);
+        oco_pt p1 = point_at<GREY, UNIFORM>(a, col, 1, w, &smt);
         double zeta = ((p0.alpha * dz2) * (1.0 - p0.albedo)) * 0.25;
         double b = -((p0.alpha1 + dz) + zeta) * p0.alpha2;
         double c = p0.alpha;
@@ -132,7 +154,7 @@ __global__ __launch_bounds__(OCO_WAVE) void sweep_fp64(const oco_sweep_args a)
             oco_pt pc = p1;
             double av, bv, dv;
             if (i < nz - 1) {
-                p1 = point_at<GREY>(a, col, i + 1, w, &smt);
+                p1 = point_at<GREY, UNIFORM>(a, col, i + 1, w, &smt);
                 double kp = 0.5 * (p1.alpha + pc.alpha);
                 double km = 0.5 * (pc.alpha + alpha_m);
                 av = pc.alpha2 / km;

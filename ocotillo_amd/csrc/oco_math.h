@@ -12,7 +12,11 @@
  * that a CPU build using them reproduces the GPU's results exactly.
  * Accuracy is about 0.5 ulp plus 2^-60 relative (exp, log, log10, exp10) and
  * about 2^-62 relative (pow); NaN/inf/zero inputs follow C99 Annex F for the
- * cases the solver can produce.
+ * cases the solver can produce.  Results are nearly always the correctly
+ * rounded ones, as glibc's are, which keeps the two libms in agreement on
+ * the inputs where the reference's formulas amplify a one-ulp difference.
+ * The common path has no branches (range and special cases are clamps and
+ * selects), so a wavefront never diverges here.
  *
  * Methods (standard, see e.g. Tang, ACM TOMS 15(2) 1989 and Muller,
  * "Elementary Functions", ch. 6):
@@ -21,8 +25,8 @@
  *          polynomial.  Two-part arguments (hi + lo) are accepted.
  *   log:   x = 2^k z; r = z*invc - 1 is exact (invc has <= 9 bits);
  *          log x = k ln2 - log(invc) + log1p(r), carried as a double-double
- *          (about 2^-68 absolute) so pow and log10 can reuse it.  Near 1 a
- *          direct log1p series is used.
+ *          (about 2^-68 absolute) so pow and log10 can reuse it.  The cells
+ *          touching 1 have invc = 1, so no separate path is needed near 1.
  *   pow:   exp(y log x) with y*log x formed exactly by fma.
  *   exp10: exp(y ln10) with y*ln10 formed exactly by fma.
  *   sqrt:  IEEE square root (correctly rounded on both targets).
@@ -83,85 +87,52 @@ OCO_HD static inline double oco_asf64(uint64_t u)
 
 OCO_HD static inline double oco_fma(double a, double b, double c) { return __builtin_fma(a, b, c); }
 
-/* exp(hi + lo), |lo| <= ulp(hi). */
+#define OCO_INF (0x1p1023 * 2.0)
+
+OCO_HD static inline double oco_sel(int c, double a, double b) { return c ? a : b; }
+
+/* exp(hi + lo), |lo| <= ulp(hi).  No branch on the common path: the
+ * argument is clamped, the scale exponent is clamped to the normal range and
+ * the remainder applied by ldexp (exact, or one rounding into the subnormal
+ * range), and the special results are selected at the end. */
 OCO_HD static inline double oco_exp_dd(double hi, double lo, const oco_mtab* T)
 {
-    if (!(hi <= OCO_EXP_XMAX)) {
-        if (hi != hi) return hi + hi;
-        return 0x1p1023 * 2.0;                 /* +inf */
-    }
-    if (hi < OCO_EXP_XMIN) return 0.0;
-    double kd = hi * OCO_INVLN2N + OCO_SHIFT;
+    double hc = hi > OCO_EXP_XMAX ? OCO_EXP_XMAX : hi;
+    hc = hc < OCO_EXP_XMIN ? OCO_EXP_XMIN : hc;
+    hc = (hi != hi) ? 0.0 : hc;
+    double kd = hc * OCO_INVLN2N + OCO_SHIFT;
     kd = kd - OCO_SHIFT;
     int64_t ki = (int64_t)kd;
-    double r = (hi - kd * OCO_LN2N_HI) - kd * OCO_LN2N_LO;
+    double r = (hc - kd * OCO_LN2N_HI) - kd * OCO_LN2N_LO;
     r = r + lo;
     int idx = (int)(ki & 127);
-    int64_t e = ki >> 7;
+    int e = (int)(ki >> 7);
     double r2 = r * r;
     double p = r + r2 * (0.5 + r * 0x1.5555555555555p-3)
                  + (r2 * r2) * (0x1.5555555555555p-5 + r * 0x1.1111111111111p-7 + r2 * 0x1.6c16c16c16c17p-10);
     double tmp = T->exp_tlo[idx] + p;
-    uint64_t tb = oco_asu64(T->exp_thi[idx]);
-    if (e >= -1000 && e <= 1023) {
-        double scale = oco_asf64(tb + ((uint64_t)e << 52));
-        return scale + scale * tmp;
-    }
-    if (e > 1023) {
-        double scale = oco_asf64(tb + ((uint64_t)(e - 1) << 52));
-        return (scale + scale * tmp) * 2.0;
-    }
-    /* tiny results: form the sum with 2^1022 headroom so scale*tmp keeps
-     * full precision, then scale down (one rounding if subnormal) */
-    double scale = oco_asf64(tb + ((uint64_t)(e + 1022) << 52));
-    return (scale + scale * tmp) * 0x1p-1022;
+    int ec = e < -1000 ? -1000 : (e > 1023 ? 1023 : e);
+    double scale = oco_asf64(oco_asu64(T->exp_thi[idx]) + ((uint64_t)(int64_t)ec << 52));
+    double y = __builtin_ldexp(scale + scale * tmp, e - ec);
+    y = oco_sel(hi > OCO_EXP_XMAX, OCO_INF, y);
+    y = oco_sel(hi < OCO_EXP_XMIN, 0.0, y);
+    return oco_sel(hi != hi, hi + hi, y);
 }
 
 OCO_HD static inline double oco_exp_t(double x, const oco_mtab* T) { return oco_exp_dd(x, 0.0, T); }
 
-/* log(x) as a double-double *hi + *lo for finite x > 0. */
+/* log(x) as a double-double *hi + *lo for finite x > 0 (other inputs give
+ * values that the callers replace).  The two table cells touching 1 have
+ * invc = 1 and log c = 0, so near 1 the result is log1p(x - 1) with x - 1
+ * exact. */
 OCO_HD static inline void oco_log_dd(double x, const oco_mtab* T, double* hi, double* lo)
 {
-    if (x > 0x1.f8p-1 && x < 0x1.04p+0) {
-        /* |x - 1| < 2^-5 (the inner cell of a table-based split would cancel) */
-        double r = x - 1.0;                    /* exact */
-        double r2 = r * r;
-        double r2e = oco_fma(r, r, -r2);
-        /* log1p(r) = r - r^2/2 + r^3 q(r), q = sum_{n=3..16} (-1)^(n+1) r^(n-3) / n */
-        double q = -(1.0 / 16);
-        q = (1.0 / 15) + r * q;
-        q = -(1.0 / 14) + r * q;
-        q = (1.0 / 13) + r * q;
-        q = -(1.0 / 12) + r * q;
-        q = (1.0 / 11) + r * q;
-        q = -(1.0 / 10) + r * q;
-        q = (1.0 / 9) + r * q;
-        q = -(1.0 / 8) + r * q;
-        q = (1.0 / 7) + r * q;
-        q = -(1.0 / 6) + r * q;
-        q = (1.0 / 5) + r * q;
-        q = -(1.0 / 4) + r * q;
-        q = (1.0 / 3) + r * q;
-        double tail = (r * r2) * q - 0.5 * r2e;
-        double mh = -0.5 * r2;                 /* exact */
-        double h = r + mh;
-        double bb = h - r;
-        double e = (r - (h - bb)) + (mh - bb);
-        double l = e + tail;
-        double s = h + l;
-        *lo = l - (s - h);
-        *hi = s;
-        return;
-    }
     uint64_t ix = oco_asu64(x);
-    int64_t ksub = 0;
-    if (ix < 0x0010000000000000ULL) {          /* subnormal */
-        ix = oco_asu64(x * 0x1p52);
-        ksub = -52;
-    }
+    int sub = ix < 0x0010000000000000ULL;    /* subnormal: scale by 2^52 */
+    ix = sub ? oco_asu64(x * 0x1p52) : ix;
     uint64_t tmp = ix - OCO_LOG_OFF;
     int i = (int)((tmp >> 45) & 127);
-    int64_t k = ((int64_t)tmp >> 52) + ksub;
+    int k = (int)((int64_t)tmp >> 52) - (sub ? 52 : 0);
     double z = oco_asf64(ix - (tmp & (0xfffULL << 52)));
     double r = oco_fma(z, T->log_invc[i], -1.0);  /* exact */
     double kd = (double)k;
@@ -175,14 +146,16 @@ OCO_HD static inline void oco_log_dd(double x, const oco_mtab* T, double* hi, do
     double e2 = (s - (h - bb)) + (r - bb);
     double r2 = r * r;
     double r2e = oco_fma(r, r, -r2);
-    double q = -0x1.999999999999ap-4;          /* -1/10 .. 1/3 */
-    q = 0x1.c71c71c71c71cp-4 + r * q;
-    q = -0x1.0000000000000p-3 + r * q;
-    q = 0x1.2492492492492p-3 + r * q;
-    q = -0x1.5555555555555p-3 + r * q;
-    q = 0x1.999999999999ap-3 + r * q;
-    q = -0x1.0000000000000p-2 + r * q;
-    q = 0x1.5555555555555p-2 + r * q;
+    /* log1p(r) = r - r^2/2 + r^3 q(r), q = sum_{n=3..11} (-1)^(n+1) r^(n-3) / n */
+    double q = (1.0 / 11);
+    q = -(1.0 / 10) + r * q;
+    q = (1.0 / 9) + r * q;
+    q = -(1.0 / 8) + r * q;
+    q = (1.0 / 7) + r * q;
+    q = -(1.0 / 6) + r * q;
+    q = (1.0 / 5) + r * q;
+    q = -(1.0 / 4) + r * q;
+    q = (1.0 / 3) + r * q;
     double p = (r * r2) * q;
     double l = -0.5 * r2 + ((p - 0.5 * r2e) + (((kd * OCO_LN2_LO + T->log_llo[i]) + e1) + e2));
     double hs = h + l;
@@ -190,27 +163,29 @@ OCO_HD static inline void oco_log_dd(double x, const oco_mtab* T, double* hi, do
     *hi = hs;
 }
 
+/* result for x outside (0, inf): log(0) = -inf, log(inf) = inf, else NaN */
+OCO_HD static inline double oco_log_special(double x)
+{
+    return x == 0.0 ? -OCO_INF : (x == OCO_INF ? x : (x - x) / (x - x));
+}
+
 OCO_HD static inline double oco_log_t(double x, const oco_mtab* T)
 {
-    if (x == 1.0) return 0.0;
-    if (!(x > 0.0) || x == 0x1p1023 * 2.0) {
-        if (x == 0.0) return -(0x1p1023 * 2.0);
-        return (x > 0.0) ? x : (x - x) / (x - x);   /* +inf, or NaN */
-    }
     double h, l;
     oco_log_dd(x, T, &h, &l);
-    return h;
+    h = oco_sel(x == 1.0, 0.0, h);
+    return oco_sel(x > 0.0 && x < OCO_INF, h, oco_log_special(x));
 }
 
 OCO_HD static inline double oco_log10_t(double x, const oco_mtab* T)
 {
-    if (x == 1.0) return 0.0;
-    if (!(x > 0.0) || x == 0x1p1023 * 2.0) return oco_log_t(x, T);
     double h, l;
     oco_log_dd(x, T, &h, &l);
     double p = h * OCO_IVLN10_HI;
     double pe = oco_fma(h, OCO_IVLN10_HI, -p);
-    return p + (pe + (h * OCO_IVLN10_LO + l * OCO_IVLN10_HI));
+    double y = p + (pe + (h * OCO_IVLN10_LO + l * OCO_IVLN10_HI));
+    y = oco_sel(x == 1.0, 0.0, y);
+    return oco_sel(x > 0.0 && x < OCO_INF, y, oco_log_special(x));
 }
 
 /* 10**y */
@@ -224,16 +199,16 @@ OCO_HD static inline double oco_exp10_t(double y, const oco_mtab* T)
 /* x**y for x >= 0 (the solver never raises a negative base). */
 OCO_HD static inline double oco_pow_t(double x, double y, const oco_mtab* T)
 {
-    if (y == 0.0 || x == 1.0) return 1.0;
-    if (x != x || y != y) return x + y;
-    if (x == 0.0) return (y > 0.0) ? 0.0 : 0x1p1023 * 2.0;
-    if (x < 0.0) return (x - x) / (x - x);
-    if (x == 0x1p1023 * 2.0) return (y > 0.0) ? x : 0.0;
     double h, l;
     oco_log_dd(x, T, &h, &l);
     double yh = y * h;
     double yl = oco_fma(y, h, -yh) + y * l;
-    return oco_exp_dd(yh, yl, T);
+    double r = oco_exp_dd(yh, yl, T);
+    int finite_pos = x > 0.0 && x < OCO_INF;
+    r = oco_sel(finite_pos, r, oco_sel(x == 0.0, oco_sel(y > 0.0, 0.0, OCO_INF),
+                                       oco_sel(x == OCO_INF, oco_sel(y > 0.0, OCO_INF, 0.0), (x - x) / (x - x))));
+    r = oco_sel(x != x || y != y, x + y, r);
+    return oco_sel(y == 0.0 || x == 1.0, 1.0, r);
 }
 
 OCO_HD static inline double oco_sqrt(double x) { return __builtin_sqrt(x); }
