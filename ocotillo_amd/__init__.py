@@ -50,11 +50,12 @@ _DP = C.POINTER(C.c_double)
 
 
 def load_library(path=None):
-    """Load libocotillo.so (raises if it has not been built)."""
+    """Load libocotillo.so (raises if it has not been built).  OCO_LIBRARY in
+    the environment selects another build of the same library."""
     global _lib
     if _lib is not None and path is None:
         return _lib
-    p = path or LIB_PATH
+    p = path or os.environ.get("OCO_LIBRARY") or LIB_PATH
     if not os.path.exists(p):
         raise OcoError(E_STATE, f"HIP library {p} is not built (run python -m ocotillo_amd.build)")
     L = C.CDLL(p)

@@ -30,22 +30,24 @@ def _run(cmd, cwd=None):
     subprocess.check_call(cmd, cwd=cwd)
 
 
-def build_lib():
-    os.makedirs(BUILD, exist_ok=True)
+def build_lib(out=LIB, defines=(), build_dir=BUILD):
+    """Compile and link the library.  `defines` are extra -D tuning macros
+    (e.g. OCO_SWEEP_WAVES_PER_EU=4) for the kernels."""
+    os.makedirs(build_dir, exist_ok=True)
     objs = []
     for src, device in (("oco_kernels.hip", True), ("oco_api.cpp", False)):
-        obj = os.path.join(BUILD, os.path.splitext(src)[0] + ".o")
+        obj = os.path.join(build_dir, os.path.splitext(src)[0] + ".o")
         if device:
-            cmd = [HIPCC] + COMMON + DEVICE + ["-c", os.path.join(CSRC, src), "-o", obj]
+            cmd = [HIPCC] + COMMON + DEVICE + ["-D" + d for d in defines] + ["-c", os.path.join(CSRC, src), "-o", obj]
         else:   # host-only C++ (the C ABI): no device pass
             cmd = [CXX] + COMMON + ["-D__HIP_PLATFORM_AMD__", "-I" + os.path.join(ROCM, "include"),
                                     "-c", os.path.join(CSRC, src), "-o", obj]
         _run(cmd)
         objs.append(obj)
-    tmp = LIB + ".tmp"
+    tmp = out + ".tmp"
     _run([HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", tmp] + objs)
-    os.replace(tmp, LIB)
-    return LIB
+    os.replace(tmp, out)
+    return out
 
 
 def build_fortran():

@@ -254,7 +254,7 @@ int oco_solve_block_device(oco_ctx* c, int nz, int nyloc, int nxloc, double dz, 
     if (!c->lgrey && (st = ensure(c, c->cells, (size_t)ncells * OCO_CELL_STRIDE * sizeof(double)))) return st;
     if ((st = ensure(c, c->scratch, (size_t)nblocks * nz * 3 * 64 * sizeof(double)))) return st;
     if ((st = ensure(c, c->fail, sizeof(unsigned long long)))) return st;
-    if ((st = ensure(c, c->canary, 4 * sizeof(double)))) return st;
+    if ((st = ensure(c, c->canary, (4 + 5 * (size_t)nz) * sizeof(double)))) return st;
 
     const unsigned long long none = OCO_FAIL_NONE;
     HIPCHK(hipMemcpyAsync(c->fail.p, &none, sizeof(none), hipMemcpyHostToDevice, c->stream));

@@ -21,7 +21,7 @@ struct oco_sweep_args {
     double* A;           /* (nz, ncol, nw) */
     double* scratch;     /* per-block slots of nz*3*64 doubles */
     unsigned long long* fail;  /* first zero pivot key (atomicMin) */
-    double* canary;      /* sum(a), sum(b), sum(c), sum(d) of system 0 */
+    double* canary;      /* 4 + 5*nz doubles: sum(a..d) of system 0, then its point values */
 };
 
 /* Launchers (enqueue on `stream`, no synchronisation). */

@@ -68,21 +68,6 @@ __device__ static inline void load_record(const oco_sweep_args& a, long col, int
     }
 }
 
-/* opacity, albedo, source function of cell z for one lane */
-template <bool GREY, bool UNIFORM>
-__device__ static inline oco_pt point_at(const oco_sweep_args& a, long col, int z, const double* w,
-                                         const oco_mtab* M)
-{
-    if (GREY) {
-        long c = col * a.nz + z;
-        return oco_grey_point(a.rho[c], a.T[c], a.sigma_grey, M);
-    } else {
-        double r[CR_NFIELD];
-        load_record<UNIFORM>(a, col, z, r);
-        return oco_point(r, w, M);
-    }
-}
-
 __device__ static inline void record_fail(unsigned long long* fail, long sys, int stage, int row)
 {
     unsigned long long key = ((unsigned long long)sys << 32) | ((unsigned long long)stage << 28) |
@@ -90,8 +75,48 @@ __device__ static inline void record_fail(unsigned long long* fail, long sys, in
     atomicMin(fail, key);
 }
 
+/* Row coefficients (source/auxiliary.f90:147-197) -- interior row from the
+ * cell and its neighbours' alpha, boundary rows from the cell alone. */
+__device__ static inline void interior_row(double am, const oco_pt& pc, double ap, double dz2, double& av,
+                                           double& bv, double& cv, double& dv)
+{
+    double kp = 0.5 * (ap + pc.alpha);
+    double km = 0.5 * (pc.alpha + am);
+    av = pc.alpha2 / km;
+    cv = pc.alpha2 / kp;
+    double ze = (dz2 * ((pc.alpha * pc.alpha) * pc.alpha)) * (1.0 - pc.albedo);
+    bv = -((av + cv) + ze);
+    dv = -(pc.S * ze);
+}
+
+__device__ static inline void boundary_row(const oco_pt& pc, double dz, double dz2, double& bv, double& dv)
+{
+    double ze = ((pc.alpha * dz2) * (1.0 - pc.albedo)) * 0.25;
+    bv = -((pc.alpha1 + dz) + ze) * pc.alpha2;
+    dv = -((pc.S * ze) * pc.alpha2);
+}
+
 template <bool GREY, bool UNIFORM>
-__global__ __launch_bounds__(OCO_WAVE) void sweep_fp64(const oco_sweep_args a)
+__device__ static inline oco_pt point_rec(const oco_sweep_args& a, long col, int z, const double* rec,
+                                          const double* w, const oco_mtab* M)
+{
+    if (GREY) {
+        long c = col * a.nz + z;
+        return oco_grey_point(a.rho[c], a.T[c], a.sigma_grey, M);
+    }
+    return oco_point(rec, w, M);
+}
+
+/* Tuning knob: OCO_SWEEP_WAVES_PER_EU=n asks the compiler to fit n waves per
+ * SIMD (register budget 512/n VGPRs). */
+#ifdef OCO_SWEEP_WAVES_PER_EU
+#define OCO_SWEEP_ATTR __attribute__((amdgpu_waves_per_eu(OCO_SWEEP_WAVES_PER_EU)))
+#else
+#define OCO_SWEEP_ATTR
+#endif
+
+template <bool GREY, bool UNIFORM>
+__global__ __launch_bounds__(OCO_WAVE) OCO_SWEEP_ATTR void sweep_fp64(const oco_sweep_args a)
 {
     __shared__ oco_mtab smt;
     load_mtab(&smt);
@@ -126,69 +151,93 @@ __global__ __launch_bounds__(OCO_WAVE) void sweep_fp64(const oco_sweep_args a)
 #pragma unroll
             for (int k = 0; k < OCO_WT_STRIDE; k++) w[k] = a.wtab[iw * OCO_WT_STRIDE + k];
         }
-        const bool canary = (sys == 0) && (a.canary != nullptr);
-        double sa = 0.0, sb = 0.0, sc = 0.0, sd = 0.0;
 
-        /* ---- forward elimination (source/auxiliary.f90:25-41, rows from :147-197) ---- */
-        oco_pt p0 = point_at<GREY, UNIFORM>(a, col, 0, w, &smt);
-        oco_pt p1 = point_at<GREY, UNIFORM>(a, col, 1, w, &smt);
-        double zeta = ((p0.alpha * dz2) * (1.0 - p0.albedo)) * 0.25;
-        double b = -((p0.alpha1 + dz) + zeta) * p0.alpha2;
-        double c = p0.alpha;
-        double d = -((p0.S * zeta) * p0.alpha2);
-        if (canary) { sb = b; sc = c; sd = d; }
-        bool failed = false;
-        if (b == 0.0) {
-            record_fail(a.fail, sys, 1, 1);
-            failed = true;
-        }
-        double bet = b;
-        double u = d / bet;
+        /* ---- forward elimination (source/auxiliary.f90:25-41) ----
+         * Two-deep pipeline: iteration j eliminates row j (which needs the
+         * points j-1, j, j+1, all computed earlier) and evaluates point j+2,
+         * whose work is independent of the elimination chain; the record of
+         * cell j+3 is fetched one iteration ahead. */
+        double rec[CR_NFIELD];
+        if (!GREY) load_record<UNIFORM>(a, col, 0, rec);
+        oco_pt pB = point_rec<GREY, UNIFORM>(a, col, 0, rec, w, &smt);
+        if (!GREY) load_record<UNIFORM>(a, col, 1, rec);
+        oco_pt pC = point_rec<GREY, UNIFORM>(a, col, 1, rec, w, &smt);
+        if (!GREY) load_record<UNIFORM>(a, col, 2, rec);
+
+        double bv, dv, av, cv;
+        boundary_row(pB, dz, dz2, bv, dv);      /* row 0: a = 0, c = alpha */
+        cv = pB.alpha;
+        int fail_stage = (bv == 0.0) ? 1 : 0;
+        int fail_row = 1;
+        double bet = bv;
+        double u = dv / bet;
         scr[0 * OCO_WAVE] = 0.0;
         scr[1 * OCO_WAVE] = u;
-        scr[2 * OCO_WAVE] = p0.alpha1;
-        Ao[0] = p0.alpha;
-        double alpha_m = p0.alpha;
-        for (int i = 1; i < nz && !failed; i++) {
-            double cprev = c;
-            oco_pt pc = p1;
-            double av, bv, dv;
-            if (i < nz - 1) {
-                p1 = point_at<GREY, UNIFORM>(a, col, i + 1, w, &smt);
-                double kp = 0.5 * (p1.alpha + pc.alpha);
-                double km = 0.5 * (pc.alpha + alpha_m);
-                av = pc.alpha2 / km;
-                c = pc.alpha2 / kp;
-                double ze = (dz2 * ((pc.alpha * pc.alpha) * pc.alpha)) * (1.0 - pc.albedo);
-                bv = -((av + c) + ze);
-                dv = -(pc.S * ze);
-            } else {
-                double ze = ((pc.alpha * dz2) * (1.0 - pc.albedo)) * 0.25;
-                av = pc.alpha;
-                bv = -((pc.alpha1 + dz) + ze) * pc.alpha2;
-                c = 0.0;
-                dv = -((pc.S * ze) * pc.alpha2);
-            }
-            if (canary) { sa = sa + av; sb = sb + bv; sc = sc + c; sd = sd + dv; }
+        scr[2 * OCO_WAVE] = pB.alpha1;
+        Ao[0] = pB.alpha;
+        double aA = pB.alpha;   /* alpha of row j-1 */
+        pB = pC;                /* row j = 1 */
+        pC = point_rec<GREY, UNIFORM>(a, col, 2, rec, w, &smt);
+
+        for (int j = 1; j <= nz - 3; j++) {
+            double nrec[CR_NFIELD];
+            if (!GREY) load_record<UNIFORM>(a, col, j + 2, nrec);
+            /* row j and its elimination step */
+            double cprev = cv;
+            interior_row(aA, pB, pC.alpha, dz2, av, bv, cv, dv);
             double gam = cprev / bet;
             bet = bv - av * gam;
-            if (bet == 0.0) {
-                record_fail(a.fail, sys, 2, i + 1);
-                failed = true;
-                break;
-            }
+            int first = (bet == 0.0) && (fail_stage == 0);
+            fail_row = first ? j + 1 : fail_row;
+            fail_stage = first ? 2 : fail_stage;
             u = (dv - av * u) / bet;
-            double* s = scr + (long)i * 3 * OCO_WAVE;
-            s[0] = gam;
-            s[OCO_WAVE] = u;
-            s[2 * OCO_WAVE] = pc.alpha1;
-            Ao[i] = pc.alpha;
-            alpha_m = pc.alpha;
+            double* sp = scr + (long)j * 3 * OCO_WAVE;
+            sp[0] = gam;
+            sp[OCO_WAVE] = u;
+            sp[2 * OCO_WAVE] = pB.alpha1;
+            Ao[j] = pB.alpha;
+            /* point j+2 (independent of the chain above) */
+            oco_pt pD = point_rec<GREY, UNIFORM>(a, col, j + 2, nrec, w, &smt);
+            aA = pB.alpha;
+            pB = pC;
+            pC = pD;
         }
-        if (canary) {
-            a.canary[0] = sa; a.canary[1] = sb; a.canary[2] = sc; a.canary[3] = sd;
+        {   /* row nz-2 (interior, no new point) and row nz-1 (boundary) */
+            int j = nz - 2;
+            double cprev = cv;
+            interior_row(aA, pB, pC.alpha, dz2, av, bv, cv, dv);
+            double gam = cprev / bet;
+            bet = bv - av * gam;
+            int first = (bet == 0.0) && (fail_stage == 0);
+            fail_row = first ? j + 1 : fail_row;
+            fail_stage = first ? 2 : fail_stage;
+            u = (dv - av * u) / bet;
+            double* sp = scr + (long)j * 3 * OCO_WAVE;
+            sp[0] = gam;
+            sp[OCO_WAVE] = u;
+            sp[2 * OCO_WAVE] = pB.alpha1;
+            Ao[j] = pB.alpha;
+            j = nz - 1;
+            cprev = cv;
+            boundary_row(pC, dz, dz2, bv, dv);
+            av = pC.alpha;                       /* a(nz) = alpha, c(nz) = 0 */
+            gam = cprev / bet;
+            bet = bv - av * gam;
+            first = (bet == 0.0) && (fail_stage == 0);
+            fail_row = first ? j + 1 : fail_row;
+            fail_stage = first ? 2 : fail_stage;
+            u = (dv - av * u) / bet;
+            sp = scr + (long)j * 3 * OCO_WAVE;
+            sp[0] = gam;
+            sp[OCO_WAVE] = u;
+            sp[2 * OCO_WAVE] = pC.alpha1;
+            Ao[j] = pC.alpha;
         }
-        if (failed) continue;
+        if (fail_stage != 0) {
+            /* the reference stops at this system (source/auxiliary.f90:27-39) */
+            record_fail(a.fail, sys, fail_stage, fail_row);
+            continue;
+        }
 
         /* ---- back-substitution (:43-45) fused with update_ghosts (:49-62),
          *      der (:64-78) and calc_flux (:96-109) ---- */
@@ -203,10 +252,10 @@ __global__ __launch_bounds__(OCO_WAVE) void sweep_fp64(const oco_sweep_args a)
         t[0] = Un;
         for (int k = 1; k < 4; k++) {
             int j = nz - 1 - k;
-            const double* s = scr + (long)j * 3 * OCO_WAVE;
-            double Uj = s[OCO_WAVE] - gnext * t[k - 1];
-            gnext = s[0];
-            double a1 = s[2 * OCO_WAVE];
+            const double* sp = scr + (long)j * 3 * OCO_WAVE;
+            double Uj = sp[OCO_WAVE] - gnext * t[k - 1];
+            gnext = sp[0];
+            double a1 = sp[2 * OCO_WAVE];
             a1q3 = a1q2; a1q2 = a1q1; a1q1 = a1q0; a1q0 = a1;
             t[k] = Uj;
             Uo[3 + j] = Uj;
@@ -217,16 +266,15 @@ __global__ __launch_bounds__(OCO_WAVE) void sweep_fp64(const oco_sweep_args a)
         Uo[3 + nz + 1] = g2;
         Uo[3 + nz + 2] = g3;
         double w0 = t[3], w1 = t[2], w2 = t[1], w3 = t[0], w4 = g1, w5 = g2, w6 = g3;
-        /* a1q3 holds 1/alpha of row nz-1 */
         {
             double df = c60 * (((45.0 * (w4 - w2)) - (9.0 * (w5 - w1))) + (w6 - w0));
             Vo[nz - 1] = ((-1.0 * a1q3) * df) * dz1;
         }
         for (int j = nz - 5; j >= 0; j--) {
-            const double* s = scr + (long)j * 3 * OCO_WAVE;
-            double Uj = s[OCO_WAVE] - gnext * w0;
-            gnext = s[0];
-            double a1 = s[2 * OCO_WAVE];
+            const double* sp = scr + (long)j * 3 * OCO_WAVE;
+            double Uj = sp[OCO_WAVE] - gnext * w0;
+            gnext = sp[0];
+            double a1 = sp[2 * OCO_WAVE];
             a1q3 = a1q2; a1q2 = a1q1; a1q1 = a1q0; a1q0 = a1;
             Uo[3 + j] = Uj;
             w6 = w5; w5 = w4; w4 = w3; w3 = w2; w2 = w1; w1 = w0; w0 = Uj;
@@ -249,6 +297,59 @@ __global__ __launch_bounds__(OCO_WAVE) void sweep_fp64(const oco_sweep_args a)
     }
 }
 
+/* First-pillar canaries (source/flux_feautrier.f90:198-201): sum(a), sum(b),
+ * sum(c), sum(d) of system (ix,iy,iw) = (1,1,1).  Printed diagnostics only
+ * (Fortran leaves the summation order of SUM to the compiler), so points,
+ * rows and the sums are done in parallel by one workgroup.  `buf` holds
+ * 4 + 5*nz doubles: the sums, then alpha, 1/alpha, alpha^2, albedo, S. */
+template <bool GREY>
+__global__ __launch_bounds__(256) void canary_fp64(const oco_sweep_args a, double* buf)
+{
+    __shared__ oco_mtab smt;
+    __shared__ double red[4][256];
+    load_mtab(&smt);
+    const int nz = a.nz, tid = threadIdx.x;
+    double* pts = buf + 4;
+    double w[OCO_WT_STRIDE];
+    if (!GREY)
+        for (int k = 0; k < OCO_WT_STRIDE; k++) w[k] = a.wtab[k];
+    for (int z = tid; z < nz; z += blockDim.x) {
+        double rec[CR_NFIELD];
+        if (!GREY) load_record<false>(a, 0, z, rec);
+        oco_pt p = point_rec<GREY, false>(a, 0, z, rec, w, &smt);
+        double* q = pts + 5 * z;
+        q[0] = p.alpha; q[1] = p.alpha1; q[2] = p.alpha2; q[3] = p.albedo; q[4] = p.S;
+    }
+    __syncthreads();
+    double sa = 0.0, sb = 0.0, sc = 0.0, sd = 0.0;
+    for (int r = tid; r < nz; r += blockDim.x) {
+        const double* q = pts + 5 * r;
+        oco_pt pc;
+        pc.alpha = q[0]; pc.alpha1 = q[1]; pc.alpha2 = q[2]; pc.albedo = q[3]; pc.S = q[4];
+        double av, bv, cv, dv;
+        if (r == 0) {
+            boundary_row(pc, a.dz, a.dz2, bv, dv);
+            av = 0.0;
+            cv = pc.alpha;
+        } else if (r == nz - 1) {
+            boundary_row(pc, a.dz, a.dz2, bv, dv);
+            av = pc.alpha;
+            cv = 0.0;
+        } else {
+            interior_row(pts[5 * (r - 1)], pc, pts[5 * (r + 1)], a.dz2, av, bv, cv, dv);
+        }
+        sa += av; sb += bv; sc += cv; sd += dv;
+    }
+    red[0][tid] = sa; red[1][tid] = sb; red[2][tid] = sc; red[3][tid] = sd;
+    __syncthreads();
+    for (int h = blockDim.x / 2; h > 0; h >>= 1) {
+        if (tid < h)
+            for (int k = 0; k < 4; k++) red[k][tid] += red[k][tid + h];
+        __syncthreads();
+    }
+    if (tid < 4) buf[tid] = red[tid][0];
+}
+
 hipError_t oco_launch_cells(const double* rho, const double* T, long ncells, double fully_ionized_T,
                             double* cells, hipStream_t stream)
 {
@@ -265,6 +366,12 @@ int oco_sweep_max_blocks(void) { return 4096; }
 hipError_t oco_launch_sweep(const oco_sweep_args& a, int lgrey, int nblocks, hipStream_t stream)
 {
     const bool uniform = (a.nw % OCO_WAVE) == 0;
+    if (a.canary) {
+        if (lgrey)
+            hipLaunchKernelGGL((canary_fp64<true>), dim3(1), dim3(256), 0, stream, a, a.canary);
+        else
+            hipLaunchKernelGGL((canary_fp64<false>), dim3(1), dim3(256), 0, stream, a, a.canary);
+    }
     if (lgrey) {
         hipLaunchKernelGGL((sweep_fp64<true, false>), dim3(nblocks), dim3(OCO_WAVE), 0, stream, a);
     } else if (uniform) {
