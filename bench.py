@@ -23,7 +23,32 @@ REPO = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, REPO)
 
 HBM_PEAK_GBS = 8000.0
-FP64_PEAK_TFLOPS = 78.6
+FP64_PEAK_TFLOPS = 78.6          # MI355X FP64 (vector and matrix): 256 CUs x 64 lanes x 2 x 2.4 GHz
+FP64_LANE_OPS = 256 * 64 * 2.4e9  # VALU FP64 instruction-lanes per second (one op per lane per cycle)
+
+# VALU instructions per call of each elementary function / FP64 divide, counted
+# in this build's gfx950 ISA (oco_math.h, -O3 -ffp-contract=off), frozen here.
+W_EXP, W_EXP10, W_LOG, W_DIV = 37, 40, 74, 11
+# Operations per point after hoisting the temperature-only transcendentals
+# into the per-cell kernel: the reference's 138 (neutral) / 143 (ionised)
+# arithmetic operations with 7 divides, plus the wavelength-dependent
+# transcendentals (neutral: 2 exp10 + 1 exp; ionised adds 3 log + 3 exp).
+W_NEUTRAL = 138 + 7 * (W_DIV - 1) + 2 * W_EXP10 + W_EXP
+W_IONISED = 143 + 7 * (W_DIV - 1) + 2 * W_EXP10 + W_EXP + 3 * W_LOG + 3 * W_EXP
+# HBM bytes per sweep launch on this block from the rocprofv3 FETCH_SIZE /
+# WRITE_SIZE passes (x2.0 / x1.0 copy-kernel calibration); None if not measured
+SWEEP_TRAFFIC_BYTES = None
+
+
+def ionised_fraction(rho, T, fully_ionized_T=2e4):
+    """Share of cells on the bremsstrahlung branch ((1-x) <= 0.01,
+    source/continuous_opacity.f90:262), from the Saha root of the inputs."""
+    k_ev, mp = 8.617333262145179e-05, 1.67262192369e-24
+    with np.errstate(over="ignore", under="ignore"):
+        C = 2414683039571967.0 * T ** 1.5 * np.exp(-13.6 / (k_ev * T)) * mp / rho
+        x = 0.5 * (np.sqrt(C * C + 4 * C) - C)
+    x = np.where(T > fully_ionized_T, 1.0, x)
+    return float(((1.0 - x) <= 0.01).mean())
 
 
 def algorithmic_bytes(nz, ncol, nw):
@@ -106,10 +131,21 @@ def main():
     value = points / (ms_per_step * 1e-3)
 
     sw = float(np.mean(sweep_ms))
+    fion = ionised_fraction(rho, T)
+    w_pt = (1.0 - fion) * W_NEUTRAL + fion * W_IONISED
+    lane_ops = points * w_pt / (sw * 1e-3)
+    # the binding roof of the sweep is FP64 VALU issue (no MFMA: nothing here is
+    # a dense contraction); TFLOP/s counts each FP64 lane-op at the FMA rate
+    roofline = {"kernel": "sweep_fp64", "bound": "valu", "achieved": round(2 * lane_ops / 1e12, 3),
+                "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": round(lane_ops / FP64_LANE_OPS, 4),
+                "traffic": SWEEP_TRAFFIC_BYTES, "kernel_ms": round(sw, 4),
+                "ops_per_point": round(w_pt, 1), "ionised_fraction": round(fion, 4),
+                "weights": {"exp": W_EXP, "exp10": W_EXP10, "log": W_LOG, "div": W_DIV},
+                "cells_kernel_ms": round(float(np.mean(cells_ms)), 4)}
     achieved_gbs = sweep_bytes(nz, ncol, nw) / (sw * 1e-3) / 1e9
-    roofline = {"kernel": "sweep_fp64", "bound": "hbm", "achieved": round(achieved_gbs, 1),
-                "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved_gbs / HBM_PEAK_GBS, 4),
-                "traffic": None, "kernel_ms": round(sw, 4), "cells_kernel_ms": round(float(np.mean(cells_ms)), 4)}
+    roofline_hbm = {"kernel": "sweep_fp64", "bound": "hbm", "achieved": round(achieved_gbs, 1),
+                    "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved_gbs / HBM_PEAK_GBS, 4),
+                    "bytes_per_point": round(sweep_bytes(nz, ncol, nw) / points, 3)}
 
     cpu = None
     if not args.no_cpu:
@@ -131,6 +167,7 @@ def main():
                                "seeded disk profiles (rho floor 1e-15)" % (nx, ny, nz, nw),
                    "points_per_step": points, "mode": "fp64"},
         "roofline": roofline,
+        "roofline_hbm": roofline_hbm,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -252,7 +252,9 @@ int oco_solve_block_device(oco_ctx* c, int nz, int nyloc, int nxloc, double dz, 
     int nblocks = (int)(ntiles < oco_sweep_max_blocks() ? ntiles : oco_sweep_max_blocks());
     int st;
     if (!c->lgrey && (st = ensure(c, c->cells, (size_t)ncells * OCO_CELL_STRIDE * sizeof(double)))) return st;
-    if ((st = ensure(c, c->scratch, (size_t)nblocks * nz * 3 * 64 * sizeof(double)))) return st;
+    /* one scratch slot per wavefront; multi-wavefront workgroups round up */
+    const size_t slots = ((size_t)nblocks + 7) / 8 * 8;
+    if ((st = ensure(c, c->scratch, slots * nz * 3 * 64 * sizeof(double)))) return st;
     if ((st = ensure(c, c->fail, sizeof(unsigned long long)))) return st;
     if ((st = ensure(c, c->canary, (4 + 5 * (size_t)nz) * sizeof(double)))) return st;
 

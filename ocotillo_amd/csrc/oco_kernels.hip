@@ -25,6 +25,9 @@
 __constant__ oco_mtab oco_mtab_dev = OCO_MTAB_INIT;
 
 #define OCO_WAVE 64
+#ifndef OCO_BACK_CHUNK
+#define OCO_BACK_CHUNK 2   /* back-substitution rows whose scratch loads are batched */
+#endif
 
 __device__ static inline void load_mtab(oco_mtab* smt)
 {
@@ -96,7 +99,7 @@ __device__ static inline void boundary_row(const oco_pt& pc, double dz, double d
     dv = -((pc.S * ze) * pc.alpha2);
 }
 
-template <bool GREY, bool UNIFORM>
+template <bool GREY, bool UNIFORM, int WS = 1>
 __device__ static inline oco_pt point_rec(const oco_sweep_args& a, long col, int z, const double* rec,
                                           const double* w, const oco_mtab* M)
 {
@@ -104,7 +107,7 @@ __device__ static inline oco_pt point_rec(const oco_sweep_args& a, long col, int
         long c = col * a.nz + z;
         return oco_grey_point(a.rho[c], a.T[c], a.sigma_grey, M);
     }
-    return oco_point(rec, w, M);
+    return oco_point_s<WS>(rec, w, M);
 }
 
 /* Tuning knob: OCO_SWEEP_WAVES_PER_EU=n asks the compiler to fit n waves per
@@ -115,24 +118,209 @@ __device__ static inline oco_pt point_rec(const oco_sweep_args& a, long col, int
 #define OCO_SWEEP_ATTR
 #endif
 
+/* One (column, wavelength) system, one lane: forward elimination with the
+ * opacity of the cells recomputed in registers, then back-substitution fused
+ * with ghosts, der and V.  `w` is the lane's wavelength-table row with
+ * stride WS; `scr` its scratch column (stride 64 doubles per entry). */
+template <bool GREY, bool UNIFORM, int WS>
+__device__ static inline void solve_system(const oco_sweep_args& a, long col, int iw, const double* wbase,
+                                           int woff, double* scr, const oco_mtab* M)
+{
+    const double* w = wbase + woff;
+    const int nz = a.nz, nw = a.nw;
+    const double dz = a.dz, dz1 = a.dz1, dz2 = a.dz2;
+    const long sys = col * nw + iw;
+    const long osys = (long)iw * a.ncol + col;
+    double* Uo = a.U + osys * (nz + 6);
+    double* Vo = a.V + osys * nz;
+    double* Ao = a.A + osys * nz;
+
+    /* ---- forward elimination (source/auxiliary.f90:25-41) ----
+     * Two-deep pipeline: iteration j eliminates row j (which needs the
+     * points j-1, j, j+1, all computed earlier) and evaluates point j+2,
+     * whose work is independent of the elimination chain. */
+    double rec[CR_NFIELD];
+    if (!GREY) load_record<UNIFORM>(a, col, 0, rec);
+    oco_pt pB = point_rec<GREY, UNIFORM, WS>(a, col, 0, rec, w, M);
+    if (!GREY) load_record<UNIFORM>(a, col, 1, rec);
+    oco_pt pC = point_rec<GREY, UNIFORM, WS>(a, col, 1, rec, w, M);
+    if (!GREY) load_record<UNIFORM>(a, col, 2, rec);
+
+    double bv, dv, av, cv;
+    boundary_row(pB, dz, dz2, bv, dv);      /* row 0: a = 0, c = alpha */
+    cv = pB.alpha;
+    int fail_stage = (bv == 0.0) ? 1 : 0;
+    int fail_row = 1;
+    double bet = bv;
+    double u = dv / bet;
+    scr[0 * OCO_WAVE] = 0.0;
+    scr[1 * OCO_WAVE] = u;
+    scr[2 * OCO_WAVE] = pB.alpha1;
+    Ao[0] = pB.alpha;
+    double aA = pB.alpha;   /* alpha of row j-1 */
+    pB = pC;                /* row j = 1 */
+    pC = point_rec<GREY, UNIFORM, WS>(a, col, 2, rec, w, M);
+
+    for (int j = 1; j <= nz - 3; j++) {
+        if (!GREY) load_record<UNIFORM>(a, col, j + 2, rec);
+        /* row j and its elimination step */
+        double cprev = cv;
+        interior_row(aA, pB, pC.alpha, dz2, av, bv, cv, dv);
+        double gam = cprev / bet;
+        bet = bv - av * gam;
+        int first = (bet == 0.0) && (fail_stage == 0);
+        fail_row = first ? j + 1 : fail_row;
+        fail_stage = first ? 2 : fail_stage;
+        u = (dv - av * u) / bet;
+        double* sp = scr + (long)j * 3 * OCO_WAVE;
+        sp[0] = gam;
+        sp[OCO_WAVE] = u;
+        sp[2 * OCO_WAVE] = pB.alpha1;
+        Ao[j] = pB.alpha;
+        /* point j+2 (independent of the chain above) */
+        oco_pt pD = point_rec<GREY, UNIFORM, WS>(a, col, j + 2, rec, w, M);
+        aA = pB.alpha;
+        pB = pC;
+        pC = pD;
+    }
+    {   /* row nz-2 (interior, no new point) and row nz-1 (boundary) */
+        int j = nz - 2;
+        double cprev = cv;
+        interior_row(aA, pB, pC.alpha, dz2, av, bv, cv, dv);
+        double gam = cprev / bet;
+        bet = bv - av * gam;
+        int first = (bet == 0.0) && (fail_stage == 0);
+        fail_row = first ? j + 1 : fail_row;
+        fail_stage = first ? 2 : fail_stage;
+        u = (dv - av * u) / bet;
+        double* sp = scr + (long)j * 3 * OCO_WAVE;
+        sp[0] = gam;
+        sp[OCO_WAVE] = u;
+        sp[2 * OCO_WAVE] = pB.alpha1;
+        Ao[j] = pB.alpha;
+        j = nz - 1;
+        cprev = cv;
+        boundary_row(pC, dz, dz2, bv, dv);
+        av = pC.alpha;                       /* a(nz) = alpha, c(nz) = 0 */
+        gam = cprev / bet;
+        bet = bv - av * gam;
+        first = (bet == 0.0) && (fail_stage == 0);
+        fail_row = first ? j + 1 : fail_row;
+        fail_stage = first ? 2 : fail_stage;
+        u = (dv - av * u) / bet;
+        sp = scr + (long)j * 3 * OCO_WAVE;
+        sp[0] = gam;
+        sp[OCO_WAVE] = u;
+        sp[2 * OCO_WAVE] = pC.alpha1;
+        Ao[j] = pC.alpha;
+    }
+    if (fail_stage != 0) {
+        /* the reference stops at this system (source/auxiliary.f90:27-39) */
+        record_fail(a.fail, sys, fail_stage, fail_row);
+        return;
+    }
+
+    /* ---- back-substitution (:43-45) fused with update_ghosts (:49-62),
+     *      der (:64-78) and calc_flux (:96-109) ---- */
+    const double c60 = 1.0 / 60.0;
+    /* window w0..w6 = extended U at e = j .. j+6 (e = z + 3) */
+    double Un = u;                       /* U(nz-1) */
+    double gnext = scr[(long)(nz - 1) * 3 * OCO_WAVE];
+    double a1q0 = scr[(long)(nz - 1) * 3 * OCO_WAVE + 2 * OCO_WAVE];   /* 1/alpha of the V row pending */
+    double a1q1 = 0.0, a1q2 = 0.0, a1q3 = 0.0;
+    Uo[3 + nz - 1] = Un;
+    double t[4];
+    t[0] = Un;
+    for (int k = 1; k < 4; k++) {
+        int j = nz - 1 - k;
+        const double* sp = scr + (long)j * 3 * OCO_WAVE;
+        double Uj = sp[OCO_WAVE] - gnext * t[k - 1];
+        gnext = sp[0];
+        double a1 = sp[2 * OCO_WAVE];
+        a1q3 = a1q2; a1q2 = a1q1; a1q1 = a1q0; a1q0 = a1;
+        t[k] = Uj;
+        Uo[3 + j] = Uj;
+    }
+    /* top ghosts (f(n2+i) = 2 f(n2) - f(n2-i)) */
+    double g1 = 2.0 * t[0] - t[1], g2 = 2.0 * t[0] - t[2], g3 = 2.0 * t[0] - t[3];
+    Uo[3 + nz] = g1;
+    Uo[3 + nz + 1] = g2;
+    Uo[3 + nz + 2] = g3;
+    double w0 = t[3], w1 = t[2], w2 = t[1], w3 = t[0], w4 = g1, w5 = g2, w6 = g3;
+    {
+        double df = c60 * (((45.0 * (w4 - w2)) - (9.0 * (w5 - w1))) + (w6 - w0));
+        Vo[nz - 1] = ((-1.0 * a1q3) * df) * dz1;
+    }
+    /* rows nz-5 .. 0 in chunks of OCO_BACK_CHUNK: each chunk's scratch
+     * entries are loaded together, one chunk ahead of their use, so the loop
+     * pays one memory latency per chunk instead of one per row */
+    {
+        double cg[OCO_BACK_CHUNK], cu[OCO_BACK_CHUNK], ca[OCO_BACK_CHUNK];
+        double ng[OCO_BACK_CHUNK], nu[OCO_BACK_CHUNK], na[OCO_BACK_CHUNK];
+        int jc = nz - 5;
+#pragma unroll
+        for (int k = 0; k < OCO_BACK_CHUNK; k++) {
+            int j = jc - k;
+            const double* sp = scr + (long)(j < 0 ? 0 : j) * 3 * OCO_WAVE;
+            cg[k] = sp[0]; cu[k] = sp[OCO_WAVE]; ca[k] = sp[2 * OCO_WAVE];
+        }
+        for (; jc >= 0; jc -= OCO_BACK_CHUNK) {
+#pragma unroll
+            for (int k = 0; k < OCO_BACK_CHUNK; k++) {      /* next chunk, in flight */
+                int j = jc - OCO_BACK_CHUNK - k;
+                const double* sp = scr + (long)(j < 0 ? 0 : j) * 3 * OCO_WAVE;
+                ng[k] = sp[0]; nu[k] = sp[OCO_WAVE]; na[k] = sp[2 * OCO_WAVE];
+            }
+#pragma unroll
+            for (int k = 0; k < OCO_BACK_CHUNK; k++) {
+                int j = jc - k;
+                if (j >= 0) {
+                    double Uj = cu[k] - gnext * w0;
+                    gnext = cg[k];
+                    a1q3 = a1q2; a1q2 = a1q1; a1q1 = a1q0; a1q0 = ca[k];
+                    Uo[3 + j] = Uj;
+                    w6 = w5; w5 = w4; w4 = w3; w3 = w2; w2 = w1; w1 = w0; w0 = Uj;
+                    double df = c60 * (((45.0 * (w4 - w2)) - (9.0 * (w5 - w1))) + (w6 - w0));
+                    Vo[j + 3] = ((-1.0 * a1q3) * df) * dz1;
+                }
+            }
+#pragma unroll
+            for (int k = 0; k < OCO_BACK_CHUNK; k++) { cg[k] = ng[k]; cu[k] = nu[k]; ca[k] = na[k]; }
+        }
+    }
+    /* bottom ghosts (f(n1-i) = 2 f(n1) - f(n1+i)); window now e = 3..9 */
+    double u0 = w0, G1 = 2.0 * u0 - w1, G2 = 2.0 * u0 - w2, G3 = 2.0 * u0 - w3;
+    Uo[2] = G1;
+    Uo[1] = G2;
+    Uo[0] = G3;
+    const double gh[3] = {G1, G2, G3};
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        w6 = w5; w5 = w4; w4 = w3; w3 = w2; w2 = w1; w1 = w0; w0 = gh[k];
+        a1q3 = a1q2; a1q2 = a1q1; a1q1 = a1q0;
+        double df = c60 * (((45.0 * (w4 - w2)) - (9.0 * (w5 - w1))) + (w6 - w0));
+        Vo[2 - k] = ((-1.0 * a1q3) * df) * dz1;
+    }
+}
+
+/* One wavefront per workgroup; the lane's wavelength-table row lives in
+ * registers.  Used when a wavefront's 64 systems span several columns
+ * (nw not a multiple of 64) and for the grey case. */
 template <bool GREY, bool UNIFORM>
 __global__ __launch_bounds__(OCO_WAVE) OCO_SWEEP_ATTR void sweep_fp64(const oco_sweep_args a)
 {
     __shared__ oco_mtab smt;
     load_mtab(&smt);
     const int lane = threadIdx.x;
-    const int nz = a.nz, nw = a.nw;
+    const int nw = a.nw;
     const long nsys = a.ncol * nw;
     const long ntiles = (nsys + OCO_WAVE - 1) / OCO_WAVE;
-    const double dz = a.dz, dz1 = a.dz1, dz2 = a.dz2;
-    double* scr = a.scratch + (long)blockIdx.x * nz * 3 * OCO_WAVE + lane;
-    const long ostride = a.ncol;   /* systems per wavelength in the output layout */
-
+    double* scr = a.scratch + (long)blockIdx.x * a.nz * 3 * OCO_WAVE + lane;
     for (long tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
         long sys0 = tile * OCO_WAVE;
         long col;
         int iw;
-        if (UNIFORM) {            /* nw % 64 == 0: one column per wavefront */
+        if (UNIFORM) {
             col = sys0 / nw;
             iw = (int)(sys0 % nw) + lane;
         } else {
@@ -141,161 +329,16 @@ __global__ __launch_bounds__(OCO_WAVE) OCO_SWEEP_ATTR void sweep_fp64(const oco_
             col = s / nw;
             iw = (int)(s % nw);
         }
-        const long sys = col * nw + iw;
-        const long osys = (long)iw * ostride + col;
-        double* Uo = a.U + osys * (nz + 6);
-        double* Vo = a.V + osys * nz;
-        double* Ao = a.A + osys * nz;
         double w[OCO_WT_STRIDE];
         if (!GREY) {
 #pragma unroll
             for (int k = 0; k < OCO_WT_STRIDE; k++) w[k] = a.wtab[iw * OCO_WT_STRIDE + k];
         }
-
-        /* ---- forward elimination (source/auxiliary.f90:25-41) ----
-         * Two-deep pipeline: iteration j eliminates row j (which needs the
-         * points j-1, j, j+1, all computed earlier) and evaluates point j+2,
-         * whose work is independent of the elimination chain; the record of
-         * cell j+3 is fetched one iteration ahead. */
-        double rec[CR_NFIELD];
-        if (!GREY) load_record<UNIFORM>(a, col, 0, rec);
-        oco_pt pB = point_rec<GREY, UNIFORM>(a, col, 0, rec, w, &smt);
-        if (!GREY) load_record<UNIFORM>(a, col, 1, rec);
-        oco_pt pC = point_rec<GREY, UNIFORM>(a, col, 1, rec, w, &smt);
-        if (!GREY) load_record<UNIFORM>(a, col, 2, rec);
-
-        double bv, dv, av, cv;
-        boundary_row(pB, dz, dz2, bv, dv);      /* row 0: a = 0, c = alpha */
-        cv = pB.alpha;
-        int fail_stage = (bv == 0.0) ? 1 : 0;
-        int fail_row = 1;
-        double bet = bv;
-        double u = dv / bet;
-        scr[0 * OCO_WAVE] = 0.0;
-        scr[1 * OCO_WAVE] = u;
-        scr[2 * OCO_WAVE] = pB.alpha1;
-        Ao[0] = pB.alpha;
-        double aA = pB.alpha;   /* alpha of row j-1 */
-        pB = pC;                /* row j = 1 */
-        pC = point_rec<GREY, UNIFORM>(a, col, 2, rec, w, &smt);
-
-        for (int j = 1; j <= nz - 3; j++) {
-            double nrec[CR_NFIELD];
-            if (!GREY) load_record<UNIFORM>(a, col, j + 2, nrec);
-            /* row j and its elimination step */
-            double cprev = cv;
-            interior_row(aA, pB, pC.alpha, dz2, av, bv, cv, dv);
-            double gam = cprev / bet;
-            bet = bv - av * gam;
-            int first = (bet == 0.0) && (fail_stage == 0);
-            fail_row = first ? j + 1 : fail_row;
-            fail_stage = first ? 2 : fail_stage;
-            u = (dv - av * u) / bet;
-            double* sp = scr + (long)j * 3 * OCO_WAVE;
-            sp[0] = gam;
-            sp[OCO_WAVE] = u;
-            sp[2 * OCO_WAVE] = pB.alpha1;
-            Ao[j] = pB.alpha;
-            /* point j+2 (independent of the chain above) */
-            oco_pt pD = point_rec<GREY, UNIFORM>(a, col, j + 2, nrec, w, &smt);
-            aA = pB.alpha;
-            pB = pC;
-            pC = pD;
-        }
-        {   /* row nz-2 (interior, no new point) and row nz-1 (boundary) */
-            int j = nz - 2;
-            double cprev = cv;
-            interior_row(aA, pB, pC.alpha, dz2, av, bv, cv, dv);
-            double gam = cprev / bet;
-            bet = bv - av * gam;
-            int first = (bet == 0.0) && (fail_stage == 0);
-            fail_row = first ? j + 1 : fail_row;
-            fail_stage = first ? 2 : fail_stage;
-            u = (dv - av * u) / bet;
-            double* sp = scr + (long)j * 3 * OCO_WAVE;
-            sp[0] = gam;
-            sp[OCO_WAVE] = u;
-            sp[2 * OCO_WAVE] = pB.alpha1;
-            Ao[j] = pB.alpha;
-            j = nz - 1;
-            cprev = cv;
-            boundary_row(pC, dz, dz2, bv, dv);
-            av = pC.alpha;                       /* a(nz) = alpha, c(nz) = 0 */
-            gam = cprev / bet;
-            bet = bv - av * gam;
-            first = (bet == 0.0) && (fail_stage == 0);
-            fail_row = first ? j + 1 : fail_row;
-            fail_stage = first ? 2 : fail_stage;
-            u = (dv - av * u) / bet;
-            sp = scr + (long)j * 3 * OCO_WAVE;
-            sp[0] = gam;
-            sp[OCO_WAVE] = u;
-            sp[2 * OCO_WAVE] = pC.alpha1;
-            Ao[j] = pC.alpha;
-        }
-        if (fail_stage != 0) {
-            /* the reference stops at this system (source/auxiliary.f90:27-39) */
-            record_fail(a.fail, sys, fail_stage, fail_row);
-            continue;
-        }
-
-        /* ---- back-substitution (:43-45) fused with update_ghosts (:49-62),
-         *      der (:64-78) and calc_flux (:96-109) ---- */
-        const double c60 = 1.0 / 60.0;
-        /* window w0..w6 = extended U at e = j .. j+6 (e = z + 3) */
-        double Un = u;                       /* U(nz-1) */
-        double gnext = scr[(long)(nz - 1) * 3 * OCO_WAVE];
-        double a1q0 = scr[(long)(nz - 1) * 3 * OCO_WAVE + 2 * OCO_WAVE];   /* 1/alpha of the V row pending */
-        double a1q1 = 0.0, a1q2 = 0.0, a1q3 = 0.0;
-        Uo[3 + nz - 1] = Un;
-        double t[4];
-        t[0] = Un;
-        for (int k = 1; k < 4; k++) {
-            int j = nz - 1 - k;
-            const double* sp = scr + (long)j * 3 * OCO_WAVE;
-            double Uj = sp[OCO_WAVE] - gnext * t[k - 1];
-            gnext = sp[0];
-            double a1 = sp[2 * OCO_WAVE];
-            a1q3 = a1q2; a1q2 = a1q1; a1q1 = a1q0; a1q0 = a1;
-            t[k] = Uj;
-            Uo[3 + j] = Uj;
-        }
-        /* top ghosts (f(n2+i) = 2 f(n2) - f(n2-i)) */
-        double g1 = 2.0 * t[0] - t[1], g2 = 2.0 * t[0] - t[2], g3 = 2.0 * t[0] - t[3];
-        Uo[3 + nz] = g1;
-        Uo[3 + nz + 1] = g2;
-        Uo[3 + nz + 2] = g3;
-        double w0 = t[3], w1 = t[2], w2 = t[1], w3 = t[0], w4 = g1, w5 = g2, w6 = g3;
-        {
-            double df = c60 * (((45.0 * (w4 - w2)) - (9.0 * (w5 - w1))) + (w6 - w0));
-            Vo[nz - 1] = ((-1.0 * a1q3) * df) * dz1;
-        }
-        for (int j = nz - 5; j >= 0; j--) {
-            const double* sp = scr + (long)j * 3 * OCO_WAVE;
-            double Uj = sp[OCO_WAVE] - gnext * w0;
-            gnext = sp[0];
-            double a1 = sp[2 * OCO_WAVE];
-            a1q3 = a1q2; a1q2 = a1q1; a1q1 = a1q0; a1q0 = a1;
-            Uo[3 + j] = Uj;
-            w6 = w5; w5 = w4; w4 = w3; w3 = w2; w2 = w1; w1 = w0; w0 = Uj;
-            double df = c60 * (((45.0 * (w4 - w2)) - (9.0 * (w5 - w1))) + (w6 - w0));
-            Vo[j + 3] = ((-1.0 * a1q3) * df) * dz1;
-        }
-        /* bottom ghosts (f(n1-i) = 2 f(n1) - f(n1+i)); window now e = 3..9 */
-        double u0 = w0, G1 = 2.0 * u0 - w1, G2 = 2.0 * u0 - w2, G3 = 2.0 * u0 - w3;
-        Uo[2] = G1;
-        Uo[1] = G2;
-        Uo[0] = G3;
-        const double gh[3] = {G1, G2, G3};
-#pragma unroll
-        for (int k = 0; k < 3; k++) {
-            w6 = w5; w5 = w4; w4 = w3; w3 = w2; w2 = w1; w1 = w0; w0 = gh[k];
-            a1q3 = a1q2; a1q2 = a1q1; a1q1 = a1q0;
-            double df = c60 * (((45.0 * (w4 - w2)) - (9.0 * (w5 - w1))) + (w6 - w0));
-            Vo[2 - k] = ((-1.0 * a1q3) * df) * dz1;
-        }
+        solve_system<GREY, UNIFORM, 1>(a, col, iw, w, 0, scr, &smt);
     }
 }
+
+
 
 /* First-pillar canaries (source/flux_feautrier.f90:198-201): sum(a), sum(b),
  * sum(c), sum(d) of system (ix,iy,iw) = (1,1,1).  Printed diagnostics only

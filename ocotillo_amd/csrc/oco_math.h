@@ -67,8 +67,6 @@ typedef struct {
 #define OCO_IVLN10_HI  0x1.bcb7b1526e50ep-2
 #define OCO_IVLN10_LO  0x1.95355baaafad3p-57
 #define OCO_LOG_OFF    0x3fe6a00000000000ULL   /* bits of 0x1.6ap-1 */
-#define OCO_EXP_XMAX   0x1.62e42fefa39efp+9    /* largest x with exp(x) finite */
-#define OCO_EXP_XMIN  -0x1.74910d52d3052p+9    /* below: exp(x) < 2^-1075 */
 #define OCO_SHIFT      0x1.8p52
 
 OCO_HD static inline uint64_t oco_asu64(double d)
@@ -92,30 +90,27 @@ OCO_HD static inline double oco_fma(double a, double b, double c) { return __bui
 OCO_HD static inline double oco_sel(int c, double a, double b) { return c ? a : b; }
 
 /* exp(hi + lo), |lo| <= ulp(hi).  No branch on the common path: the
- * argument is clamped, the scale exponent is clamped to the normal range and
- * the remainder applied by ldexp (exact, or one rounding into the subnormal
- * range), and the special results are selected at the end. */
+ * argument is clamped to +-2000 (beyond which the result is inf or 0 anyway),
+ * the table scale's exponent to the normal range, and ldexp applies the rest
+ * (exact, or one rounding into the subnormal range, or overflow to inf);
+ * only NaN needs a select. */
 OCO_HD static inline double oco_exp_dd(double hi, double lo, const oco_mtab* T)
 {
-    double hc = hi > OCO_EXP_XMAX ? OCO_EXP_XMAX : hi;
-    hc = hc < OCO_EXP_XMIN ? OCO_EXP_XMIN : hc;
-    hc = (hi != hi) ? 0.0 : hc;
+    double hc = __builtin_fmin(__builtin_fmax(hi, -2000.0), 2000.0);
     double kd = hc * OCO_INVLN2N + OCO_SHIFT;
     kd = kd - OCO_SHIFT;
-    int64_t ki = (int64_t)kd;
+    int ki = (int)kd;
     double r = (hc - kd * OCO_LN2N_HI) - kd * OCO_LN2N_LO;
     r = r + lo;
-    int idx = (int)(ki & 127);
-    int e = (int)(ki >> 7);
+    int idx = ki & 127;
+    int e = ki >> 7;
     double r2 = r * r;
-    double p = r + r2 * (0.5 + r * 0x1.5555555555555p-3)
-                 + (r2 * r2) * (0x1.5555555555555p-5 + r * 0x1.1111111111111p-7 + r2 * 0x1.6c16c16c16c17p-10);
+    double p = oco_fma(r2, oco_fma(r, 0x1.5555555555555p-3, 0.5), r);
+    p = oco_fma(r2 * r2, oco_fma(r2, 0x1.6c16c16c16c17p-10, oco_fma(r, 0x1.1111111111111p-7, 0x1.5555555555555p-5)), p);
     double tmp = T->exp_tlo[idx] + p;
     int ec = e < -1000 ? -1000 : (e > 1023 ? 1023 : e);
     double scale = oco_asf64(oco_asu64(T->exp_thi[idx]) + ((uint64_t)(int64_t)ec << 52));
-    double y = __builtin_ldexp(scale + scale * tmp, e - ec);
-    y = oco_sel(hi > OCO_EXP_XMAX, OCO_INF, y);
-    y = oco_sel(hi < OCO_EXP_XMIN, 0.0, y);
+    double y = __builtin_ldexp(oco_fma(scale, tmp, scale), e - ec);
     return oco_sel(hi != hi, hi + hi, y);
 }
 
@@ -163,17 +158,18 @@ OCO_HD static inline void oco_log_dd(double x, const oco_mtab* T, double* hi, do
     *hi = hs;
 }
 
+#define OCO_QNAN oco_asf64(0x7ff8000000000000ULL)
+
 /* result for x outside (0, inf): log(0) = -inf, log(inf) = inf, else NaN */
 OCO_HD static inline double oco_log_special(double x)
 {
-    return x == 0.0 ? -OCO_INF : (x == OCO_INF ? x : (x - x) / (x - x));
+    return oco_sel(x == 0.0, -OCO_INF, oco_sel(x == OCO_INF, x, OCO_QNAN));
 }
 
 OCO_HD static inline double oco_log_t(double x, const oco_mtab* T)
 {
     double h, l;
-    oco_log_dd(x, T, &h, &l);
-    h = oco_sel(x == 1.0, 0.0, h);
+    oco_log_dd(x, T, &h, &l);   /* log(1) = 0 exactly: invc = 1 on its cell */
     return oco_sel(x > 0.0 && x < OCO_INF, h, oco_log_special(x));
 }
 
@@ -184,7 +180,6 @@ OCO_HD static inline double oco_log10_t(double x, const oco_mtab* T)
     double p = h * OCO_IVLN10_HI;
     double pe = oco_fma(h, OCO_IVLN10_HI, -p);
     double y = p + (pe + (h * OCO_IVLN10_LO + l * OCO_IVLN10_HI));
-    y = oco_sel(x == 1.0, 0.0, y);
     return oco_sel(x > 0.0 && x < OCO_INF, y, oco_log_special(x));
 }
 
@@ -206,7 +201,7 @@ OCO_HD static inline double oco_pow_t(double x, double y, const oco_mtab* T)
     double r = oco_exp_dd(yh, yl, T);
     int finite_pos = x > 0.0 && x < OCO_INF;
     r = oco_sel(finite_pos, r, oco_sel(x == 0.0, oco_sel(y > 0.0, 0.0, OCO_INF),
-                                       oco_sel(x == OCO_INF, oco_sel(y > 0.0, OCO_INF, 0.0), (x - x) / (x - x))));
+                                       oco_sel(x == OCO_INF, oco_sel(y > 0.0, OCO_INF, 0.0), OCO_QNAN)));
     r = oco_sel(x != x || y != y, x + y, r);
     return oco_sel(y == 0.0 || x == 1.0, 1.0, r);
 }

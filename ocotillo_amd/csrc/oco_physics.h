@@ -149,34 +149,37 @@ typedef struct {
 } oco_pt;
 
 /* calc_opacity_and_albedo for one (cell, wavelength):
- * source/continuous_opacity.f90:175-201, 203-341, 392-409. */
-OCO_HD static inline oco_pt oco_point(const double* c, const double* w, const oco_mtab* M)
+ * source/continuous_opacity.f90:175-201, 203-341, 392-409.
+ * w[k * WS] is field k of this wavelength's table row (WS = 1 for a row in
+ * registers, 64 for a [field][lane] table in LDS). */
+template <int WS>
+OCO_HD static inline oco_pt oco_point_s(const double* c, const double* w, const oco_mtab* M)
 {
     oco_pt p;
     double theta = c[CR_THETA], fn = c[CR_FN];
-    double stim = 1.0 - oco_exp10_t(-(w[WT_CHI] * theta), M);
-    double sm = w[WT_G1];   /* n = 1: g*1*exp(-0) */
-    sm = sm + w[WT_G2] * c[CR_E2];
-    sm = sm + w[WT_G3] * c[CR_E3];
-    sm = sm + w[WT_G4] * c[CR_E4];
-    sm = sm + w[WT_G5] * c[CR_E5];
-    sm = sm + w[WT_G6] * c[CR_E6];
-    double kHbf = (((OCO_AHBF * stim) * fn) * w[WT_WA3]) * (c[CR_UNSOLD] + sm);
-    double kHmbf = (w[WT_HMBF_ON] != 0.0) ? ((c[CR_HMBF] * stim) * fn) * w[WT_SCOEFF] : 0.0;
-    double flam = ((w[WT_F1] + w[WT_F2] * c[CR_LGT]) + w[WT_F3] * c[CR_LGT2]) - 26.0;
+    double stim = 1.0 - oco_exp10_t(-(w[WT_CHI * WS] * theta), M);
+    double sm = w[WT_G1 * WS];   /* n = 1: g*1*exp(-0) */
+    sm = sm + w[WT_G2 * WS] * c[CR_E2];
+    sm = sm + w[WT_G3 * WS] * c[CR_E3];
+    sm = sm + w[WT_G4 * WS] * c[CR_E4];
+    sm = sm + w[WT_G5 * WS] * c[CR_E5];
+    sm = sm + w[WT_G6 * WS] * c[CR_E6];
+    double kHbf = (((OCO_AHBF * stim) * fn) * w[WT_WA3 * WS]) * (c[CR_UNSOLD] + sm);
+    double kHmbf = (w[WT_HMBF_ON * WS] != 0.0) ? ((c[CR_HMBF] * stim) * fn) * w[WT_SCOEFF * WS] : 0.0;
+    double flam = ((w[WT_F1 * WS] + w[WT_F2 * WS] * c[CR_LGT]) + w[WT_F3 * WS] * c[CR_LGT2]) - 26.0;
     double kHmff = c[CR_PHMFF] * oco_exp10_t(flam, M);
     double kappa = (((kHbf + kHmbf) + kHmff) + c[CR_ES]) * OCO_MP1;
     double rho = c[CR_RHO];
     double alpha = kappa * rho;
     if (c[CR_NEUTRAL] != 0.0) {
-        double g_ff = 1.0 + w[WT_GFF] * ((c[CR_U0] * w[WT_CHI1]) + 0.5);
-        double kHff = ((((stim * fn) * OCO_ALPHA0) * w[WT_WA3]) * g_ff) * c[CR_U1];
+        double g_ff = 1.0 + w[WT_GFF * WS] * ((c[CR_U0] * w[WT_CHI1 * WS]) + 0.5);
+        double kHff = ((((stim * fn) * OCO_ALPHA0) * w[WT_WA3 * WS]) * g_ff) * c[CR_U1];
         kappa = kappa + kHff * OCO_MP1;
         alpha = alpha + kappa * rho;
     } else {
-        double inner = w[WT_NU9] * c[CR_U0];
+        double inner = w[WT_NU9 * WS] * c[CR_U0];
         double gB = oco_log_t(oco_exp_t(5.960 - (OCO_SQRT3 * OCO_PI1) * oco_log_t(inner, M), M) + OCO_EXP1, M);
-        double lnQ = (c[CR_U1] - w[WT_L3NU]) + oco_log_t(1.0 - oco_exp_t(-(w[WT_HNK] * c[CR_U2]), M), M);
+        double lnQ = (c[CR_U1] - w[WT_L3NU * WS]) + oco_log_t(1.0 - oco_exp_t(-(w[WT_HNK * WS] * c[CR_U2]), M), M);
         double kRho = (OCO_C_BREMS * oco_exp_t(lnQ, M)) * gB;
         alpha = alpha + kRho;
         kappa = kappa + alpha * c[CR_U3];
@@ -185,10 +188,15 @@ OCO_HD static inline oco_pt oco_point(const double* c, const double* w, const oc
     p.alpha = alpha;
     p.alpha1 = 1.0 / alpha;
     p.alpha2 = alpha * alpha;
-    double damp = c[CR_DT1] * w[WT_WCM1];
+    double damp = c[CR_DT1] * w[WT_WCM1 * WS];
     if (damp > OCO_LOG_OVF) damp = OCO_LOG_OVF;
-    p.S = w[WT_SFW5] / (oco_exp_t(damp, M) - 1.0);
+    p.S = w[WT_SFW5 * WS] / (oco_exp_t(damp, M) - 1.0);
     return p;
+}
+
+OCO_HD static inline oco_pt oco_point(const double* c, const double* w, const oco_mtab* M)
+{
+    return oco_point_s<1>(c, w, M);
 }
 
 /* grey_parameters (source/continuous_opacity.f90:377-390) from rho, T. */
