@@ -94,6 +94,31 @@ int oco_solve_block_device(oco_ctx* ctx, int nz, int nyloc, int nxloc, double dz
                            double dz2, const double* d_rho3d, const double* d_T3d, double* d_U,
                            double* d_V, double* d_absorp, int mode, oco_diag* diag);
 
+/* ---- Batched uneven columns ----
+ * Independent columns, each with its own nz, grid step dz and wavelength
+ * grid, solved in one persistent launch.  Replaces running the reference
+ * once per column shape (its shapes are compile-time, source/common.f90:9)
+ * over the column loop body source/flux_feautrier.f90:160-228.
+ * Layout: columns concatenated; rho, T hold sum(nz) values; column c's U is
+ * (nz_c+6, nw_c) and its V, absorp (nz_c, nw_c), each Fortran order,
+ * concatenated in column order (oco_batch_layout gives the totals).
+ * The plan keeps device-resident inputs and outputs (oco_batch_buffers);
+ * its first run autotunes the kernel and grid and captures the solve as a
+ * hipGraph that later runs replay.  fail_ix in the diagnostics is the
+ * 1-based column, fail_iw its wavelength. */
+typedef struct oco_batch oco_batch;
+int oco_batch_create(oco_ctx* ctx, int ncol, const int* nz, const int* nw, const double* waves_angstrom,
+                     const double* dz, oco_batch** plan);
+int oco_batch_destroy(oco_batch* plan);
+int oco_batch_layout(oco_batch* plan, size_t* ncells, size_t* nsys, size_t* nU, size_t* nVA);
+int oco_batch_buffers(oco_batch* plan, double** d_rho, double** d_T, double** d_U, double** d_V,
+                      double** d_absorp);
+int oco_batch_run(oco_batch* plan, int mode, oco_diag* diag);
+int oco_batch_run_host(oco_batch* plan, const double* rho, const double* T, double* U, double* V,
+                       double* absorp, int mode, oco_diag* diag);
+/* Autotuner outcome: kernel variant, workgroups, and whether tuning ran. */
+int oco_batch_tuning(oco_batch* plan, int* variant, int* nblocks, int* tuned);
+
 /* Message text for a status code. */
 const char* oco_strerror(int status);
 

@@ -17,7 +17,7 @@ import os
 
 import numpy as np
 
-__all__ = ["OcoError", "Context", "Diag", "load_library", "LIB_PATH", "MODE_FP64", "MODE_MIXED",
+__all__ = ["OcoError", "Context", "Batch", "Diag", "load_library", "LIB_PATH", "MODE_FP64", "MODE_MIXED",
            "EXPORTS"]
 
 PKG = os.path.dirname(os.path.abspath(__file__))
@@ -28,7 +28,9 @@ MODE_FP64, MODE_MIXED = 0, 1
 
 # every symbol include/ocotillo.h declares
 EXPORTS = ["oco_create", "oco_destroy", "oco_set_wavelengths", "oco_set_physics", "oco_solve_block",
-           "oco_solve_block_device", "oco_strerror", "oco_version"]
+           "oco_solve_block_device", "oco_strerror", "oco_version",
+           "oco_batch_create", "oco_batch_destroy", "oco_batch_layout", "oco_batch_buffers", "oco_batch_run",
+           "oco_batch_run_host", "oco_batch_tuning"]
 
 
 class Diag(C.Structure):
@@ -68,6 +70,16 @@ def load_library(path=None):
     L.oco_solve_block_device.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double,
                                          C.c_double, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                          C.c_void_p, C.c_int, C.POINTER(Diag)]
+    _SZ = C.POINTER(C.c_size_t)
+    _PP = C.POINTER(C.c_void_p)
+    L.oco_batch_create.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int), _DP, _DP,
+                                   C.POINTER(C.c_void_p)]
+    L.oco_batch_destroy.argtypes = [C.c_void_p]
+    L.oco_batch_layout.argtypes = [C.c_void_p, _SZ, _SZ, _SZ, _SZ]
+    L.oco_batch_buffers.argtypes = [C.c_void_p, _PP, _PP, _PP, _PP, _PP]
+    L.oco_batch_run.argtypes = [C.c_void_p, C.c_int, C.POINTER(Diag)]
+    L.oco_batch_run_host.argtypes = [C.c_void_p, _DP, _DP, _DP, _DP, _DP, C.c_int, C.POINTER(Diag)]
+    L.oco_batch_tuning.argtypes = [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]
     L.oco_strerror.restype = C.c_char_p
     L.oco_strerror.argtypes = [C.c_int]
     L.oco_version.restype = C.c_char_p
@@ -162,3 +174,85 @@ class Context:
         if st != OK and (raise_on_pivot or st not in (E_PIVOT0_STAGE1, E_PIVOT0_STAGE2)):
             raise OcoError(st, "oco_solve_block_device: " + strerror(st), d)
         return d
+
+
+class Batch:
+    """A plan for many independent columns of uneven shape (oco_batch_*).
+
+    nz, nw: per-column cell and wavelength counts; waves: all columns'
+    wavelength grids concatenated (Angstrom); dz: per-column grid step.
+    Inputs rho, T are the columns concatenated (sum(nz) values); outputs are
+    per column U (nw_c, nz_c+6), V and absorp (nw_c, nz_c) -- the Fortran
+    (z, iw) layout -- concatenated.  The first run autotunes and captures a
+    hipGraph; later runs replay it."""
+
+    def __init__(self, ctx, nz, nw, waves, dz):
+        self.ctx = ctx
+        self.L = ctx.L
+        self.nz = np.ascontiguousarray(nz, dtype=np.int32)
+        self.nw = np.ascontiguousarray(nw, dtype=np.int32)
+        w = np.ascontiguousarray(waves, dtype=np.float64)
+        d = np.ascontiguousarray(dz, dtype=np.float64)
+        if w.size != int(self.nw.sum()) or d.size != self.nz.size or self.nw.size != self.nz.size:
+            raise OcoError(E_ARG, "Batch: nz, nw, dz must have one entry per column and waves sum(nw)")
+        h = C.c_void_p()
+        st = self.L.oco_batch_create(ctx.h, self.nz.size, self.nz.ctypes.data_as(C.POINTER(C.c_int)),
+                                     self.nw.ctypes.data_as(C.POINTER(C.c_int)), _ptr(w), _ptr(d), C.byref(h))
+        if st != OK:
+            raise OcoError(st, "oco_batch_create: " + strerror(st))
+        self.h = h
+        sizes = [C.c_size_t() for _ in range(4)]
+        self.L.oco_batch_layout(h, *[C.byref(x) for x in sizes])
+        self.ncells, self.nsys, self.nU, self.nVA = [x.value for x in sizes]
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.oco_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def buffers(self):
+        """Device pointers (rho, T, U, V, absorp) of the plan's resident arrays."""
+        p = [C.c_void_p() for _ in range(5)]
+        self.L.oco_batch_buffers(self.h, *[C.byref(x) for x in p])
+        return [x.value for x in p]
+
+    def tuning(self):
+        v, nb, t = C.c_int(), C.c_int(), C.c_int()
+        self.L.oco_batch_tuning(self.h, C.byref(v), C.byref(nb), C.byref(t))
+        return v.value, nb.value, t.value
+
+    def run(self, mode=MODE_FP64, raise_on_pivot=True):
+        d = Diag()
+        st = self.L.oco_batch_run(self.h, mode, C.byref(d))
+        if st != OK and (raise_on_pivot or st not in (E_PIVOT0_STAGE1, E_PIVOT0_STAGE2)):
+            raise OcoError(st, "oco_batch_run: " + strerror(st), d)
+        return d
+
+    def run_host(self, rho, T, mode=MODE_FP64, raise_on_pivot=True):
+        rho = np.ascontiguousarray(rho, dtype=np.float64)
+        T = np.ascontiguousarray(T, dtype=np.float64)
+        if rho.size != self.ncells or T.size != self.ncells:
+            raise OcoError(E_ARG, "Batch.run_host: rho and T need sum(nz) values")
+        U = np.empty(self.nU)
+        V = np.empty(self.nVA)
+        A = np.empty(self.nVA)
+        d = Diag()
+        st = self.L.oco_batch_run_host(self.h, _ptr(rho), _ptr(T), _ptr(U), _ptr(V), _ptr(A), mode, C.byref(d))
+        if st != OK and (raise_on_pivot or st not in (E_PIVOT0_STAGE1, E_PIVOT0_STAGE2)):
+            raise OcoError(st, "oco_batch_run_host: " + strerror(st), d)
+        return U, V, A, d
+
+    def split(self, U, V, A):
+        """Per-column views: U[c] is (nw_c, nz_c+6), V[c], A[c] are (nw_c, nz_c)."""
+        outU, outV, outA = [], [], []
+        u = v = 0
+        for nz, nw in zip(self.nz.tolist(), self.nw.tolist()):
+            outU.append(U[u:u + (nz + 6) * nw].reshape(nw, nz + 6))
+            outV.append(V[v:v + nz * nw].reshape(nw, nz))
+            outA.append(A[v:v + nz * nw].reshape(nw, nz))
+            u += (nz + 6) * nw
+            v += nz * nw
+        return outU, outV, outA

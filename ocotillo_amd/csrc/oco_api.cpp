@@ -2,10 +2,12 @@
  * oco_api.cpp -- the C ABI (include/ocotillo.h): context, resident
  * wavelength tables, device buffers and the block solve.
  */
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <new>
+#include <numeric>
 #include <vector>
 
 #include "../../include/ocotillo.h"
@@ -360,6 +362,313 @@ int oco_solve_block(oco_ctx* c, int nz, int nyloc, int nxloc, double dz, double 
     HIPCHK(hipMemcpyAsync(U, c->U.p, nU * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipMemcpyAsync(V, c->V.p, nout * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipMemcpyAsync(absorp, c->A.p, nout * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return st;
+}
+
+/* ---------------------------------------------------------------------- */
+/* Batched uneven columns                                                  */
+/* ---------------------------------------------------------------------- */
+
+struct oco_batch {
+    oco_ctx* ctx = nullptr;
+    int ncol = 0, max_nz = 0, ntiles = 0;
+    long ncells = 0, nsys = 0, nU = 0, nVA = 0;
+    std::vector<int> nz, nw;
+    std::vector<long> woff;
+    DevBuf ints, longs, dz, wtab, rho, T, cells, U, V, A, scratch, counter, fail;
+    int nblocks = 0, variant = 1, tuned = 0;
+    float tune_ms[16] = {0};
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
+static void batch_release(oco_batch* b)
+{
+    oco_ctx* c = b->ctx;
+    for (DevBuf* d : {&b->ints, &b->longs, &b->dz, &b->wtab, &b->rho, &b->T, &b->cells, &b->U, &b->V, &b->A,
+                      &b->scratch, &b->counter, &b->fail})
+        release(c, *d);
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    b->exec = nullptr;
+    b->graph = nullptr;
+}
+
+static oco_batch_args batch_args(oco_batch* b)
+{
+    oco_batch_args a;
+    const int ncol = b->ncol;
+    a.ntiles = b->ntiles;
+    a.nz = (const int*)b->ints.p;
+    a.nw = a.nz + ncol;
+    a.tiles = a.nw + ncol;
+    a.cell_off = (const long*)b->longs.p;
+    a.wt_off = a.cell_off + ncol;
+    a.u_off = a.wt_off + ncol;
+    a.v_off = a.u_off + ncol;
+    a.dz = (const double*)b->dz.p;
+    a.cells = (const double*)b->cells.p;
+    a.rho = (const double*)b->rho.p;
+    a.T = (const double*)b->T.p;
+    a.wtab = (const double*)b->wtab.p;
+    a.U = (double*)b->U.p;
+    a.V = (double*)b->V.p;
+    a.A = (double*)b->A.p;
+    a.scratch = (double*)b->scratch.p;
+    a.scr_stride = (long)b->max_nz * 3 * 64;
+    a.counter = (unsigned int*)b->counter.p;
+    a.fail = (unsigned long long*)b->fail.p;
+    return a;
+}
+
+/* enqueue one full batch solve with the given kernel variant / grid */
+static hipError_t batch_enqueue(oco_batch* b, int variant, int nblocks)
+{
+    oco_ctx* c = b->ctx;
+    hipError_t e = oco_launch_cells((const double*)b->rho.p, (const double*)b->T.p, b->ncells, c->fully_ionized_T,
+                                    (double*)b->cells.p, c->stream);
+    if (e != hipSuccess) return e;
+    return oco_launch_batch(batch_args(b), variant, nblocks, c->stream);
+}
+
+int oco_batch_create(oco_ctx* c, int ncol, const int* nz, const int* nw, const double* waves, const double* dz,
+                     oco_batch** out)
+{
+    if (!c || !out || ncol <= 0 || !nz || !nw || !waves || !dz) return OCO_E_ARG;
+    if (c->lgrey) return OCO_E_ARG;
+    *out = nullptr;
+    oco_batch* b = new (std::nothrow) oco_batch;
+    if (!b) return OCO_E_OOM;
+    b->ctx = c;
+    b->ncol = ncol;
+    b->nz.assign(nz, nz + ncol);
+    b->nw.assign(nw, nw + ncol);
+    std::vector<long> coff(ncol), woff(ncol), uoff(ncol), voff(ncol);
+    std::vector<double> dz3(3 * (size_t)ncol);
+    long nc = 0, ns = 0, nU = 0, nVA = 0;
+    for (int i = 0; i < ncol; i++) {
+        if (nz[i] < 4 || nw[i] < 1 || !(dz[i] > 0.0)) {
+            delete b;
+            return OCO_E_ARG;
+        }
+        coff[i] = nc;
+        woff[i] = ns;
+        uoff[i] = nU;
+        voff[i] = nVA;
+        nc += nz[i];
+        ns += nw[i];
+        nU += (long)(nz[i] + 6) * nw[i];
+        nVA += (long)nz[i] * nw[i];
+        b->max_nz = std::max(b->max_nz, nz[i]);
+        dz3[3 * i] = dz[i];
+        dz3[3 * i + 1] = 1.0 / dz[i];
+        dz3[3 * i + 2] = dz[i] * dz[i];
+    }
+    for (long i = 0; i < ns; i++)
+        if (!(waves[i] > 0.0) || !std::isfinite(waves[i])) {
+            delete b;
+            return OCO_E_ARG;
+        }
+    b->ncells = nc;
+    b->nsys = ns;
+    b->nU = nU;
+    b->nVA = nVA;
+    b->woff = woff;
+    /* tiles of 64 wavelengths, longest columns first (LPT order) */
+    std::vector<int> order(ncol);
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return nz[x] > nz[y]; });
+    std::vector<int> tiles;
+    for (int i : order)
+        for (int w0 = 0; w0 < nw[i]; w0 += 64) {
+            tiles.push_back(i);
+            tiles.push_back(w0);
+        }
+    b->ntiles = (int)(tiles.size() / 2);
+    /* wavelength tables, one grid per column */
+    std::vector<double> wt((size_t)ns * OCO_WT_STRIDE), one;
+    for (int i = 0; i < ncol; i++) {
+        std::vector<double> g(waves + woff[i], waves + woff[i] + nw[i]);
+        build_wtab(g, one);
+        std::copy(one.begin(), one.end(), wt.begin() + woff[i] * OCO_WT_STRIDE);
+    }
+    int dev_cus = 256;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, c->device) == hipSuccess && prop.multiProcessorCount > 0)
+        dev_cus = prop.multiProcessorCount;
+    b->nblocks = std::min(b->ntiles, dev_cus * 4 * 3);
+    const int max_blocks = dev_cus * 4 * 4;
+    int st = OCO_OK;
+    std::vector<int> ints;
+    ints.insert(ints.end(), nz, nz + ncol);
+    ints.insert(ints.end(), nw, nw + ncol);
+    ints.insert(ints.end(), tiles.begin(), tiles.end());
+    std::vector<long> longs;
+    for (auto* v : {&coff, &woff, &uoff, &voff}) longs.insert(longs.end(), v->begin(), v->end());
+    if (!st) st = ensure(c, b->ints, ints.size() * sizeof(int));
+    if (!st) st = ensure(c, b->longs, longs.size() * sizeof(long));
+    if (!st) st = ensure(c, b->dz, dz3.size() * sizeof(double));
+    if (!st) st = ensure(c, b->wtab, wt.size() * sizeof(double));
+    if (!st) st = ensure(c, b->rho, nc * sizeof(double));
+    if (!st) st = ensure(c, b->T, nc * sizeof(double));
+    if (!st) st = ensure(c, b->cells, (size_t)nc * OCO_CELL_STRIDE * sizeof(double));
+    if (!st) st = ensure(c, b->U, nU * sizeof(double));
+    if (!st) st = ensure(c, b->V, nVA * sizeof(double));
+    if (!st) st = ensure(c, b->A, nVA * sizeof(double));
+    if (!st) st = ensure(c, b->scratch, (size_t)std::min(b->ntiles, max_blocks) * b->max_nz * 3 * 64 * sizeof(double));
+    if (!st) st = ensure(c, b->counter, sizeof(unsigned int));
+    if (!st) st = ensure(c, b->fail, sizeof(unsigned long long));
+    if (!st) {
+        hipError_t e = hipMemcpyAsync(b->ints.p, ints.data(), ints.size() * sizeof(int), hipMemcpyHostToDevice, c->stream);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(b->longs.p, longs.data(), longs.size() * sizeof(long), hipMemcpyHostToDevice, c->stream);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(b->dz.p, dz3.data(), dz3.size() * sizeof(double), hipMemcpyHostToDevice, c->stream);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(b->wtab.p, wt.data(), wt.size() * sizeof(double), hipMemcpyHostToDevice, c->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+        st = hip_status(e);
+    }
+    if (st) {
+        batch_release(b);
+        delete b;
+        return st;
+    }
+    *out = b;
+    return OCO_OK;
+}
+
+int oco_batch_destroy(oco_batch* b)
+{
+    if (!b) return OCO_OK;
+    (void)hipSetDevice(b->ctx->device);
+    (void)hipStreamSynchronize(b->ctx->stream);
+    batch_release(b);
+    delete b;
+    return OCO_OK;
+}
+
+int oco_batch_layout(oco_batch* b, size_t* ncells, size_t* nsys, size_t* nU, size_t* nVA)
+{
+    if (!b) return OCO_E_ARG;
+    if (ncells) *ncells = (size_t)b->ncells;
+    if (nsys) *nsys = (size_t)b->nsys;
+    if (nU) *nU = (size_t)b->nU;
+    if (nVA) *nVA = (size_t)b->nVA;
+    return OCO_OK;
+}
+
+int oco_batch_buffers(oco_batch* b, double** rho, double** T, double** U, double** V, double** absorp)
+{
+    if (!b) return OCO_E_ARG;
+    if (rho) *rho = (double*)b->rho.p;
+    if (T) *T = (double*)b->T.p;
+    if (U) *U = (double*)b->U.p;
+    if (V) *V = (double*)b->V.p;
+    if (absorp) *absorp = (double*)b->A.p;
+    return OCO_OK;
+}
+
+int oco_batch_tuning(oco_batch* b, int* variant, int* nblocks, int* tuned)
+{
+    if (!b) return OCO_E_ARG;
+    if (variant) *variant = b->variant;
+    if (nblocks) *nblocks = b->nblocks;
+    if (tuned) *tuned = b->tuned;
+    return OCO_OK;
+}
+
+/* first run: time every kernel variant x grid size once and keep the best;
+ * then capture the whole solve (reset, cells, sweep) as one hipGraph */
+static int batch_prepare(oco_batch* b)
+{
+    oco_ctx* c = b->ctx;
+    if (!b->tuned) {
+        int dev_cus = 256;
+        hipDeviceProp_t prop;
+        if (hipGetDeviceProperties(&prop, c->device) == hipSuccess && prop.multiProcessorCount > 0)
+            dev_cus = prop.multiProcessorCount;
+        float best = 1e30f;
+        int k = 0;
+        for (int v = 0; v < oco_batch_variants(); v++) {
+            for (int wps = 2; wps <= 4; wps++) {
+                int nb = std::min(b->ntiles, dev_cus * 4 * wps);
+                HIPCHK(hipEventRecord(c->ev[0], c->stream));
+                HIPCHK(batch_enqueue(b, v, nb));
+                HIPCHK(hipEventRecord(c->ev[1], c->stream));
+                HIPCHK(hipEventSynchronize(c->ev[1]));
+                float ms = 0.f;
+                HIPCHK(hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
+                if (k < 16) b->tune_ms[k++] = ms;
+                if (ms < best) {
+                    best = ms;
+                    b->variant = v;
+                    b->nblocks = nb;
+                }
+            }
+        }
+        b->tuned = 1;
+    }
+    if (!b->exec) {
+        HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        hipError_t e = batch_enqueue(b, b->variant, b->nblocks);
+        hipGraph_t g = nullptr;
+        hipError_t e2 = hipStreamEndCapture(c->stream, &g);
+        if (e != hipSuccess || e2 != hipSuccess) {
+            if (g) (void)hipGraphDestroy(g);
+            return OCO_E_HIP;
+        }
+        b->graph = g;
+        HIPCHK(hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0));
+    }
+    return OCO_OK;
+}
+
+int oco_batch_run(oco_batch* b, int mode, oco_diag* diag)
+{
+    if (!b) return OCO_E_ARG;
+    if (mode != OCO_MODE_FP64) return OCO_E_ARG;
+    oco_ctx* c = b->ctx;
+    HIPCHK(hipSetDevice(c->device));
+    int st = batch_prepare(b);
+    if (st) return st;
+    HIPCHK(hipEventRecord(c->ev[0], c->stream));
+    HIPCHK(hipGraphLaunch(b->exec, c->stream));
+    HIPCHK(hipEventRecord(c->ev[2], c->stream));
+    unsigned long long key = OCO_FAIL_NONE;
+    HIPCHK(hipMemcpyAsync(&key, b->fail.p, sizeof(key), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    oco_diag d;
+    memset(&d, 0, sizeof(d));
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, c->ev[0], c->ev[2]) == hipSuccess) d.total_ms = d.sweep_ms = ms;
+    if (key != OCO_FAIL_NONE) {
+        long sys = (long)(key >> 32);
+        d.status = (int)((key >> 28) & 0xf);
+        d.fail_j = (int)(key & 0x0fffffffULL);
+        int col = (int)(std::upper_bound(b->woff.begin(), b->woff.end(), sys) - b->woff.begin()) - 1;
+        d.fail_ix = col + 1;
+        d.fail_iy = 1;
+        d.fail_iw = (int)(sys - b->woff[col]) + 1;
+    }
+    if (diag) *diag = d;
+    return d.status;
+}
+
+int oco_batch_run_host(oco_batch* b, const double* rho, const double* T, double* U, double* V, double* absorp,
+                       int mode, oco_diag* diag)
+{
+    if (!b || !rho || !T || !U || !V || !absorp) return OCO_E_ARG;
+    oco_ctx* c = b->ctx;
+    HIPCHK(hipSetDevice(c->device));
+    HIPCHK(hipMemcpyAsync(b->rho.p, rho, b->ncells * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(b->T.p, T, b->ncells * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    int st = oco_batch_run(b, mode, diag);
+    if (st == OCO_E_ARG || st == OCO_E_HIP || st == OCO_E_OOM) return st;
+    HIPCHK(hipMemcpyAsync(U, b->U.p, b->nU * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(V, b->V.p, b->nVA * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(absorp, b->A.p, b->nVA * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
     return st;
 }

@@ -24,7 +24,33 @@ struct oco_sweep_args {
     double* canary;      /* 4 + 5*nz doubles: sum(a..d) of system 0, then its point values */
 };
 
+/* Batched uneven columns (per-column shapes and wavelength grids). */
+struct oco_batch_args {
+    int ntiles;
+    const int* tiles;      /* ntiles x (column, first wavelength) */
+    const int* nz;         /* per column */
+    const int* nw;
+    const double* dz;      /* per column: dz, 1/dz, dz**2 */
+    const long* cell_off;  /* per column: first cell (inputs and cell records) */
+    const long* wt_off;    /* per column: first wavelength-table row (= first system) */
+    const long* u_off;     /* per column: offset of U (nz+6, nw) */
+    const long* v_off;     /* per column: offset of V and alpha (nz, nw) */
+    const double* cells;
+    const double* rho;
+    const double* T;
+    const double* wtab;
+    double* U;
+    double* V;
+    double* A;
+    double* scratch;
+    long scr_stride;       /* doubles per wavefront slot: max_nz * 3 * 64 */
+    unsigned int* counter;
+    unsigned long long* fail;
+};
+
 /* Launchers (enqueue on `stream`, no synchronisation). */
+hipError_t oco_launch_batch(const oco_batch_args& a, int variant, int nblocks, hipStream_t stream);
+int oco_batch_variants(void);
 hipError_t oco_launch_cells(const double* rho, const double* T, long ncells, double fully_ionized_T,
                             double* cells, hipStream_t stream);
 hipError_t oco_launch_sweep(const oco_sweep_args& a, int lgrey, int nblocks, hipStream_t stream);

@@ -58,18 +58,37 @@ __global__ __launch_bounds__(256) void cells_fp64(const double* __restrict__ rho
 typedef const __attribute__((address_space(4))) double* oco_cptr;
 
 template <bool UNIFORM>
-__device__ static inline void load_record(const oco_sweep_args& a, long col, int z, double* r)
+__device__ static inline void load_rec(const double* cells, int z, double* r)
 {
     if (UNIFORM) {
-        oco_cptr p = (oco_cptr)(a.cells + (col * a.nz + z) * OCO_CELL_STRIDE);
+        oco_cptr p = (oco_cptr)(cells + (long)z * OCO_CELL_STRIDE);
 #pragma unroll
         for (int k = 0; k < CR_NFIELD; k++) r[k] = p[k];
     } else {
-        const double* p = a.cells + (col * a.nz + z) * OCO_CELL_STRIDE;
+        const double* p = cells + (long)z * OCO_CELL_STRIDE;
 #pragma unroll
         for (int k = 0; k < CR_NFIELD; k++) r[k] = p[k];
     }
 }
+
+template <bool UNIFORM>
+__device__ static inline void load_record(const oco_sweep_args& a, long col, int z, double* r)
+{
+    load_rec<UNIFORM>(a.cells + col * a.nz * OCO_CELL_STRIDE, z, r);
+}
+
+/* One (column, wavelength) system as the solver sees it. */
+struct oco_sys {
+    int nz;
+    double dz, dz1, dz2, sigma_grey;
+    const double* cells;   /* the column's cell records */
+    const double* rho;     /* the column's rho, T (grey case) */
+    const double* T;
+    double* U;             /* this system's U (nz+6), V (nz), alpha (nz) */
+    double* V;
+    double* A;
+    long key;              /* position in the reference's loop order */
+};
 
 __device__ static inline void record_fail(unsigned long long* fail, long sys, int stage, int row)
 {
@@ -110,6 +129,14 @@ __device__ static inline oco_pt point_rec(const oco_sweep_args& a, long col, int
     return oco_point_s<WS>(rec, w, M);
 }
 
+template <bool GREY, int WS = 1>
+__device__ static inline oco_pt point_sys(const oco_sys& d, int z, const double* rec, const double* w,
+                                          const oco_mtab* M)
+{
+    if (GREY) return oco_grey_point(d.rho[z], d.T[z], d.sigma_grey, M);
+    return oco_point_s<WS>(rec, w, M);
+}
+
 /* Tuning knob: OCO_SWEEP_WAVES_PER_EU=n asks the compiler to fit n waves per
  * SIMD (register budget 512/n VGPRs). */
 #ifdef OCO_SWEEP_WAVES_PER_EU
@@ -122,29 +149,26 @@ __device__ static inline oco_pt point_rec(const oco_sweep_args& a, long col, int
  * opacity of the cells recomputed in registers, then back-substitution fused
  * with ghosts, der and V.  `w` is the lane's wavelength-table row with
  * stride WS; `scr` its scratch column (stride 64 doubles per entry). */
-template <bool GREY, bool UNIFORM, int WS>
-__device__ static inline void solve_system(const oco_sweep_args& a, long col, int iw, const double* wbase,
-                                           int woff, double* scr, const oco_mtab* M)
+template <bool GREY, bool UNIFORM, int WS, int BC = OCO_BACK_CHUNK>
+__device__ static inline void solve_system(const oco_sys& d, const double* w, double* scr,
+                                           unsigned long long* fail, const oco_mtab* M)
 {
-    const double* w = wbase + woff;
-    const int nz = a.nz, nw = a.nw;
-    const double dz = a.dz, dz1 = a.dz1, dz2 = a.dz2;
-    const long sys = col * nw + iw;
-    const long osys = (long)iw * a.ncol + col;
-    double* Uo = a.U + osys * (nz + 6);
-    double* Vo = a.V + osys * nz;
-    double* Ao = a.A + osys * nz;
+    const int nz = d.nz;
+    const double dz = d.dz, dz1 = d.dz1, dz2 = d.dz2;
+    double* Uo = d.U;
+    double* Vo = d.V;
+    double* Ao = d.A;
 
     /* ---- forward elimination (source/auxiliary.f90:25-41) ----
      * Two-deep pipeline: iteration j eliminates row j (which needs the
      * points j-1, j, j+1, all computed earlier) and evaluates point j+2,
      * whose work is independent of the elimination chain. */
     double rec[CR_NFIELD];
-    if (!GREY) load_record<UNIFORM>(a, col, 0, rec);
-    oco_pt pB = point_rec<GREY, UNIFORM, WS>(a, col, 0, rec, w, M);
-    if (!GREY) load_record<UNIFORM>(a, col, 1, rec);
-    oco_pt pC = point_rec<GREY, UNIFORM, WS>(a, col, 1, rec, w, M);
-    if (!GREY) load_record<UNIFORM>(a, col, 2, rec);
+    if (!GREY) load_rec<UNIFORM>(d.cells, 0, rec);
+    oco_pt pB = point_sys<GREY, WS>(d, 0, rec, w, M);
+    if (!GREY) load_rec<UNIFORM>(d.cells, 1, rec);
+    oco_pt pC = point_sys<GREY, WS>(d, 1, rec, w, M);
+    if (!GREY) load_rec<UNIFORM>(d.cells, 2, rec);
 
     double bv, dv, av, cv;
     boundary_row(pB, dz, dz2, bv, dv);      /* row 0: a = 0, c = alpha */
@@ -159,10 +183,10 @@ __device__ static inline void solve_system(const oco_sweep_args& a, long col, in
     Ao[0] = pB.alpha;
     double aA = pB.alpha;   /* alpha of row j-1 */
     pB = pC;                /* row j = 1 */
-    pC = point_rec<GREY, UNIFORM, WS>(a, col, 2, rec, w, M);
+    pC = point_sys<GREY, WS>(d, 2, rec, w, M);
 
     for (int j = 1; j <= nz - 3; j++) {
-        if (!GREY) load_record<UNIFORM>(a, col, j + 2, rec);
+        if (!GREY) load_rec<UNIFORM>(d.cells, j + 2, rec);
         /* row j and its elimination step */
         double cprev = cv;
         interior_row(aA, pB, pC.alpha, dz2, av, bv, cv, dv);
@@ -178,7 +202,7 @@ __device__ static inline void solve_system(const oco_sweep_args& a, long col, in
         sp[2 * OCO_WAVE] = pB.alpha1;
         Ao[j] = pB.alpha;
         /* point j+2 (independent of the chain above) */
-        oco_pt pD = point_rec<GREY, UNIFORM, WS>(a, col, j + 2, rec, w, M);
+        oco_pt pD = point_sys<GREY, WS>(d, j + 2, rec, w, M);
         aA = pB.alpha;
         pB = pC;
         pC = pD;
@@ -216,7 +240,7 @@ __device__ static inline void solve_system(const oco_sweep_args& a, long col, in
     }
     if (fail_stage != 0) {
         /* the reference stops at this system (source/auxiliary.f90:27-39) */
-        record_fail(a.fail, sys, fail_stage, fail_row);
+        record_fail(fail, d.key, fail_stage, fail_row);
         return;
     }
 
@@ -251,28 +275,28 @@ __device__ static inline void solve_system(const oco_sweep_args& a, long col, in
         double df = c60 * (((45.0 * (w4 - w2)) - (9.0 * (w5 - w1))) + (w6 - w0));
         Vo[nz - 1] = ((-1.0 * a1q3) * df) * dz1;
     }
-    /* rows nz-5 .. 0 in chunks of OCO_BACK_CHUNK: each chunk's scratch
+    /* rows nz-5 .. 0 in chunks of BC: each chunk's scratch
      * entries are loaded together, one chunk ahead of their use, so the loop
      * pays one memory latency per chunk instead of one per row */
     {
-        double cg[OCO_BACK_CHUNK], cu[OCO_BACK_CHUNK], ca[OCO_BACK_CHUNK];
-        double ng[OCO_BACK_CHUNK], nu[OCO_BACK_CHUNK], na[OCO_BACK_CHUNK];
+        double cg[BC], cu[BC], ca[BC];
+        double ng[BC], nu[BC], na[BC];
         int jc = nz - 5;
 #pragma unroll
-        for (int k = 0; k < OCO_BACK_CHUNK; k++) {
+        for (int k = 0; k < BC; k++) {
             int j = jc - k;
             const double* sp = scr + (long)(j < 0 ? 0 : j) * 3 * OCO_WAVE;
             cg[k] = sp[0]; cu[k] = sp[OCO_WAVE]; ca[k] = sp[2 * OCO_WAVE];
         }
-        for (; jc >= 0; jc -= OCO_BACK_CHUNK) {
+        for (; jc >= 0; jc -= BC) {
 #pragma unroll
-            for (int k = 0; k < OCO_BACK_CHUNK; k++) {      /* next chunk, in flight */
-                int j = jc - OCO_BACK_CHUNK - k;
+            for (int k = 0; k < BC; k++) {      /* next chunk, in flight */
+                int j = jc - BC - k;
                 const double* sp = scr + (long)(j < 0 ? 0 : j) * 3 * OCO_WAVE;
                 ng[k] = sp[0]; nu[k] = sp[OCO_WAVE]; na[k] = sp[2 * OCO_WAVE];
             }
 #pragma unroll
-            for (int k = 0; k < OCO_BACK_CHUNK; k++) {
+            for (int k = 0; k < BC; k++) {
                 int j = jc - k;
                 if (j >= 0) {
                     double Uj = cu[k] - gnext * w0;
@@ -285,7 +309,7 @@ __device__ static inline void solve_system(const oco_sweep_args& a, long col, in
                 }
             }
 #pragma unroll
-            for (int k = 0; k < OCO_BACK_CHUNK; k++) { cg[k] = ng[k]; cu[k] = nu[k]; ca[k] = na[k]; }
+            for (int k = 0; k < BC; k++) { cg[k] = ng[k]; cu[k] = nu[k]; ca[k] = na[k]; }
         }
     }
     /* bottom ghosts (f(n1-i) = 2 f(n1) - f(n1+i)); window now e = 3..9 */
@@ -334,7 +358,18 @@ __global__ __launch_bounds__(OCO_WAVE) OCO_SWEEP_ATTR void sweep_fp64(const oco_
 #pragma unroll
             for (int k = 0; k < OCO_WT_STRIDE; k++) w[k] = a.wtab[iw * OCO_WT_STRIDE + k];
         }
-        solve_system<GREY, UNIFORM, 1>(a, col, iw, w, 0, scr, &smt);
+        const long osys = (long)iw * a.ncol + col;
+        oco_sys d;
+        d.nz = a.nz;
+        d.dz = a.dz; d.dz1 = a.dz1; d.dz2 = a.dz2; d.sigma_grey = a.sigma_grey;
+        d.cells = a.cells + col * a.nz * OCO_CELL_STRIDE;
+        d.rho = a.rho + col * a.nz;
+        d.T = a.T + col * a.nz;
+        d.U = a.U + osys * (a.nz + 6);
+        d.V = a.V + osys * a.nz;
+        d.A = a.A + osys * a.nz;
+        d.key = col * nw + iw;
+        solve_system<GREY, UNIFORM, 1>(d, w, scr, a.fail, &smt);
     }
 }
 
@@ -392,6 +427,85 @@ __global__ __launch_bounds__(256) void canary_fp64(const oco_sweep_args a, doubl
     }
     if (tid < 4) buf[tid] = red[tid][0];
 }
+
+/* ---- batched uneven columns ----
+ * Tiles are (column, block of 64 wavelengths), sorted longest column first
+ * on the host; a persistent grid draws them from an atomic counter, so
+ * waves that finish short columns take the next tile (dynamic balancing).
+ * Every column carries its own nz, grid step and wavelength grid. */
+__global__ void batch_reset(unsigned int* counter, unsigned long long* fail)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        *counter = 0u;
+        *fail = OCO_FAIL_NONE;
+    }
+}
+
+typedef const __attribute__((address_space(4))) int* oco_cptr_i;
+typedef const __attribute__((address_space(4))) long* oco_cptr_l;
+typedef const __attribute__((address_space(4))) double* oco_cptr_d;
+
+template <int BC>
+__global__ __launch_bounds__(OCO_WAVE) void batch_fp64(const oco_batch_args a)
+{
+    __shared__ oco_mtab smt;
+    __shared__ int tile_s;
+    load_mtab(&smt);
+    const int lane = threadIdx.x;
+    double* scr = a.scratch + (long)blockIdx.x * a.scr_stride + lane;
+    oco_cptr_i tiles = (oco_cptr_i)a.tiles;
+    oco_cptr_i nzs = (oco_cptr_i)a.nz;
+    oco_cptr_i nws = (oco_cptr_i)a.nw;
+    oco_cptr_l coff = (oco_cptr_l)a.cell_off;
+    oco_cptr_l woff = (oco_cptr_l)a.wt_off;
+    oco_cptr_l uoff = (oco_cptr_l)a.u_off;
+    oco_cptr_l voff = (oco_cptr_l)a.v_off;
+    oco_cptr_d dzs = (oco_cptr_d)a.dz;
+    for (;;) {
+        if (lane == 0) tile_s = (int)atomicAdd(a.counter, 1u);
+        __syncthreads();
+        int t = __builtin_amdgcn_readfirstlane(tile_s);
+        __syncthreads();
+        if (t >= a.ntiles) break;
+        const int col = tiles[2 * t];
+        const int iw = tiles[2 * t + 1] + lane;
+        const int nz = nzs[col];
+        const int nw = nws[col];
+        if (iw < nw) {
+            oco_sys d;
+            d.nz = nz;
+            d.dz = dzs[3 * col];
+            d.dz1 = dzs[3 * col + 1];
+            d.dz2 = dzs[3 * col + 2];
+            d.sigma_grey = 0.0;
+            d.cells = a.cells + coff[col] * OCO_CELL_STRIDE;
+            d.rho = a.rho + coff[col];
+            d.T = a.T + coff[col];
+            d.U = a.U + uoff[col] + (long)iw * (nz + 6);
+            d.V = a.V + voff[col] + (long)iw * nz;
+            d.A = a.A + voff[col] + (long)iw * nz;
+            d.key = woff[col] + iw;
+            double w[OCO_WT_STRIDE];
+            const double* wr = a.wtab + (woff[col] + iw) * OCO_WT_STRIDE;
+#pragma unroll
+            for (int k = 0; k < OCO_WT_STRIDE; k++) w[k] = wr[k];
+            solve_system<false, true, 1, BC>(d, w, scr, a.fail, &smt);
+        }
+    }
+}
+
+hipError_t oco_launch_batch(const oco_batch_args& a, int variant, int nblocks, hipStream_t stream)
+{
+    hipLaunchKernelGGL(batch_reset, dim3(1), dim3(64), 0, stream, a.counter, a.fail);
+    switch (variant) {
+    case 0: hipLaunchKernelGGL((batch_fp64<1>), dim3(nblocks), dim3(OCO_WAVE), 0, stream, a); break;
+    case 1: hipLaunchKernelGGL((batch_fp64<2>), dim3(nblocks), dim3(OCO_WAVE), 0, stream, a); break;
+    default: hipLaunchKernelGGL((batch_fp64<4>), dim3(nblocks), dim3(OCO_WAVE), 0, stream, a); break;
+    }
+    return hipGetLastError();
+}
+
+int oco_batch_variants(void) { return 3; }
 
 hipError_t oco_launch_cells(const double* rho, const double* T, long ncells, double fully_ionized_T,
                             double* cells, hipStream_t stream)

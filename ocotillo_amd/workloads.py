@@ -62,3 +62,24 @@ def config2(nx=64, ny=64, nz=256, nw=128):
     100 A - 10 um, disk_columns profiles.  Returns rho, T, dz, waves."""
     rho, T, dz = disk_columns(nx, ny, nz, seed=SEED + 2)
     return rho, T, dz, log_waves(nw)
+
+
+def config3(ncol=4096, seed=SEED + 3, nz_range=(32, 1024), nw_range=(16, 256), sigma_dex=0.5):
+    """Batched uneven columns: nz ~ U{32..1024}, nw ~ U{16..256} (each column
+    its own log-spaced 100 A - 10 um grid), disk profiles as config2 with a
+    per-cell lognormal rho perturbation of sigma_dex dex, then the rho floor.
+    Returns nz, nw (int arrays), waves (concatenated), dz (per column), and
+    rho, T concatenated over columns."""
+    rng = np.random.default_rng(seed)
+    nz = rng.integers(nz_range[0], nz_range[1] + 1, ncol)
+    nw = rng.integers(nw_range[0], nw_range[1] + 1, ncol)
+    waves, dzs, rhos, Ts = [], [], [], []
+    for c in range(ncol):
+        rho, T, dz = disk_columns(1, 1, int(nz[c]), rng=rng)
+        rho = np.maximum(rho.reshape(-1) * 10.0 ** rng.normal(0.0, sigma_dex, int(nz[c])), RHO_FLOOR)
+        rhos.append(rho)
+        Ts.append(T.reshape(-1))
+        dzs.append(dz)
+        waves.append(log_waves(int(nw[c])))
+    return (nz.astype(np.int32), nw.astype(np.int32), np.concatenate(waves), np.array(dzs),
+            np.concatenate(rhos), np.concatenate(Ts))

@@ -1,0 +1,80 @@
+"""Batched uneven columns (oco_batch_*): every column bit-exact against the
+oracle built with the kernels' elementary functions, on the autotuned
+kernel, on every hipGraph replay (with eager work between replays), and a
+zero pivot reported for the first failing column in column order."""
+import numpy as np
+import pytest
+
+from conftest import golden
+
+pytestmark = pytest.mark.gpu
+
+
+def oracle_columns(nz, nw, waves, dz, rho, T, portable=True):
+    from oracle.oracle import Oracle
+    outs = []
+    c0 = w0 = 0
+    for n, m, d in zip(nz.tolist(), nw.tolist(), dz.tolist()):
+        o = Oracle(waves[w0:w0 + m], portable=portable)
+        U, V, A, dg = o.solve_block(rho[c0:c0 + n].reshape(1, 1, n), T[c0:c0 + n].reshape(1, 1, n), d)
+        outs.append((U[:, 0, 0], V[:, 0, 0], A[:, 0, 0], dg.status))
+        c0 += n
+        w0 += m
+    return outs
+
+
+def small_batch():
+    from ocotillo_amd.workloads import config3
+    nz, nw, waves, dz, rho, T = config3(ncol=14, seed=77, nz_range=(4, 300), nw_range=(1, 140))
+    return nz, nw, waves, dz, rho, T
+
+
+def test_batch_bit_exact_and_graph_replays():
+    import ocotillo_amd as oco
+    nz, nw, waves, dz, rho, T = small_batch()
+    print("batch nz", nz.tolist(), "nw", nw.tolist(), flush=True)
+    ref = oracle_columns(nz, nw, waves, dz, rho, T)
+    with oco.Context(0) as ctx:
+        ctx.set_physics(2e4, False, 0.4)
+        b = oco.Batch(ctx, nz, nw, waves, dz)
+        other = oco.Context(0)
+        other.set_wavelengths(waves[:5])
+        for rep in range(4):
+            U, V, A, d = b.run_host(rho, T)
+            assert d.status == 0
+            Us, Vs, As = b.split(U, V, A)
+            for c, (rU, rV, rA, st) in enumerate(ref):
+                assert st == 0
+                assert np.array_equal(Us[c], rU) and np.array_equal(Vs[c], rV) and np.array_equal(As[c], rA), \
+                    ("replay", rep, "column", c)
+            # the program's own eager work between replays
+            other.solve_block(rho[:nz[0]].reshape(1, 1, -1), T[:nz[0]].reshape(1, 1, -1), float(dz[0]))
+        v, nb, tuned = b.tuning()
+        assert tuned == 1 and nb > 0
+        other.close()
+        b.close()
+
+
+def test_batch_zero_pivot_first_column():
+    import ocotillo_amd as oco
+    g = golden("dump_pivot0.npz")
+    nz0 = int(g["nz"])
+    good_nz, good_nw, good_w, good_dz, good_rho, good_T = small_batch()
+    # column 0: a good one; column 1: the column the reference stopped on
+    nz = np.array([good_nz[0], nz0, good_nz[1]], np.int32)
+    nw = np.array([good_nw[0], int(g["nw"]), good_nw[1]], np.int32)
+    waves = np.concatenate([good_w[:good_nw[0]], g["waves"], good_w[good_nw[0]:good_nw[0] + good_nw[1]]])
+    dz = np.array([good_dz[0], float(g["dz"]), good_dz[1]])
+    bad = int(g["ref_stop_col"]) - 1
+    rho = np.concatenate([good_rho[:good_nz[0]], g["rho"][bad], good_rho[good_nz[0]:good_nz[0] + good_nz[1]]])
+    T = np.concatenate([good_T[:good_nz[0]], g["T"][bad], good_T[good_nz[0]:good_nz[0] + good_nz[1]]])
+    with oco.Context(0) as ctx:
+        ctx.set_physics(2e4, False, 0.4)
+        b = oco.Batch(ctx, nz, nw, waves, dz)
+        U, V, A, d = b.run_host(rho, T, raise_on_pivot=False)
+        assert d.status == 2 and d.fail_ix == 2 and d.fail_iw == int(g["ref_stop_iw"])
+        ref = oracle_columns(nz, nw, waves, dz, rho, T)
+        Us, Vs, As = b.split(U, V, A)
+        for c in (0, 2):
+            assert np.array_equal(Us[c], ref[c][0]) and np.array_equal(Vs[c], ref[c][1])
+        b.close()
