@@ -116,8 +116,9 @@ int oco_batch_buffers(oco_batch* plan, double** d_rho, double** d_T, double** d_
 int oco_batch_run(oco_batch* plan, int mode, oco_diag* diag);
 int oco_batch_run_host(oco_batch* plan, const double* rho, const double* T, double* U, double* V,
                        double* absorp, int mode, oco_diag* diag);
-/* Autotuner outcome: kernel variant, workgroups, and whether tuning ran. */
-int oco_batch_tuning(oco_batch* plan, int* variant, int* nblocks, int* tuned);
+/* Autotuner outcome for a precision mode: kernel variant, workgroups, and
+ * whether tuning ran (each mode is tuned and captured on its first run). */
+int oco_batch_tuning(oco_batch* plan, int mode, int* variant, int* nblocks, int* tuned);
 
 /* Message text for a status code. */
 const char* oco_strerror(int status);

@@ -79,7 +79,8 @@ def load_library(path=None):
     L.oco_batch_buffers.argtypes = [C.c_void_p, _PP, _PP, _PP, _PP, _PP]
     L.oco_batch_run.argtypes = [C.c_void_p, C.c_int, C.POINTER(Diag)]
     L.oco_batch_run_host.argtypes = [C.c_void_p, _DP, _DP, _DP, _DP, _DP, C.c_int, C.POINTER(Diag)]
-    L.oco_batch_tuning.argtypes = [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]
+    L.oco_batch_tuning.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                   C.POINTER(C.c_int)]
     L.oco_strerror.restype = C.c_char_p
     L.oco_strerror.argtypes = [C.c_int]
     L.oco_version.restype = C.c_char_p
@@ -219,9 +220,10 @@ class Batch:
         self.L.oco_batch_buffers(self.h, *[C.byref(x) for x in p])
         return [x.value for x in p]
 
-    def tuning(self):
+    def tuning(self, mode=MODE_FP64):
+        """(kernel variant, workgroups, tuned?) of the autotuner for `mode`."""
         v, nb, t = C.c_int(), C.c_int(), C.c_int()
-        self.L.oco_batch_tuning(self.h, C.byref(v), C.byref(nb), C.byref(t))
+        self.L.oco_batch_tuning(self.h, mode, C.byref(v), C.byref(nb), C.byref(t))
         return v.value, nb.value, t.value
 
     def run(self, mode=MODE_FP64, raise_on_pivot=True):

@@ -243,7 +243,7 @@ int oco_solve_block_device(oco_ctx* c, int nz, int nyloc, int nxloc, double dz, 
 {
     if (!c) return OCO_E_ARG;
     if (nz < 4 || nyloc < 1 || nxloc < 1 || !d_rho || !d_T || !d_U || !d_V || !d_A) return OCO_E_ARG;
-    if (mode != OCO_MODE_FP64) return OCO_E_ARG;
+    if (mode != OCO_MODE_FP64 && mode != OCO_MODE_MIXED) return OCO_E_ARG;
     if (c->nw <= 0) return OCO_E_STATE;
     if (c->lgrey && c->nw != 1) return OCO_E_ARG;   /* source/flux_feautrier.f90:84-87 */
     HIPCHK(hipSetDevice(c->device));
@@ -251,7 +251,10 @@ int oco_solve_block_device(oco_ctx* c, int nz, int nyloc, int nxloc, double dz, 
     const long ncells = ncol * nz;
     const long nsys = ncol * c->nw;
     const long ntiles = (nsys + 63) / 64;
-    int nblocks = (int)(ntiles < oco_sweep_max_blocks() ? ntiles : oco_sweep_max_blocks());
+    const int mixed = mode == OCO_MODE_MIXED;
+    const int uniform = (c->nw % 64) == 0;
+    const int grid = oco_sweep_grid(c->lgrey, mixed, uniform, c->device);
+    int nblocks = (int)(ntiles < grid ? ntiles : grid);
     int st;
     if (!c->lgrey && (st = ensure(c, c->cells, (size_t)ncells * OCO_CELL_STRIDE * sizeof(double)))) return st;
     /* one scratch slot per wavefront; multi-wavefront workgroups round up */
@@ -285,7 +288,7 @@ int oco_solve_block_device(oco_ctx* c, int nz, int nyloc, int nxloc, double dz, 
     HIPCHK(hipEventRecord(c->ev[0], c->stream));
     if (!c->lgrey) HIPCHK(oco_launch_cells(d_rho, d_T, ncells, c->fully_ionized_T, (double*)c->cells.p, c->stream));
     HIPCHK(hipEventRecord(c->ev[1], c->stream));
-    HIPCHK(oco_launch_sweep(a, c->lgrey, nblocks, c->stream));
+    HIPCHK(oco_launch_sweep(a, c->lgrey, mixed, nblocks, c->stream));
     HIPCHK(hipEventRecord(c->ev[2], c->stream));
 
     unsigned long long key = none;
@@ -377,10 +380,11 @@ struct oco_batch {
     std::vector<int> nz, nw;
     std::vector<long> woff;
     DevBuf ints, longs, dz, wtab, rho, T, cells, U, V, A, scratch, counter, fail;
-    int nblocks = 0, variant = 1, tuned = 0;
-    float tune_ms[16] = {0};
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t exec = nullptr;
+    /* per precision mode: autotuned kernel variant and grid, captured graph */
+    int nblocks[2] = {0, 0}, variant[2] = {1, 1}, tuned[2] = {0, 0};
+    float tune_ms[2][16] = {{0}};
+    hipGraph_t graph[2] = {nullptr, nullptr};
+    hipGraphExec_t exec[2] = {nullptr, nullptr};
 };
 
 static void batch_release(oco_batch* b)
@@ -389,10 +393,12 @@ static void batch_release(oco_batch* b)
     for (DevBuf* d : {&b->ints, &b->longs, &b->dz, &b->wtab, &b->rho, &b->T, &b->cells, &b->U, &b->V, &b->A,
                       &b->scratch, &b->counter, &b->fail})
         release(c, *d);
-    if (b->exec) (void)hipGraphExecDestroy(b->exec);
-    if (b->graph) (void)hipGraphDestroy(b->graph);
-    b->exec = nullptr;
-    b->graph = nullptr;
+    for (int m = 0; m < 2; m++) {
+        if (b->exec[m]) (void)hipGraphExecDestroy(b->exec[m]);
+        if (b->graph[m]) (void)hipGraphDestroy(b->graph[m]);
+        b->exec[m] = nullptr;
+        b->graph[m] = nullptr;
+    }
 }
 
 static oco_batch_args batch_args(oco_batch* b)
@@ -423,13 +429,13 @@ static oco_batch_args batch_args(oco_batch* b)
 }
 
 /* enqueue one full batch solve with the given kernel variant / grid */
-static hipError_t batch_enqueue(oco_batch* b, int variant, int nblocks)
+static hipError_t batch_enqueue(oco_batch* b, int mixed, int variant, int nblocks)
 {
     oco_ctx* c = b->ctx;
     hipError_t e = oco_launch_cells((const double*)b->rho.p, (const double*)b->T.p, b->ncells, c->fully_ionized_T,
                                     (double*)b->cells.p, c->stream);
     if (e != hipSuccess) return e;
-    return oco_launch_batch(batch_args(b), variant, nblocks, c->stream);
+    return oco_launch_batch(batch_args(b), mixed, variant, nblocks, c->stream);
 }
 
 int oco_batch_create(oco_ctx* c, int ncol, const int* nz, const int* nw, const double* waves, const double* dz,
@@ -497,7 +503,7 @@ int oco_batch_create(oco_ctx* c, int ncol, const int* nz, const int* nw, const d
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, c->device) == hipSuccess && prop.multiProcessorCount > 0)
         dev_cus = prop.multiProcessorCount;
-    b->nblocks = std::min(b->ntiles, dev_cus * 4 * 3);
+    b->nblocks[0] = b->nblocks[1] = std::min(b->ntiles, dev_cus * 4 * 3);
     const int max_blocks = dev_cus * 4 * 4;
     int st = OCO_OK;
     std::vector<int> ints;
@@ -570,21 +576,21 @@ int oco_batch_buffers(oco_batch* b, double** rho, double** T, double** U, double
     return OCO_OK;
 }
 
-int oco_batch_tuning(oco_batch* b, int* variant, int* nblocks, int* tuned)
+int oco_batch_tuning(oco_batch* b, int mode, int* variant, int* nblocks, int* tuned)
 {
-    if (!b) return OCO_E_ARG;
-    if (variant) *variant = b->variant;
-    if (nblocks) *nblocks = b->nblocks;
-    if (tuned) *tuned = b->tuned;
+    if (!b || (mode != OCO_MODE_FP64 && mode != OCO_MODE_MIXED)) return OCO_E_ARG;
+    if (variant) *variant = b->variant[mode];
+    if (nblocks) *nblocks = b->nblocks[mode];
+    if (tuned) *tuned = b->tuned[mode];
     return OCO_OK;
 }
 
-/* first run: time every kernel variant x grid size once and keep the best;
- * then capture the whole solve (reset, cells, sweep) as one hipGraph */
-static int batch_prepare(oco_batch* b)
+/* first run in a mode: time every kernel variant x grid size once and keep
+ * the best; then capture the whole solve (reset, cells, sweep) as one hipGraph */
+static int batch_prepare(oco_batch* b, int m)
 {
     oco_ctx* c = b->ctx;
-    if (!b->tuned) {
+    if (!b->tuned[m]) {
         int dev_cus = 256;
         hipDeviceProp_t prop;
         if (hipGetDeviceProperties(&prop, c->device) == hipSuccess && prop.multiProcessorCount > 0)
@@ -595,32 +601,32 @@ static int batch_prepare(oco_batch* b)
             for (int wps = 2; wps <= 4; wps++) {
                 int nb = std::min(b->ntiles, dev_cus * 4 * wps);
                 HIPCHK(hipEventRecord(c->ev[0], c->stream));
-                HIPCHK(batch_enqueue(b, v, nb));
+                HIPCHK(batch_enqueue(b, m, v, nb));
                 HIPCHK(hipEventRecord(c->ev[1], c->stream));
                 HIPCHK(hipEventSynchronize(c->ev[1]));
                 float ms = 0.f;
                 HIPCHK(hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
-                if (k < 16) b->tune_ms[k++] = ms;
+                if (k < 16) b->tune_ms[m][k++] = ms;
                 if (ms < best) {
                     best = ms;
-                    b->variant = v;
-                    b->nblocks = nb;
+                    b->variant[m] = v;
+                    b->nblocks[m] = nb;
                 }
             }
         }
-        b->tuned = 1;
+        b->tuned[m] = 1;
     }
-    if (!b->exec) {
+    if (!b->exec[m]) {
         HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-        hipError_t e = batch_enqueue(b, b->variant, b->nblocks);
+        hipError_t e = batch_enqueue(b, m, b->variant[m], b->nblocks[m]);
         hipGraph_t g = nullptr;
         hipError_t e2 = hipStreamEndCapture(c->stream, &g);
         if (e != hipSuccess || e2 != hipSuccess) {
             if (g) (void)hipGraphDestroy(g);
             return OCO_E_HIP;
         }
-        b->graph = g;
-        HIPCHK(hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0));
+        b->graph[m] = g;
+        HIPCHK(hipGraphInstantiate(&b->exec[m], g, nullptr, nullptr, 0));
     }
     return OCO_OK;
 }
@@ -628,13 +634,13 @@ static int batch_prepare(oco_batch* b)
 int oco_batch_run(oco_batch* b, int mode, oco_diag* diag)
 {
     if (!b) return OCO_E_ARG;
-    if (mode != OCO_MODE_FP64) return OCO_E_ARG;
+    if (mode != OCO_MODE_FP64 && mode != OCO_MODE_MIXED) return OCO_E_ARG;
     oco_ctx* c = b->ctx;
     HIPCHK(hipSetDevice(c->device));
-    int st = batch_prepare(b);
+    int st = batch_prepare(b, mode);
     if (st) return st;
     HIPCHK(hipEventRecord(c->ev[0], c->stream));
-    HIPCHK(hipGraphLaunch(b->exec, c->stream));
+    HIPCHK(hipGraphLaunch(b->exec[mode], c->stream));
     HIPCHK(hipEventRecord(c->ev[2], c->stream));
     unsigned long long key = OCO_FAIL_NONE;
     HIPCHK(hipMemcpyAsync(&key, b->fail.p, sizeof(key), hipMemcpyDeviceToHost, c->stream));

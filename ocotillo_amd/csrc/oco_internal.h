@@ -49,12 +49,13 @@ struct oco_batch_args {
 };
 
 /* Launchers (enqueue on `stream`, no synchronisation). */
-hipError_t oco_launch_batch(const oco_batch_args& a, int variant, int nblocks, hipStream_t stream);
+/* mixed != 0 selects the mixed-precision kernels (non-grey only). */
+hipError_t oco_launch_batch(const oco_batch_args& a, int mixed, int variant, int nblocks, hipStream_t stream);
 int oco_batch_variants(void);
 hipError_t oco_launch_cells(const double* rho, const double* T, long ncells, double fully_ionized_T,
                             double* cells, hipStream_t stream);
-hipError_t oco_launch_sweep(const oco_sweep_args& a, int lgrey, int nblocks, hipStream_t stream);
-int oco_sweep_max_blocks(void);
+hipError_t oco_launch_sweep(const oco_sweep_args& a, int lgrey, int mixed, int nblocks, hipStream_t stream);
+int oco_sweep_grid(int lgrey, int mixed, int uniform, int device);
 
 /* failure key layout: system index << 32 | stage << 28 | row */
 #define OCO_FAIL_NONE 0xffffffffffffffffULL

@@ -16,11 +16,14 @@
  *               back-substitution fused with the ghost extrapolation, the
  *               6th-order derivative (7-point register window) and V.
  *
- * All FP64, compiled with -ffp-contract=off; the elementary functions are
- * oco_math.h's, read from an LDS copy of their tables.
+ * The FP64 path is compiled with -ffp-contract=off and its elementary
+ * functions are oco_math.h's, read from an LDS copy of their tables.
+ * sweep_mixed / batch_mixed are the same kernels with the mixed-precision
+ * opacity of oco_mixed.h (OCO_MODE_MIXED).
  */
 #include "oco_internal.h"
 #include "oco_physics.h"
+#include "oco_mixed.h"
 
 __constant__ oco_mtab oco_mtab_dev = OCO_MTAB_INIT;
 
@@ -97,15 +100,77 @@ __device__ static inline void record_fail(unsigned long long* fail, long sys, in
     atomicMin(fail, key);
 }
 
+/* x / y: the IEEE divide of the FP64 path, or the reciprocal-based divide of
+ * the mixed path (MIX is a compile-time switch; the FP64 path compiles to
+ * exactly the reference's operations). */
+template <bool MIX>
+__device__ static inline double qdiv(double x, double y)
+{
+    if constexpr (MIX) return mx_div(x, y);
+    else return x / y;
+}
+
+/* ---- coalesced output stores ----
+ * A lane owns one system, and its outputs U(:), V(:), alpha(:) are contiguous
+ * runs, but consecutive lanes' runs lie (nz+6)*ncol doubles apart in the
+ * reference's (z, column, wavelength) layout, so a plain per-row store sends
+ * 64 lines per instruction.  Rows are staged in LDS instead, R per lane, and
+ * every R rows the wavefront writes them out cooperatively: each store
+ * instruction then covers 64/R whole runs of R consecutive doubles.
+ * Every lane of the wavefront takes part in a flush, so lanes without a
+ * system of their own run a duplicate one with their stores switched off. */
+#ifndef OCO_STAGE_ROWS
+#define OCO_STAGE_ROWS 8
+#endif
+struct oco_stage {
+    double ub[OCO_WAVE * (OCO_STAGE_ROWS + 1)];   /* U rows; alpha rows during the forward pass */
+    double vb[OCO_WAVE * (OCO_STAGE_ROWS + 1)];   /* V rows */
+    double* base[3][OCO_WAVE];                    /* per lane: U, V, alpha; null = lane stores nothing */
+};
+
+/* LDS traffic of one wavefront is performed in issue order, so ordering the
+ * wavefront's own accesses needs only a compiler barrier at wavefront scope */
+__device__ static inline void wave_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+}
+
+__device__ static inline void stage_put(double* buf, int idx, double v)
+{
+    buf[threadIdx.x * (OCO_STAGE_ROWS + 1) + (idx & (OCO_STAGE_ROWS - 1))] = v;
+}
+
+/* write block b (elements b*R .. b*R+R-1, clipped to [0, len)) of every lane:
+ * thread t handles element r = t mod R of lanes t/R + k*(64/R), k = 0..R-1,
+ * so R consecutive threads store one lane's run */
+__device__ static inline void stage_flush(const double* buf, double* const* base, int b, int len)
+{
+    constexpr int R = OCO_STAGE_ROWS, L = OCO_WAVE / OCO_STAGE_ROWS;
+    const int r = (int)threadIdx.x & (R - 1);
+    const int s0 = (int)threadIdx.x / R;
+    const double* bp = buf + s0 * (R + 1) + r;
+    double* const* pp = base + s0;
+    const int idx = b * R + r;
+    const bool inb = idx < len;
+#pragma unroll
+    for (int k = 0; k < R; k++) {
+        double* p = pp[k * L];
+        double v = bp[k * L * (R + 1)];
+        if (p != nullptr && inb) p[idx] = v;
+    }
+}
+
 /* Row coefficients (source/auxiliary.f90:147-197) -- interior row from the
  * cell and its neighbours' alpha, boundary rows from the cell alone. */
+template <bool MIX = false>
 __device__ static inline void interior_row(double am, const oco_pt& pc, double ap, double dz2, double& av,
                                            double& bv, double& cv, double& dv)
 {
     double kp = 0.5 * (ap + pc.alpha);
     double km = 0.5 * (pc.alpha + am);
-    av = pc.alpha2 / km;
-    cv = pc.alpha2 / kp;
+    av = qdiv<MIX>(pc.alpha2, km);
+    cv = qdiv<MIX>(pc.alpha2, kp);
     double ze = (dz2 * ((pc.alpha * pc.alpha) * pc.alpha)) * (1.0 - pc.albedo);
     bv = -((av + cv) + ze);
     dv = -(pc.S * ze);
@@ -129,12 +194,13 @@ __device__ static inline oco_pt point_rec(const oco_sweep_args& a, long col, int
     return oco_point_s<WS>(rec, w, M);
 }
 
-template <bool GREY, int WS = 1>
+template <bool GREY, int WS = 1, bool MIX = false>
 __device__ static inline oco_pt point_sys(const oco_sys& d, int z, const double* rec, const double* w,
                                           const oco_mtab* M)
 {
     if (GREY) return oco_grey_point(d.rho[z], d.T[z], d.sigma_grey, M);
-    return oco_point_s<WS>(rec, w, M);
+    if constexpr (MIX) return oco_point_mixed<WS>(rec, w);
+    else return oco_point_s<WS>(rec, w, M);
 }
 
 /* Tuning knob: OCO_SWEEP_WAVES_PER_EU=n asks the compiler to fit n waves per
@@ -148,16 +214,21 @@ __device__ static inline oco_pt point_sys(const oco_sys& d, int z, const double*
 /* One (column, wavelength) system, one lane: forward elimination with the
  * opacity of the cells recomputed in registers, then back-substitution fused
  * with ghosts, der and V.  `w` is the lane's wavelength-table row with
- * stride WS; `scr` its scratch column (stride 64 doubles per entry). */
-template <bool GREY, bool UNIFORM, int WS, int BC = OCO_BACK_CHUNK>
-__device__ static inline void solve_system(const oco_sys& d, const double* w, double* scr,
-                                           unsigned long long* fail, const oco_mtab* M)
+ * stride WS; `scr` its scratch column (stride 64 doubles per entry).
+ * MIX selects the mixed-precision opacity and reciprocal-based divides; the
+ * elimination itself stays FP64 either way. */
+template <bool GREY, bool UNIFORM, int WS, int BC = OCO_BACK_CHUNK, bool MIX = false>
+__device__ static inline void solve_system(const oco_sys& d, bool valid, const double* w, double* scr,
+                                           unsigned long long* fail, const oco_mtab* M, oco_stage* S)
 {
     const int nz = d.nz;
     const double dz = d.dz, dz1 = d.dz1, dz2 = d.dz2;
-    double* Uo = d.U;
-    double* Vo = d.V;
-    double* Ao = d.A;
+    constexpr int R = OCO_STAGE_ROWS;
+    const int lane = threadIdx.x;
+    wave_sync();   /* the previous system's flushes have read the buffers */
+    S->base[0][lane] = valid ? d.U : nullptr;
+    S->base[1][lane] = valid ? d.V : nullptr;
+    S->base[2][lane] = valid ? d.A : nullptr;
 
     /* ---- forward elimination (source/auxiliary.f90:25-41) ----
      * Two-deep pipeline: iteration j eliminates row j (which needs the
@@ -165,9 +236,9 @@ __device__ static inline void solve_system(const oco_sys& d, const double* w, do
      * whose work is independent of the elimination chain. */
     double rec[CR_NFIELD];
     if (!GREY) load_rec<UNIFORM>(d.cells, 0, rec);
-    oco_pt pB = point_sys<GREY, WS>(d, 0, rec, w, M);
+    oco_pt pB = point_sys<GREY, WS, MIX>(d, 0, rec, w, M);
     if (!GREY) load_rec<UNIFORM>(d.cells, 1, rec);
-    oco_pt pC = point_sys<GREY, WS>(d, 1, rec, w, M);
+    oco_pt pC = point_sys<GREY, WS, MIX>(d, 1, rec, w, M);
     if (!GREY) load_rec<UNIFORM>(d.cells, 2, rec);
 
     double bv, dv, av, cv;
@@ -176,33 +247,38 @@ __device__ static inline void solve_system(const oco_sys& d, const double* w, do
     int fail_stage = (bv == 0.0) ? 1 : 0;
     int fail_row = 1;
     double bet = bv;
-    double u = dv / bet;
+    double u = qdiv<MIX>(dv, bet);
     scr[0 * OCO_WAVE] = 0.0;
     scr[1 * OCO_WAVE] = u;
     scr[2 * OCO_WAVE] = pB.alpha1;
-    Ao[0] = pB.alpha;
+    stage_put(S->ub, 0, pB.alpha);
     double aA = pB.alpha;   /* alpha of row j-1 */
     pB = pC;                /* row j = 1 */
-    pC = point_sys<GREY, WS>(d, 2, rec, w, M);
+    pC = point_sys<GREY, WS, MIX>(d, 2, rec, w, M);
 
     for (int j = 1; j <= nz - 3; j++) {
         if (!GREY) load_rec<UNIFORM>(d.cells, j + 2, rec);
         /* row j and its elimination step */
         double cprev = cv;
-        interior_row(aA, pB, pC.alpha, dz2, av, bv, cv, dv);
-        double gam = cprev / bet;
+        interior_row<MIX>(aA, pB, pC.alpha, dz2, av, bv, cv, dv);
+        double gam = qdiv<MIX>(cprev, bet);
         bet = bv - av * gam;
         int first = (bet == 0.0) && (fail_stage == 0);
         fail_row = first ? j + 1 : fail_row;
         fail_stage = first ? 2 : fail_stage;
-        u = (dv - av * u) / bet;
+        u = qdiv<MIX>(dv - av * u, bet);
         double* sp = scr + (long)j * 3 * OCO_WAVE;
         sp[0] = gam;
         sp[OCO_WAVE] = u;
         sp[2 * OCO_WAVE] = pB.alpha1;
-        Ao[j] = pB.alpha;
+        stage_put(S->ub, j, pB.alpha);
+        if ((j & (R - 1)) == R - 1) {
+            wave_sync();
+            stage_flush(S->ub, S->base[2], j / R, nz);
+            wave_sync();
+        }
         /* point j+2 (independent of the chain above) */
-        oco_pt pD = point_sys<GREY, WS>(d, j + 2, rec, w, M);
+        oco_pt pD = point_sys<GREY, WS, MIX>(d, j + 2, rec, w, M);
         aA = pB.alpha;
         pB = pC;
         pC = pD;
@@ -210,39 +286,49 @@ __device__ static inline void solve_system(const oco_sys& d, const double* w, do
     {   /* row nz-2 (interior, no new point) and row nz-1 (boundary) */
         int j = nz - 2;
         double cprev = cv;
-        interior_row(aA, pB, pC.alpha, dz2, av, bv, cv, dv);
-        double gam = cprev / bet;
+        interior_row<MIX>(aA, pB, pC.alpha, dz2, av, bv, cv, dv);
+        double gam = qdiv<MIX>(cprev, bet);
         bet = bv - av * gam;
         int first = (bet == 0.0) && (fail_stage == 0);
         fail_row = first ? j + 1 : fail_row;
         fail_stage = first ? 2 : fail_stage;
-        u = (dv - av * u) / bet;
+        u = qdiv<MIX>(dv - av * u, bet);
         double* sp = scr + (long)j * 3 * OCO_WAVE;
         sp[0] = gam;
         sp[OCO_WAVE] = u;
         sp[2 * OCO_WAVE] = pB.alpha1;
-        Ao[j] = pB.alpha;
+        stage_put(S->ub, j, pB.alpha);
+        if ((j & (R - 1)) == R - 1) {
+            wave_sync();
+            stage_flush(S->ub, S->base[2], j / R, nz);
+            wave_sync();
+        }
         j = nz - 1;
         cprev = cv;
         boundary_row(pC, dz, dz2, bv, dv);
         av = pC.alpha;                       /* a(nz) = alpha, c(nz) = 0 */
-        gam = cprev / bet;
+        gam = qdiv<MIX>(cprev, bet);
         bet = bv - av * gam;
         first = (bet == 0.0) && (fail_stage == 0);
         fail_row = first ? j + 1 : fail_row;
         fail_stage = first ? 2 : fail_stage;
-        u = (dv - av * u) / bet;
+        u = qdiv<MIX>(dv - av * u, bet);
         sp = scr + (long)j * 3 * OCO_WAVE;
         sp[0] = gam;
         sp[OCO_WAVE] = u;
         sp[2 * OCO_WAVE] = pC.alpha1;
-        Ao[j] = pC.alpha;
+        stage_put(S->ub, j, pC.alpha);
+        wave_sync();
+        stage_flush(S->ub, S->base[2], j / R, nz);   /* last (possibly partial) block */
     }
     if (fail_stage != 0) {
-        /* the reference stops at this system (source/auxiliary.f90:27-39) */
-        record_fail(fail, d.key, fail_stage, fail_row);
-        return;
+        /* the reference stops at this system (source/auxiliary.f90:27-39):
+         * its U and V are not written */
+        if (valid) record_fail(fail, d.key, fail_stage, fail_row);
+        S->base[0][lane] = nullptr;
+        S->base[1][lane] = nullptr;
     }
+    wave_sync();
 
     /* ---- back-substitution (:43-45) fused with update_ghosts (:49-62),
      *      der (:64-78) and calc_flux (:96-109) ---- */
@@ -252,7 +338,6 @@ __device__ static inline void solve_system(const oco_sys& d, const double* w, do
     double gnext = scr[(long)(nz - 1) * 3 * OCO_WAVE];
     double a1q0 = scr[(long)(nz - 1) * 3 * OCO_WAVE + 2 * OCO_WAVE];   /* 1/alpha of the V row pending */
     double a1q1 = 0.0, a1q2 = 0.0, a1q3 = 0.0;
-    Uo[3 + nz - 1] = Un;
     double t[4];
     t[0] = Un;
     for (int k = 1; k < 4; k++) {
@@ -263,17 +348,33 @@ __device__ static inline void solve_system(const oco_sys& d, const double* w, do
         double a1 = sp[2 * OCO_WAVE];
         a1q3 = a1q2; a1q2 = a1q1; a1q1 = a1q0; a1q0 = a1;
         t[k] = Uj;
-        Uo[3 + j] = Uj;
     }
     /* top ghosts (f(n2+i) = 2 f(n2) - f(n2-i)) */
     double g1 = 2.0 * t[0] - t[1], g2 = 2.0 * t[0] - t[2], g3 = 2.0 * t[0] - t[3];
-    Uo[3 + nz] = g1;
-    Uo[3 + nz + 1] = g2;
-    Uo[3 + nz + 2] = g3;
+    /* U(e) for e = nz+5 down to nz-1, in descending order: a block is
+     * flushed when its lowest element is staged */
+    {
+        const double top[7] = {g3, g2, g1, t[0], t[1], t[2], t[3]};
+#pragma unroll
+        for (int k = 0; k < 7; k++) {
+            const int e = nz + 5 - k;
+            stage_put(S->ub, e, top[k]);
+            if ((e & (R - 1)) == 0) {
+                wave_sync();
+                stage_flush(S->ub, S->base[0], e / R, nz + 6);
+                wave_sync();
+            }
+        }
+    }
     double w0 = t[3], w1 = t[2], w2 = t[1], w3 = t[0], w4 = g1, w5 = g2, w6 = g3;
     {
         double df = c60 * (((45.0 * (w4 - w2)) - (9.0 * (w5 - w1))) + (w6 - w0));
-        Vo[nz - 1] = ((-1.0 * a1q3) * df) * dz1;
+        stage_put(S->vb, nz - 1, ((-1.0 * a1q3) * df) * dz1);
+        if (((nz - 1) & (R - 1)) == 0) {
+            wave_sync();
+            stage_flush(S->vb, S->base[1], (nz - 1) / R, nz);
+            wave_sync();
+        }
     }
     /* rows nz-5 .. 0 in chunks of BC: each chunk's scratch
      * entries are loaded together, one chunk ahead of their use, so the loop
@@ -302,10 +403,17 @@ __device__ static inline void solve_system(const oco_sys& d, const double* w, do
                     double Uj = cu[k] - gnext * w0;
                     gnext = cg[k];
                     a1q3 = a1q2; a1q2 = a1q1; a1q1 = a1q0; a1q0 = ca[k];
-                    Uo[3 + j] = Uj;
+                    const int e = j + 3;   /* U(e) and V(z = e) */
+                    stage_put(S->ub, e, Uj);
                     w6 = w5; w5 = w4; w4 = w3; w3 = w2; w2 = w1; w1 = w0; w0 = Uj;
                     double df = c60 * (((45.0 * (w4 - w2)) - (9.0 * (w5 - w1))) + (w6 - w0));
-                    Vo[j + 3] = ((-1.0 * a1q3) * df) * dz1;
+                    stage_put(S->vb, e, ((-1.0 * a1q3) * df) * dz1);
+                    if ((e & (R - 1)) == 0) {
+                        wave_sync();
+                        stage_flush(S->ub, S->base[0], e / R, nz + 6);
+                        stage_flush(S->vb, S->base[1], e / R, nz);
+                        wave_sync();
+                    }
                 }
             }
 #pragma unroll
@@ -314,27 +422,27 @@ __device__ static inline void solve_system(const oco_sys& d, const double* w, do
     }
     /* bottom ghosts (f(n1-i) = 2 f(n1) - f(n1+i)); window now e = 3..9 */
     double u0 = w0, G1 = 2.0 * u0 - w1, G2 = 2.0 * u0 - w2, G3 = 2.0 * u0 - w3;
-    Uo[2] = G1;
-    Uo[1] = G2;
-    Uo[0] = G3;
+    stage_put(S->ub, 2, G1);
+    stage_put(S->ub, 1, G2);
+    stage_put(S->ub, 0, G3);
     const double gh[3] = {G1, G2, G3};
 #pragma unroll
     for (int k = 0; k < 3; k++) {
         w6 = w5; w5 = w4; w4 = w3; w3 = w2; w2 = w1; w1 = w0; w0 = gh[k];
         a1q3 = a1q2; a1q2 = a1q1; a1q1 = a1q0;
         double df = c60 * (((45.0 * (w4 - w2)) - (9.0 * (w5 - w1))) + (w6 - w0));
-        Vo[2 - k] = ((-1.0 * a1q3) * df) * dz1;
+        stage_put(S->vb, 2 - k, ((-1.0 * a1q3) * df) * dz1);
     }
+    wave_sync();
+    stage_flush(S->ub, S->base[0], 0, nz + 6);
+    stage_flush(S->vb, S->base[1], 0, nz);
 }
 
-/* One wavefront per workgroup; the lane's wavelength-table row lives in
- * registers.  Used when a wavefront's 64 systems span several columns
- * (nw not a multiple of 64) and for the grey case. */
-template <bool GREY, bool UNIFORM>
-__global__ __launch_bounds__(OCO_WAVE) OCO_SWEEP_ATTR void sweep_fp64(const oco_sweep_args a)
+/* One wavefront per workgroup, grid-stride over tiles of 64 systems; the
+ * lane's wavelength-table row lives in registers. */
+template <bool GREY, bool UNIFORM, bool MIX>
+__device__ static inline void sweep_tiles(const oco_sweep_args& a, const oco_mtab* M, oco_stage* S)
 {
-    __shared__ oco_mtab smt;
-    load_mtab(&smt);
     const int lane = threadIdx.x;
     const int nw = a.nw;
     const long nsys = a.ncol * nw;
@@ -349,10 +457,11 @@ __global__ __launch_bounds__(OCO_WAVE) OCO_SWEEP_ATTR void sweep_fp64(const oco_
             iw = (int)(sys0 % nw) + lane;
         } else {
             long s = sys0 + lane;
-            if (s >= nsys) break;
+            if (s >= nsys) s = nsys - 1;   /* past the end: duplicate of the last system, no stores */
             col = s / nw;
             iw = (int)(s % nw);
         }
+        const bool valid = UNIFORM || sys0 + lane < nsys;
         double w[OCO_WT_STRIDE];
         if (!GREY) {
 #pragma unroll
@@ -369,11 +478,29 @@ __global__ __launch_bounds__(OCO_WAVE) OCO_SWEEP_ATTR void sweep_fp64(const oco_
         d.V = a.V + osys * a.nz;
         d.A = a.A + osys * a.nz;
         d.key = col * nw + iw;
-        solve_system<GREY, UNIFORM, 1>(d, w, scr, a.fail, &smt);
+        solve_system<GREY, UNIFORM, 1, OCO_BACK_CHUNK, MIX>(d, valid, w, scr, a.fail, M, S);
     }
 }
 
+/* FP64 drop-in sweep.  UNIFORM: nw is a multiple of 64, so a wavefront's
+ * systems share one column. */
+template <bool GREY, bool UNIFORM>
+__global__ __launch_bounds__(OCO_WAVE) OCO_SWEEP_ATTR void sweep_fp64(const oco_sweep_args a)
+{
+    __shared__ oco_mtab smt;
+    __shared__ oco_stage stg;
+    load_mtab(&smt);
+    sweep_tiles<GREY, UNIFORM, false>(a, &smt, &stg);
+}
 
+/* Mixed-precision sweep (non-grey): FP32-core opacity, FP64 elimination.
+ * Its elementary functions use no tables. */
+template <bool UNIFORM>
+__global__ __launch_bounds__(OCO_WAVE) OCO_SWEEP_ATTR void sweep_mixed(const oco_sweep_args a)
+{
+    __shared__ oco_stage stg;
+    sweep_tiles<false, UNIFORM, true>(a, nullptr, &stg);
+}
 
 /* First-pillar canaries (source/flux_feautrier.f90:198-201): sum(a), sum(b),
  * sum(c), sum(d) of system (ix,iy,iw) = (1,1,1).  Printed diagnostics only
@@ -445,12 +572,9 @@ typedef const __attribute__((address_space(4))) int* oco_cptr_i;
 typedef const __attribute__((address_space(4))) long* oco_cptr_l;
 typedef const __attribute__((address_space(4))) double* oco_cptr_d;
 
-template <int BC>
-__global__ __launch_bounds__(OCO_WAVE) void batch_fp64(const oco_batch_args a)
+template <int BC, bool MIX>
+__device__ static inline void batch_tiles(const oco_batch_args& a, const oco_mtab* M, int* tile_s, oco_stage* S)
 {
-    __shared__ oco_mtab smt;
-    __shared__ int tile_s;
-    load_mtab(&smt);
     const int lane = threadIdx.x;
     double* scr = a.scratch + (long)blockIdx.x * a.scr_stride + lane;
     oco_cptr_i tiles = (oco_cptr_i)a.tiles;
@@ -462,16 +586,18 @@ __global__ __launch_bounds__(OCO_WAVE) void batch_fp64(const oco_batch_args a)
     oco_cptr_l voff = (oco_cptr_l)a.v_off;
     oco_cptr_d dzs = (oco_cptr_d)a.dz;
     for (;;) {
-        if (lane == 0) tile_s = (int)atomicAdd(a.counter, 1u);
+        if (lane == 0) *tile_s = (int)atomicAdd(a.counter, 1u);
         __syncthreads();
-        int t = __builtin_amdgcn_readfirstlane(tile_s);
+        int t = __builtin_amdgcn_readfirstlane(*tile_s);
         __syncthreads();
         if (t >= a.ntiles) break;
         const int col = tiles[2 * t];
-        const int iw = tiles[2 * t + 1] + lane;
         const int nz = nzs[col];
         const int nw = nws[col];
-        if (iw < nw) {
+        const bool valid = tiles[2 * t + 1] + lane < nw;
+        /* lanes past the column's last wavelength run a duplicate of it, no stores */
+        const int iw = valid ? tiles[2 * t + 1] + lane : nw - 1;
+        {
             oco_sys d;
             d.nz = nz;
             d.dz = dzs[3 * col];
@@ -489,18 +615,45 @@ __global__ __launch_bounds__(OCO_WAVE) void batch_fp64(const oco_batch_args a)
             const double* wr = a.wtab + (woff[col] + iw) * OCO_WT_STRIDE;
 #pragma unroll
             for (int k = 0; k < OCO_WT_STRIDE; k++) w[k] = wr[k];
-            solve_system<false, true, 1, BC>(d, w, scr, a.fail, &smt);
+            solve_system<false, true, 1, BC, MIX>(d, valid, w, scr, a.fail, M, S);
         }
     }
 }
 
-hipError_t oco_launch_batch(const oco_batch_args& a, int variant, int nblocks, hipStream_t stream)
+template <int BC>
+__global__ __launch_bounds__(OCO_WAVE) void batch_fp64(const oco_batch_args a)
+{
+    __shared__ oco_mtab smt;
+    __shared__ int tile_s;
+    __shared__ oco_stage stg;
+    load_mtab(&smt);
+    batch_tiles<BC, false>(a, &smt, &tile_s, &stg);
+}
+
+template <int BC>
+__global__ __launch_bounds__(OCO_WAVE) void batch_mixed(const oco_batch_args a)
+{
+    __shared__ int tile_s;
+    __shared__ oco_stage stg;
+    batch_tiles<BC, true>(a, nullptr, &tile_s, &stg);
+}
+
+hipError_t oco_launch_batch(const oco_batch_args& a, int mixed, int variant, int nblocks, hipStream_t stream)
 {
     hipLaunchKernelGGL(batch_reset, dim3(1), dim3(64), 0, stream, a.counter, a.fail);
-    switch (variant) {
-    case 0: hipLaunchKernelGGL((batch_fp64<1>), dim3(nblocks), dim3(OCO_WAVE), 0, stream, a); break;
-    case 1: hipLaunchKernelGGL((batch_fp64<2>), dim3(nblocks), dim3(OCO_WAVE), 0, stream, a); break;
-    default: hipLaunchKernelGGL((batch_fp64<4>), dim3(nblocks), dim3(OCO_WAVE), 0, stream, a); break;
+    const dim3 g(nblocks), b(OCO_WAVE);
+    if (mixed) {
+        switch (variant) {
+        case 0: hipLaunchKernelGGL((batch_mixed<1>), g, b, 0, stream, a); break;
+        case 1: hipLaunchKernelGGL((batch_mixed<2>), g, b, 0, stream, a); break;
+        default: hipLaunchKernelGGL((batch_mixed<4>), g, b, 0, stream, a); break;
+        }
+    } else {
+        switch (variant) {
+        case 0: hipLaunchKernelGGL((batch_fp64<1>), g, b, 0, stream, a); break;
+        case 1: hipLaunchKernelGGL((batch_fp64<2>), g, b, 0, stream, a); break;
+        default: hipLaunchKernelGGL((batch_fp64<4>), g, b, 0, stream, a); break;
+        }
     }
     return hipGetLastError();
 }
@@ -518,9 +671,24 @@ hipError_t oco_launch_cells(const double* rho, const double* T, long ncells, dou
     return hipGetLastError();
 }
 
-int oco_sweep_max_blocks(void) { return 4096; }
+/* Workgroups of the sweep kernel that fit on the device at once (occupancy x
+ * CUs): the grid-stride sweep is launched with exactly that many, so every
+ * workgroup is resident from the start and takes an equal share of tiles. */
+int oco_sweep_grid(int lgrey, int mixed, int uniform, int device)
+{
+    const void* k;
+    if (lgrey) k = (const void*)sweep_fp64<true, false>;
+    else if (mixed) k = uniform ? (const void*)sweep_mixed<true> : (const void*)sweep_mixed<false>;
+    else k = uniform ? (const void*)sweep_fp64<false, true> : (const void*)sweep_fp64<false, false>;
+    int per_cu = 0, cus = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, OCO_WAVE, 0) != hipSuccess || per_cu < 1)
+        per_cu = 8;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess || cus < 1)
+        cus = 256;
+    return per_cu * cus;
+}
 
-hipError_t oco_launch_sweep(const oco_sweep_args& a, int lgrey, int nblocks, hipStream_t stream)
+hipError_t oco_launch_sweep(const oco_sweep_args& a, int lgrey, int mixed, int nblocks, hipStream_t stream)
 {
     const bool uniform = (a.nw % OCO_WAVE) == 0;
     if (a.canary) {
@@ -529,12 +697,17 @@ hipError_t oco_launch_sweep(const oco_sweep_args& a, int lgrey, int nblocks, hip
         else
             hipLaunchKernelGGL((canary_fp64<false>), dim3(1), dim3(256), 0, stream, a, a.canary);
     }
+    const dim3 g(nblocks), b(OCO_WAVE);
     if (lgrey) {
-        hipLaunchKernelGGL((sweep_fp64<true, false>), dim3(nblocks), dim3(OCO_WAVE), 0, stream, a);
+        /* grey opacity has no per-wavelength transcendentals: one kernel for both modes */
+        hipLaunchKernelGGL((sweep_fp64<true, false>), g, b, 0, stream, a);
+    } else if (mixed) {
+        if (uniform) hipLaunchKernelGGL((sweep_mixed<true>), g, b, 0, stream, a);
+        else hipLaunchKernelGGL((sweep_mixed<false>), g, b, 0, stream, a);
     } else if (uniform) {
-        hipLaunchKernelGGL((sweep_fp64<false, true>), dim3(nblocks), dim3(OCO_WAVE), 0, stream, a);
+        hipLaunchKernelGGL((sweep_fp64<false, true>), g, b, 0, stream, a);
     } else {
-        hipLaunchKernelGGL((sweep_fp64<false, false>), dim3(nblocks), dim3(OCO_WAVE), 0, stream, a);
+        hipLaunchKernelGGL((sweep_fp64<false, false>), g, b, 0, stream, a);
     }
     return hipGetLastError();
 }

@@ -1,0 +1,137 @@
+/*
+ * oco_mixed.h -- the mixed-precision opacity of OCO_MODE_MIXED (device only).
+ *
+ * On MI355X with packed FP32 disabled, FP32 VALU arithmetic issues at the
+ * same rate as FP64, so products and sums stay FP64 (which also keeps the
+ * 1e-50..1e-10 range of alpha^2 and S representable).  What is cheaper in
+ * FP32 is the transcendental core: exp and log use the hardware FP32
+ * v_exp_f32 / v_log_f32 on an FP64-reduced argument, with the FP64 exponent
+ * put back by ldexp ("exponent scaling"), and divides use v_rcp_f64 with one
+ * Newton step.  Relative error: about 1.2e-7 per exp (the FP32 core), about 1e-7
+ * absolute per log, ~1e-16 per divide.  The three cancelling forms 1 - 10^(-x), exp(d) - 1 and
+ * 1 - exp(-y) switch to an FP64 series below 0.5 so they keep full relative
+ * accuracy at long wavelengths and high temperature.
+ *
+ * The Saha state, number densities and every other temperature-only factor
+ * come from the FP64 cell kernel, unchanged.
+ */
+#ifndef OCO_MIXED_H
+#define OCO_MIXED_H
+
+#include "oco_physics.h"
+
+#define OCO_LOG2E    0x1.71547652b82fep+0
+#define OCO_LOG2_10  0x1.a934f0979a371p+1
+#define OCO_LN2      0x1.62e42fefa39efp-1
+#define OCO_LN10     0x1.26bb1bbb55516p+1
+
+/* 2^t for any finite t: t = n + f, |f| <= 1/2, 2^f in FP32, 2^n by ldexp */
+__device__ static inline double mx_exp2(double t)
+{
+    double tc = __builtin_fmin(__builtin_fmax(t, -2200.0), 2200.0);
+    double n = __builtin_rint(tc);
+    float f = (float)(tc - n);
+    double m = (double)__builtin_amdgcn_exp2f(f);
+    return __builtin_amdgcn_ldexp(m, (int)n);
+}
+
+__device__ static inline double mx_exp(double x) { return mx_exp2(x * OCO_LOG2E); }
+__device__ static inline double mx_exp10(double x) { return mx_exp2(x * OCO_LOG2_10); }
+
+/* ln x for finite x > 0: x = m 2^e, m in [1/2, 1), log2 m in FP32.  The error
+ * is absolute (about 1e-7): callers use the result as an exponent or add it
+ * to terms of order one. */
+__device__ static inline double mx_log(double x)
+{
+    double m = __builtin_amdgcn_frexp_mant(x);
+    int e = __builtin_amdgcn_frexp_exp(x);
+    float l = __builtin_amdgcn_logf((float)m);
+    return ((double)e + (double)l) * OCO_LN2;
+}
+
+/* 1 - exp(-y) = y (1 - y/2 + y^2/6 - ...) for |y| < 1/2: the Taylor series to
+ * degree 8 (truncation below 1.1e-8 relative, under the 1.2e-7 of the FP32
+ * exp core it replaces), Horner with FMA */
+__device__ static inline double mx_series_omexp(double y)
+{
+    double q = (1.0 / 362880);
+    q = __builtin_fma(y, q, -(1.0 / 40320));
+    q = __builtin_fma(y, q, (1.0 / 5040));
+    q = __builtin_fma(y, q, -(1.0 / 720));
+    q = __builtin_fma(y, q, (1.0 / 120));
+    q = __builtin_fma(y, q, -(1.0 / 24));
+    q = __builtin_fma(y, q, (1.0 / 6));
+    q = __builtin_fma(y, q, -0.5);
+    q = __builtin_fma(y, q, 1.0);
+    return y * q;
+}
+
+/* 1 - exp(-y), y >= 0 */
+__device__ static inline double mx_omexp(double y)
+{
+    double s = mx_series_omexp(y);
+    double l = 1.0 - mx_exp(-y);
+    return y < 0.5 ? s : l;
+}
+
+/* exp(d) - 1, d >= 0 */
+__device__ static inline double mx_expm1(double d)
+{
+    double s = mx_series_omexp(-d);   /* 1 - exp(d) */
+    double l = mx_exp(d) - 1.0;
+    return d < 0.5 ? -s : l;
+}
+
+/* 1/y and x/y: v_rcp_f64 refined by one Newton step */
+__device__ static inline double mx_rcp(double y)
+{
+    double r = __builtin_amdgcn_rcp(y);
+    return __builtin_fma(r, __builtin_fma(-y, r, 1.0), r);
+}
+__device__ static inline double mx_div(double x, double y) { return x * mx_rcp(y); }
+
+/* the opacity, albedo and source function of one (cell, wavelength), mixed
+ * precision; same formulas as oco_point_s (source/continuous_opacity.f90) */
+template <int WS>
+__device__ static inline oco_pt oco_point_mixed(const double* c, const double* w)
+{
+    oco_pt p;
+    double theta = c[CR_THETA], fn = c[CR_FN];
+    double stim = mx_omexp((w[WT_CHI * WS] * theta) * OCO_LN10);
+    double sm = w[WT_G1 * WS];
+    sm = sm + w[WT_G2 * WS] * c[CR_E2];
+    sm = sm + w[WT_G3 * WS] * c[CR_E3];
+    sm = sm + w[WT_G4 * WS] * c[CR_E4];
+    sm = sm + w[WT_G5 * WS] * c[CR_E5];
+    sm = sm + w[WT_G6 * WS] * c[CR_E6];
+    double kHbf = (((OCO_AHBF * stim) * fn) * w[WT_WA3 * WS]) * (c[CR_UNSOLD] + sm);
+    double kHmbf = (w[WT_HMBF_ON * WS] != 0.0) ? ((c[CR_HMBF] * stim) * fn) * w[WT_SCOEFF * WS] : 0.0;
+    double flam = ((w[WT_F1 * WS] + w[WT_F2 * WS] * c[CR_LGT]) + w[WT_F3 * WS] * c[CR_LGT2]) - 26.0;
+    double kHmff = c[CR_PHMFF] * mx_exp10(flam);
+    double kappa = (((kHbf + kHmbf) + kHmff) + c[CR_ES]) * OCO_MP1;
+    double rho = c[CR_RHO];
+    double alpha = kappa * rho;
+    if (c[CR_NEUTRAL] != 0.0) {
+        double g_ff = 1.0 + w[WT_GFF * WS] * ((c[CR_U0] * w[WT_CHI1 * WS]) + 0.5);
+        double kHff = ((((stim * fn) * OCO_ALPHA0) * w[WT_WA3 * WS]) * g_ff) * c[CR_U1];
+        kappa = kappa + kHff * OCO_MP1;
+        alpha = alpha + kappa * rho;
+    } else {
+        double inner = w[WT_NU9 * WS] * c[CR_U0];
+        double gB = mx_log(mx_exp(5.960 - (OCO_SQRT3 * OCO_PI1) * mx_log(inner)) + OCO_EXP1);
+        double lnQ = (c[CR_U1] - w[WT_L3NU * WS]) + mx_log(mx_omexp(w[WT_HNK * WS] * c[CR_U2]));
+        double kRho = (OCO_C_BREMS * mx_exp(lnQ)) * gB;
+        alpha = alpha + kRho;
+        kappa = kappa + alpha * c[CR_U3];
+    }
+    p.albedo = mx_div(c[CR_ESM], kappa);
+    p.alpha = alpha;
+    p.alpha1 = mx_rcp(alpha);
+    p.alpha2 = alpha * alpha;
+    double damp = c[CR_DT1] * w[WT_WCM1 * WS];
+    if (damp > OCO_LOG_OVF) damp = OCO_LOG_OVF;
+    p.S = mx_div(w[WT_SFW5 * WS], mx_expm1(damp));
+    return p;
+}
+
+#endif

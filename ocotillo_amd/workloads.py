@@ -83,3 +83,46 @@ def config3(ncol=4096, seed=SEED + 3, nz_range=(32, 1024), nw_range=(16, 256), s
         waves.append(log_waves(int(nw[c])))
     return (nz.astype(np.int32), nw.astype(np.int32), np.concatenate(waves), np.array(dzs),
             np.concatenate(rhos), np.concatenate(Ts))
+
+
+T_MID = 61341.0            # midplane temperature of samples/NongreyAGN/input.in:14 (K)
+
+
+def gaussian_random_field(shape, corr_cells, rng):
+    """Unit-variance Gaussian random field with Gaussian covariance
+    exp(-r^2 / (2 corr^2)) (r in cells): white noise filtered in Fourier space
+    by exp(-k^2 corr^2 / 4), periodic, then normalised to zero mean and unit
+    variance."""
+    noise = rng.standard_normal(shape)
+    f = np.fft.rfftn(noise)
+    del noise
+    k2 = np.zeros(f.shape)
+    for ax, n in enumerate(shape):
+        k = 2.0 * np.pi * (np.fft.rfftfreq(n) if ax == len(shape) - 1 else np.fft.fftfreq(n))
+        sh = [1] * len(shape)
+        sh[ax] = k.size
+        k2 = k2 + (k * k).reshape(sh)
+    f *= np.exp(-k2 * (corr_cells * corr_cells) / 4.0)
+    del k2
+    g = np.fft.irfftn(f, s=shape, axes=tuple(range(len(shape))))
+    g -= g.mean()
+    g /= g.std()
+    return g
+
+
+def config4(nx=256, ny=256, nz=512, nw=128, seed=SEED + 4, sigma_dex=1.0, corr_cells=8.0,
+            rho0=1e-9, H=AU, rho_floor=RHO_FLOOR):
+    """Athena-like snapshot for the mixed-precision comparison: Gaussian
+    background rho0 exp(-z^2/2H^2) times a lognormal 3-D Gaussian random field
+    (sigma_dex dex, correlation length corr_cells cells) over the whole
+    [ix, iy, z] box, floored at rho_floor; T = T_MID (rho/rho0)^0.3 clipped
+    to [2000, 2e5] K.  z on [-5e15, 5e15], nw log-spaced 100 A - 10 um.
+    Returns rho, T ([nx, ny, nz]), dz and waves."""
+    rng = np.random.default_rng(seed)
+    z, dz = z_grid(nz)
+    g = gaussian_random_field((nx, ny, nz), corr_cells, rng)
+    rho = rho0 * np.exp(-z ** 2 / (2 * H ** 2)) * 10.0 ** (sigma_dex * g)
+    del g
+    np.maximum(rho, rho_floor, out=rho)
+    T = np.clip(T_MID * (rho / rho0) ** 0.3, T_FLOOR, 2e5)
+    return rho, T, dz, log_waves(nw)
