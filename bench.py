@@ -137,6 +137,16 @@ def main():
     return bench_block(args)
 
 
+def mixed_kernel_name(nz):
+    """The mixed-mode sweep the library picks for columns of nz cells (the
+    wavefront-per-system sweep up to 5 rows per lane, else the lane sweep;
+    OCO_MIXED_SWEEP=wave|lane forces one)."""
+    kind = os.environ.get("OCO_MIXED_SWEEP", "")
+    rows = ((nz + 63) // 64) | 1
+    wave = kind == "wave" or (kind != "lane" and rows <= 5 and rows <= 17)
+    return "sweep_mixed_wave" if wave else "sweep_mixed"
+
+
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
@@ -198,7 +208,7 @@ def bench_block(args):
         t1 = time.perf_counter()
         ms = (t1 - t0) * 1e3 / args.steps
         sw = float(np.mean(sweep_ms))
-        kern = "sweep_mixed" if mode == "mixed" else "sweep_fp64"
+        kern = mixed_kernel_name(nz) if mode == "mixed" else "sweep_fp64"
         traffic = SWEEP_TRAFFIC_BYTES if (mode == "fp64" and args.config == "c2") else None
         roof = roofline_of(kern, points, point_weights(mode, fion), sw, traffic)
         roof["cells_kernel_ms"] = round(float(np.mean(cells_ms)), 4)

@@ -5,6 +5,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <new>
 #include <numeric>
@@ -34,6 +35,7 @@ struct oco_ctx {
     double fully_ionized_T = 2e4;
     int lgrey = 0;
     double sigma_grey = 0.4;
+    int mixed_sweep = 0;  /* mixed mode: 0 by column length, 1 wavefront per system, 2 lane per system */
     DevBuf wtab, rho, T, cells, scratch, U, V, A, fail, canary;
 };
 
@@ -182,6 +184,11 @@ int oco_create(int device, size_t hbm_budget, oco_ctx** out)
     if (!c) return OCO_E_OOM;
     c->device = device;
     c->budget = hbm_budget;
+    /* OCO_MIXED_SWEEP=wave|lane forces one of the two mixed-mode sweeps (both
+     * give results within the mixed mode's stated bounds); default: by
+     * column length */
+    if (const char* e = getenv("OCO_MIXED_SWEEP"))
+        c->mixed_sweep = strcmp(e, "wave") == 0 ? 1 : strcmp(e, "lane") == 0 ? 2 : 0;
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
         delete c;
         return OCO_E_HIP;
@@ -253,13 +260,24 @@ int oco_solve_block_device(oco_ctx* c, int nz, int nyloc, int nxloc, double dz, 
     const long ntiles = (nsys + 63) / 64;
     const int mixed = mode == OCO_MODE_MIXED;
     const int uniform = (c->nw % 64) == 0;
-    const int grid = oco_sweep_grid(c->lgrey, mixed, uniform, c->device);
-    int nblocks = (int)(ntiles < grid ? ntiles : grid);
+    /* mixed, non-grey: one wavefront per system when its LDS fits */
+    const int wave = mixed && !c->lgrey && c->mixed_sweep != 2 && oco_wave_fits(nz, c->device) &&
+                     (c->mixed_sweep == 1 || oco_wave_preferred(nz));
+    int nblocks;
+    if (wave) {
+        const int grid = oco_wave_grid(nz, c->device);
+        nblocks = (int)(nsys < grid ? nsys : grid);
+    } else {
+        const int grid = oco_sweep_grid(c->lgrey, mixed, uniform, c->device);
+        nblocks = (int)(ntiles < grid ? ntiles : grid);
+    }
     int st;
     if (!c->lgrey && (st = ensure(c, c->cells, (size_t)ncells * OCO_CELL_STRIDE * sizeof(double)))) return st;
-    /* one scratch slot per wavefront; multi-wavefront workgroups round up */
-    const size_t slots = ((size_t)nblocks + 7) / 8 * 8;
-    if ((st = ensure(c, c->scratch, slots * nz * 3 * 64 * sizeof(double)))) return st;
+    if (!wave) {
+        /* one scratch slot per wavefront; multi-wavefront workgroups round up */
+        const size_t slots = ((size_t)nblocks + 7) / 8 * 8;
+        if ((st = ensure(c, c->scratch, slots * nz * 3 * 64 * sizeof(double)))) return st;
+    }
     if ((st = ensure(c, c->fail, sizeof(unsigned long long)))) return st;
     if ((st = ensure(c, c->canary, (4 + 5 * (size_t)nz) * sizeof(double)))) return st;
 
@@ -274,6 +292,7 @@ int oco_solve_block_device(oco_ctx* c, int nz, int nyloc, int nxloc, double dz, 
     a.dz1 = dz1;
     a.dz2 = dz2;
     a.sigma_grey = c->sigma_grey;
+    a.fully_ionized_T = c->fully_ionized_T;
     a.rho = d_rho;
     a.T = d_T;
     a.cells = (const double*)c->cells.p;
@@ -286,9 +305,11 @@ int oco_solve_block_device(oco_ctx* c, int nz, int nyloc, int nxloc, double dz, 
     a.canary = (double*)c->canary.p;
 
     HIPCHK(hipEventRecord(c->ev[0], c->stream));
-    if (!c->lgrey) HIPCHK(oco_launch_cells(d_rho, d_T, ncells, c->fully_ionized_T, (double*)c->cells.p, c->stream));
+    if (!c->lgrey)
+        HIPCHK(oco_launch_cells(d_rho, d_T, ncells, c->fully_ionized_T, (double*)c->cells.p, wave ? nz : 0,
+                                c->stream));
     HIPCHK(hipEventRecord(c->ev[1], c->stream));
-    HIPCHK(oco_launch_sweep(a, c->lgrey, mixed, nblocks, c->stream));
+    HIPCHK(oco_launch_sweep(a, c->lgrey, mixed, wave, nblocks, c->stream));
     HIPCHK(hipEventRecord(c->ev[2], c->stream));
 
     unsigned long long key = none;
@@ -433,7 +454,7 @@ static hipError_t batch_enqueue(oco_batch* b, int mixed, int variant, int nblock
 {
     oco_ctx* c = b->ctx;
     hipError_t e = oco_launch_cells((const double*)b->rho.p, (const double*)b->T.p, b->ncells, c->fully_ionized_T,
-                                    (double*)b->cells.p, c->stream);
+                                    (double*)b->cells.p, 0, c->stream);
     if (e != hipSuccess) return e;
     return oco_launch_batch(batch_args(b), mixed, variant, nblocks, c->stream);
 }

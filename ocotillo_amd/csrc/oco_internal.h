@@ -12,9 +12,10 @@ struct oco_sweep_args {
     long ncol;           /* columns in the block (nxloc*nyloc) */
     double dz, dz1, dz2;
     double sigma_grey;
+    double fully_ionized_T;
     const double* rho;   /* (nz, ncol) */
     const double* T;     /* (nz, ncol) */
-    const double* cells; /* cell records, (ncol*nz, OCO_CELL_STRIDE) */
+    const double* cells; /* cell records: (ncol*nz, OCO_CELL_STRIDE), or field-major for the wave sweep */
     const double* wtab;  /* (nw, OCO_WT_STRIDE) */
     double* U;           /* (nz+6, ncol, nw) */
     double* V;           /* (nz, ncol, nw) */
@@ -52,9 +53,17 @@ struct oco_batch_args {
 /* mixed != 0 selects the mixed-precision kernels (non-grey only). */
 hipError_t oco_launch_batch(const oco_batch_args& a, int mixed, int variant, int nblocks, hipStream_t stream);
 int oco_batch_variants(void);
+/* soa_nz > 0: field-major records per column of soa_nz cells (field k of
+ * cell z of column c at (c*CR_NFIELD + k)*soa_nz + z); 0: cell-major */
 hipError_t oco_launch_cells(const double* rho, const double* T, long ncells, double fully_ionized_T,
-                            double* cells, hipStream_t stream);
-hipError_t oco_launch_sweep(const oco_sweep_args& a, int lgrey, int mixed, int nblocks, hipStream_t stream);
+                            double* cells, int soa_nz, hipStream_t stream);
+/* wave != 0 (mixed, non-grey): the wavefront-per-system sweep, which reads
+ * field-major cell records and needs no scratch */
+hipError_t oco_launch_sweep(const oco_sweep_args& a, int lgrey, int mixed, int wave, int nblocks, hipStream_t stream);
+int oco_wave_fits(int nz, int device);
+/* 1 when the wavefront sweep is the faster mixed sweep for nz (measured) */
+int oco_wave_preferred(int nz);
+int oco_wave_grid(int nz, int device);
 int oco_sweep_grid(int lgrey, int mixed, int uniform, int device);
 
 /* failure key layout: system index << 32 | stage << 28 | row */

@@ -40,17 +40,31 @@ __device__ static inline void load_mtab(oco_mtab* smt)
     __syncthreads();
 }
 
+/* SOA = false: records of OCO_CELL_STRIDE doubles, cell-major (read by the
+ * lane-per-system sweeps, a wavefront's 64 lanes sharing one cell);
+ * SOA = true: field-major per column, field k of cell z of column c at
+ * (c*CR_NFIELD + k)*nz + z (read by the wavefront-per-system sweep, 64 lanes
+ * on 64 consecutive cells of one column). */
+template <bool SOA>
 __global__ __launch_bounds__(256) void cells_fp64(const double* __restrict__ rho, const double* __restrict__ T,
-                                                  long ncells, double fully_ionized_T, double* __restrict__ cells)
+                                                  long ncells, int nz, double fully_ionized_T,
+                                                  double* __restrict__ cells)
 {
     __shared__ oco_mtab smt;
     load_mtab(&smt);
     for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < ncells; i += (long)gridDim.x * blockDim.x) {
         double rec[CR_NFIELD];
         oco_cell_record(rho[i], T[i], fully_ionized_T, &smt, rec);
-        double* out = cells + i * OCO_CELL_STRIDE;
+        if (SOA) {
+            const long col = i / nz;
+            double* out = cells + col * (long)CR_NFIELD * nz + (i - col * nz);
 #pragma unroll
-        for (int k = 0; k < CR_NFIELD; k++) out[k] = rec[k];
+            for (int k = 0; k < CR_NFIELD; k++) out[(long)k * nz] = rec[k];
+        } else {
+            double* out = cells + i * OCO_CELL_STRIDE;
+#pragma unroll
+            for (int k = 0; k < CR_NFIELD; k++) out[k] = rec[k];
+        }
     }
 }
 
@@ -100,16 +114,6 @@ __device__ static inline void record_fail(unsigned long long* fail, long sys, in
     atomicMin(fail, key);
 }
 
-/* x / y: the IEEE divide of the FP64 path, or the reciprocal-based divide of
- * the mixed path (MIX is a compile-time switch; the FP64 path compiles to
- * exactly the reference's operations). */
-template <bool MIX>
-__device__ static inline double qdiv(double x, double y)
-{
-    if constexpr (MIX) return mx_div(x, y);
-    else return x / y;
-}
-
 /* ---- coalesced output stores ----
  * A lane owns one system, and its outputs U(:), V(:), alpha(:) are contiguous
  * runs, but consecutive lanes' runs lie (nz+6)*ncol doubles apart in the
@@ -119,14 +123,24 @@ __device__ static inline double qdiv(double x, double y)
  * instruction then covers 64/R whole runs of R consecutive doubles.
  * Every lane of the wavefront takes part in a flush, so lanes without a
  * system of their own run a duplicate one with their stores switched off. */
+/* rows staged per lane: FP64 kernels also hold the elementary-function tables
+ * in LDS, so they stage 8 rows (64-byte runs); the mixed kernels stage 16
+ * (128-byte runs, whole cache lines) */
 #ifndef OCO_STAGE_ROWS
 #define OCO_STAGE_ROWS 8
 #endif
-struct oco_stage {
-    double ub[OCO_WAVE * (OCO_STAGE_ROWS + 1)];   /* U rows; alpha rows during the forward pass */
-    double vb[OCO_WAVE * (OCO_STAGE_ROWS + 1)];   /* V rows */
-    double* base[3][OCO_WAVE];                    /* per lane: U, V, alpha; null = lane stores nothing */
+#ifndef OCO_STAGE_ROWS_MIXED
+#define OCO_STAGE_ROWS_MIXED 16
+#endif
+template <int R>
+struct oco_stage_t {
+    static constexpr int rows = R;
+    double ub[OCO_WAVE * (R + 1)];   /* U rows; alpha rows during the forward pass */
+    double vb[OCO_WAVE * (R + 1)];   /* V rows */
+    double* base[3][OCO_WAVE];       /* per lane: U, V, alpha; null = lane stores nothing */
 };
+typedef oco_stage_t<OCO_STAGE_ROWS> oco_stage;
+typedef oco_stage_t<OCO_STAGE_ROWS_MIXED> oco_stage_mixed;
 
 /* LDS traffic of one wavefront is performed in issue order, so ordering the
  * wavefront's own accesses needs only a compiler barrier at wavefront scope */
@@ -136,17 +150,19 @@ __device__ static inline void wave_sync()
     __builtin_amdgcn_wave_barrier();
 }
 
+template <int R>
 __device__ static inline void stage_put(double* buf, int idx, double v)
 {
-    buf[threadIdx.x * (OCO_STAGE_ROWS + 1) + (idx & (OCO_STAGE_ROWS - 1))] = v;
+    buf[threadIdx.x * (R + 1) + (idx & (R - 1))] = v;
 }
 
 /* write block b (elements b*R .. b*R+R-1, clipped to [0, len)) of every lane:
  * thread t handles element r = t mod R of lanes t/R + k*(64/R), k = 0..R-1,
  * so R consecutive threads store one lane's run */
+template <int R>
 __device__ static inline void stage_flush(const double* buf, double* const* base, int b, int len)
 {
-    constexpr int R = OCO_STAGE_ROWS, L = OCO_WAVE / OCO_STAGE_ROWS;
+    constexpr int L = OCO_WAVE / R;
     const int r = (int)threadIdx.x & (R - 1);
     const int s0 = (int)threadIdx.x / R;
     const double* bp = buf + s0 * (R + 1) + r;
@@ -163,14 +179,13 @@ __device__ static inline void stage_flush(const double* buf, double* const* base
 
 /* Row coefficients (source/auxiliary.f90:147-197) -- interior row from the
  * cell and its neighbours' alpha, boundary rows from the cell alone. */
-template <bool MIX = false>
 __device__ static inline void interior_row(double am, const oco_pt& pc, double ap, double dz2, double& av,
                                            double& bv, double& cv, double& dv)
 {
     double kp = 0.5 * (ap + pc.alpha);
     double km = 0.5 * (pc.alpha + am);
-    av = qdiv<MIX>(pc.alpha2, km);
-    cv = qdiv<MIX>(pc.alpha2, kp);
+    av = pc.alpha2 / km;
+    cv = pc.alpha2 / kp;
     double ze = (dz2 * ((pc.alpha * pc.alpha) * pc.alpha)) * (1.0 - pc.albedo);
     bv = -((av + cv) + ze);
     dv = -(pc.S * ze);
@@ -181,6 +196,26 @@ __device__ static inline void boundary_row(const oco_pt& pc, double dz, double d
     double ze = ((pc.alpha * dz2) * (1.0 - pc.albedo)) * 0.25;
     bv = -((pc.alpha1 + dz) + ze) * pc.alpha2;
     dv = -((pc.S * ze) * pc.alpha2);
+}
+
+/* The same rows in difference form (mixed mode): -A x[i-1] + (A + C + H) x[i]
+ * - C x[i+1] = R with A, C, H, R >= 0; H is the row's excess over A + C
+ * (zeta inside; alpha^2 (dz + zeta) at the boundary rows, whose coupling is
+ * alpha). */
+__device__ static inline void rh_interior_row(double am, const oco_pt& pc, double ap, double dz2, double& A,
+                                              double& C, double& H, double& R)
+{
+    A = pc.alpha2 * mx_rcp(0.5 * (pc.alpha + am));
+    C = pc.alpha2 * mx_rcp(0.5 * (ap + pc.alpha));
+    H = (dz2 * (pc.alpha2 * pc.alpha)) * (1.0 - pc.albedo);
+    R = pc.S * H;
+}
+
+__device__ static inline void rh_boundary_row(const oco_pt& pc, double dz, double dz2, double& H, double& R)
+{
+    const double ze = ((pc.alpha * dz2) * (1.0 - pc.albedo)) * 0.25;
+    H = (dz + ze) * pc.alpha2;
+    R = (pc.S * ze) * pc.alpha2;
 }
 
 template <bool GREY, bool UNIFORM, int WS = 1>
@@ -215,15 +250,15 @@ __device__ static inline oco_pt point_sys(const oco_sys& d, int z, const double*
  * opacity of the cells recomputed in registers, then back-substitution fused
  * with ghosts, der and V.  `w` is the lane's wavelength-table row with
  * stride WS; `scr` its scratch column (stride 64 doubles per entry).
- * MIX selects the mixed-precision opacity and reciprocal-based divides; the
- * elimination itself stays FP64 either way. */
-template <bool GREY, bool UNIFORM, int WS, int BC = OCO_BACK_CHUNK, bool MIX = false>
+ * MIX selects the mixed-precision opacity and the difference-form (FP64)
+ * elimination. */
+template <bool GREY, bool UNIFORM, int WS, int BC, bool MIX, class STAGE>
 __device__ static inline void solve_system(const oco_sys& d, bool valid, const double* w, double* scr,
-                                           unsigned long long* fail, const oco_mtab* M, oco_stage* S)
+                                           unsigned long long* fail, const oco_mtab* M, STAGE* S)
 {
     const int nz = d.nz;
     const double dz = d.dz, dz1 = d.dz1, dz2 = d.dz2;
-    constexpr int R = OCO_STAGE_ROWS;
+    constexpr int R = STAGE::rows;
     const int lane = threadIdx.x;
     wave_sync();   /* the previous system's flushes have read the buffers */
     S->base[0][lane] = valid ? d.U : nullptr;
@@ -242,16 +277,31 @@ __device__ static inline void solve_system(const oco_sys& d, bool valid, const d
     if (!GREY) load_rec<UNIFORM>(d.cells, 2, rec);
 
     double bv, dv, av, cv;
-    boundary_row(pB, dz, dz2, bv, dv);      /* row 0: a = 0, c = alpha */
-    cv = pB.alpha;
-    int fail_stage = (bv == 0.0) ? 1 : 0;
+    int fail_stage = 0;
     int fail_row = 1;
-    double bet = bv;
-    double u = qdiv<MIX>(dv, bet);
+    double bet = 0.0, u;
+    /* mixed: difference form, x[j] = D[j] x[j+1] + Z[j] with om = 1 - D[j-1]
+     * carried separately; u holds Z, and the gam slot of row j holds
+     * -D[j-1], so the back pass U[j] = u[j] - gam[j+1] U[j+1] is shared */
+    double om = 1.0, Dprev = 0.0;
+    if constexpr (MIX) {
+        double H, R;
+        rh_boundary_row(pB, dz, dz2, H, R);  /* row 0: A = 0, C = alpha */
+        const double r = mx_rcp(pB.alpha + H);
+        Dprev = pB.alpha * r;
+        u = R * r;
+        om = H * r;
+    } else {
+        boundary_row(pB, dz, dz2, bv, dv);  /* row 0: a = 0, c = alpha */
+        cv = pB.alpha;
+        fail_stage = (bv == 0.0) ? 1 : 0;
+        bet = bv;
+        u = dv / bet;
+    }
     scr[0 * OCO_WAVE] = 0.0;
     scr[1 * OCO_WAVE] = u;
     scr[2 * OCO_WAVE] = pB.alpha1;
-    stage_put(S->ub, 0, pB.alpha);
+    stage_put<R>(S->ub, 0, pB.alpha);
     double aA = pB.alpha;   /* alpha of row j-1 */
     pB = pC;                /* row j = 1 */
     pC = point_sys<GREY, WS, MIX>(d, 2, rec, w, M);
@@ -259,22 +309,33 @@ __device__ static inline void solve_system(const oco_sys& d, bool valid, const d
     for (int j = 1; j <= nz - 3; j++) {
         if (!GREY) load_rec<UNIFORM>(d.cells, j + 2, rec);
         /* row j and its elimination step */
-        double cprev = cv;
-        interior_row<MIX>(aA, pB, pC.alpha, dz2, av, bv, cv, dv);
-        double gam = qdiv<MIX>(cprev, bet);
-        bet = bv - av * gam;
-        int first = (bet == 0.0) && (fail_stage == 0);
-        fail_row = first ? j + 1 : fail_row;
-        fail_stage = first ? 2 : fail_stage;
-        u = qdiv<MIX>(dv - av * u, bet);
+        double gam;
+        if constexpr (MIX) {
+            double A, C, H, R;
+            rh_interior_row(aA, pB, pC.alpha, dz2, A, C, H, R);
+            const double r = mx_rcp((A * om + C) + H);
+            gam = -Dprev;
+            Dprev = C * r;
+            u = (R + A * u) * r;
+            om = (A * om + H) * r;
+        } else {
+            double cprev = cv;
+            interior_row(aA, pB, pC.alpha, dz2, av, bv, cv, dv);
+            gam = cprev / bet;
+            bet = bv - av * gam;
+            int first = (bet == 0.0) && (fail_stage == 0);
+            fail_row = first ? j + 1 : fail_row;
+            fail_stage = first ? 2 : fail_stage;
+            u = (dv - av * u) / bet;
+        }
         double* sp = scr + (long)j * 3 * OCO_WAVE;
         sp[0] = gam;
         sp[OCO_WAVE] = u;
         sp[2 * OCO_WAVE] = pB.alpha1;
-        stage_put(S->ub, j, pB.alpha);
+        stage_put<R>(S->ub, j, pB.alpha);
         if ((j & (R - 1)) == R - 1) {
             wave_sync();
-            stage_flush(S->ub, S->base[2], j / R, nz);
+            stage_flush<R>(S->ub, S->base[2], j / R, nz);
             wave_sync();
         }
         /* point j+2 (independent of the chain above) */
@@ -285,45 +346,66 @@ __device__ static inline void solve_system(const oco_sys& d, bool valid, const d
     }
     {   /* row nz-2 (interior, no new point) and row nz-1 (boundary) */
         int j = nz - 2;
-        double cprev = cv;
-        interior_row<MIX>(aA, pB, pC.alpha, dz2, av, bv, cv, dv);
-        double gam = qdiv<MIX>(cprev, bet);
-        bet = bv - av * gam;
-        int first = (bet == 0.0) && (fail_stage == 0);
-        fail_row = first ? j + 1 : fail_row;
-        fail_stage = first ? 2 : fail_stage;
-        u = qdiv<MIX>(dv - av * u, bet);
+        double gam;
+        if constexpr (MIX) {
+            double A, C, H, R;
+            rh_interior_row(aA, pB, pC.alpha, dz2, A, C, H, R);
+            const double r = mx_rcp((A * om + C) + H);
+            gam = -Dprev;
+            Dprev = C * r;
+            u = (R + A * u) * r;
+            om = (A * om + H) * r;
+        } else {
+            double cprev = cv;
+            interior_row(aA, pB, pC.alpha, dz2, av, bv, cv, dv);
+            gam = cprev / bet;
+            bet = bv - av * gam;
+            int first = (bet == 0.0) && (fail_stage == 0);
+            fail_row = first ? j + 1 : fail_row;
+            fail_stage = first ? 2 : fail_stage;
+            u = (dv - av * u) / bet;
+        }
         double* sp = scr + (long)j * 3 * OCO_WAVE;
         sp[0] = gam;
         sp[OCO_WAVE] = u;
         sp[2 * OCO_WAVE] = pB.alpha1;
-        stage_put(S->ub, j, pB.alpha);
+        stage_put<R>(S->ub, j, pB.alpha);
         if ((j & (R - 1)) == R - 1) {
             wave_sync();
-            stage_flush(S->ub, S->base[2], j / R, nz);
+            stage_flush<R>(S->ub, S->base[2], j / R, nz);
             wave_sync();
         }
         j = nz - 1;
-        cprev = cv;
-        boundary_row(pC, dz, dz2, bv, dv);
-        av = pC.alpha;                       /* a(nz) = alpha, c(nz) = 0 */
-        gam = qdiv<MIX>(cprev, bet);
-        bet = bv - av * gam;
-        first = (bet == 0.0) && (fail_stage == 0);
-        fail_row = first ? j + 1 : fail_row;
-        fail_stage = first ? 2 : fail_stage;
-        u = qdiv<MIX>(dv - av * u, bet);
+        if constexpr (MIX) {
+            double H, R;
+            rh_boundary_row(pC, dz, dz2, H, R);   /* A = alpha, C = 0 */
+            const double r = mx_rcp(pC.alpha * om + H);
+            gam = -Dprev;
+            u = (R + pC.alpha * u) * r;
+        } else {
+            double cprev = cv;
+            boundary_row(pC, dz, dz2, bv, dv);
+            av = pC.alpha;                       /* a(nz) = alpha, c(nz) = 0 */
+            gam = cprev / bet;
+            bet = bv - av * gam;
+            int first = (bet == 0.0) && (fail_stage == 0);
+            fail_row = first ? j + 1 : fail_row;
+            fail_stage = first ? 2 : fail_stage;
+            u = (dv - av * u) / bet;
+        }
         sp = scr + (long)j * 3 * OCO_WAVE;
         sp[0] = gam;
         sp[OCO_WAVE] = u;
         sp[2 * OCO_WAVE] = pC.alpha1;
-        stage_put(S->ub, j, pC.alpha);
+        stage_put<R>(S->ub, j, pC.alpha);
         wave_sync();
-        stage_flush(S->ub, S->base[2], j / R, nz);   /* last (possibly partial) block */
+        stage_flush<R>(S->ub, S->base[2], j / R, nz);   /* last (possibly partial) block */
     }
+    (void)om;
+    (void)Dprev;
     if (fail_stage != 0) {
         /* the reference stops at this system (source/auxiliary.f90:27-39):
-         * its U and V are not written */
+         * its U and V are not written (the difference form has no zero pivot) */
         if (valid) record_fail(fail, d.key, fail_stage, fail_row);
         S->base[0][lane] = nullptr;
         S->base[1][lane] = nullptr;
@@ -358,10 +440,10 @@ __device__ static inline void solve_system(const oco_sys& d, bool valid, const d
 #pragma unroll
         for (int k = 0; k < 7; k++) {
             const int e = nz + 5 - k;
-            stage_put(S->ub, e, top[k]);
+            stage_put<R>(S->ub, e, top[k]);
             if ((e & (R - 1)) == 0) {
                 wave_sync();
-                stage_flush(S->ub, S->base[0], e / R, nz + 6);
+                stage_flush<R>(S->ub, S->base[0], e / R, nz + 6);
                 wave_sync();
             }
         }
@@ -369,10 +451,10 @@ __device__ static inline void solve_system(const oco_sys& d, bool valid, const d
     double w0 = t[3], w1 = t[2], w2 = t[1], w3 = t[0], w4 = g1, w5 = g2, w6 = g3;
     {
         double df = c60 * (((45.0 * (w4 - w2)) - (9.0 * (w5 - w1))) + (w6 - w0));
-        stage_put(S->vb, nz - 1, ((-1.0 * a1q3) * df) * dz1);
+        stage_put<R>(S->vb, nz - 1, ((-1.0 * a1q3) * df) * dz1);
         if (((nz - 1) & (R - 1)) == 0) {
             wave_sync();
-            stage_flush(S->vb, S->base[1], (nz - 1) / R, nz);
+            stage_flush<R>(S->vb, S->base[1], (nz - 1) / R, nz);
             wave_sync();
         }
     }
@@ -404,14 +486,14 @@ __device__ static inline void solve_system(const oco_sys& d, bool valid, const d
                     gnext = cg[k];
                     a1q3 = a1q2; a1q2 = a1q1; a1q1 = a1q0; a1q0 = ca[k];
                     const int e = j + 3;   /* U(e) and V(z = e) */
-                    stage_put(S->ub, e, Uj);
+                    stage_put<R>(S->ub, e, Uj);
                     w6 = w5; w5 = w4; w4 = w3; w3 = w2; w2 = w1; w1 = w0; w0 = Uj;
                     double df = c60 * (((45.0 * (w4 - w2)) - (9.0 * (w5 - w1))) + (w6 - w0));
-                    stage_put(S->vb, e, ((-1.0 * a1q3) * df) * dz1);
+                    stage_put<R>(S->vb, e, ((-1.0 * a1q3) * df) * dz1);
                     if ((e & (R - 1)) == 0) {
                         wave_sync();
-                        stage_flush(S->ub, S->base[0], e / R, nz + 6);
-                        stage_flush(S->vb, S->base[1], e / R, nz);
+                        stage_flush<R>(S->ub, S->base[0], e / R, nz + 6);
+                        stage_flush<R>(S->vb, S->base[1], e / R, nz);
                         wave_sync();
                     }
                 }
@@ -422,26 +504,26 @@ __device__ static inline void solve_system(const oco_sys& d, bool valid, const d
     }
     /* bottom ghosts (f(n1-i) = 2 f(n1) - f(n1+i)); window now e = 3..9 */
     double u0 = w0, G1 = 2.0 * u0 - w1, G2 = 2.0 * u0 - w2, G3 = 2.0 * u0 - w3;
-    stage_put(S->ub, 2, G1);
-    stage_put(S->ub, 1, G2);
-    stage_put(S->ub, 0, G3);
+    stage_put<R>(S->ub, 2, G1);
+    stage_put<R>(S->ub, 1, G2);
+    stage_put<R>(S->ub, 0, G3);
     const double gh[3] = {G1, G2, G3};
 #pragma unroll
     for (int k = 0; k < 3; k++) {
         w6 = w5; w5 = w4; w4 = w3; w3 = w2; w2 = w1; w1 = w0; w0 = gh[k];
         a1q3 = a1q2; a1q2 = a1q1; a1q1 = a1q0;
         double df = c60 * (((45.0 * (w4 - w2)) - (9.0 * (w5 - w1))) + (w6 - w0));
-        stage_put(S->vb, 2 - k, ((-1.0 * a1q3) * df) * dz1);
+        stage_put<R>(S->vb, 2 - k, ((-1.0 * a1q3) * df) * dz1);
     }
     wave_sync();
-    stage_flush(S->ub, S->base[0], 0, nz + 6);
-    stage_flush(S->vb, S->base[1], 0, nz);
+    stage_flush<R>(S->ub, S->base[0], 0, nz + 6);
+    stage_flush<R>(S->vb, S->base[1], 0, nz);
 }
 
 /* One wavefront per workgroup, grid-stride over tiles of 64 systems; the
  * lane's wavelength-table row lives in registers. */
-template <bool GREY, bool UNIFORM, bool MIX>
-__device__ static inline void sweep_tiles(const oco_sweep_args& a, const oco_mtab* M, oco_stage* S)
+template <bool GREY, bool UNIFORM, bool MIX, class STAGE>
+__device__ static inline void sweep_tiles(const oco_sweep_args& a, const oco_mtab* M, STAGE* S)
 {
     const int lane = threadIdx.x;
     const int nw = a.nw;
@@ -478,7 +560,7 @@ __device__ static inline void sweep_tiles(const oco_sweep_args& a, const oco_mta
         d.V = a.V + osys * a.nz;
         d.A = a.A + osys * a.nz;
         d.key = col * nw + iw;
-        solve_system<GREY, UNIFORM, 1, OCO_BACK_CHUNK, MIX>(d, valid, w, scr, a.fail, M, S);
+        solve_system<GREY, UNIFORM, 1, OCO_BACK_CHUNK, MIX, STAGE>(d, valid, w, scr, a.fail, M, S);
     }
 }
 
@@ -498,13 +580,15 @@ __global__ __launch_bounds__(OCO_WAVE) OCO_SWEEP_ATTR void sweep_fp64(const oco_
 template <bool UNIFORM>
 __global__ __launch_bounds__(OCO_WAVE) OCO_SWEEP_ATTR void sweep_mixed(const oco_sweep_args a)
 {
-    __shared__ oco_stage stg;
+    __shared__ oco_stage_mixed stg;
     sweep_tiles<false, UNIFORM, true>(a, nullptr, &stg);
 }
 
 /* First-pillar canaries (source/flux_feautrier.f90:198-201): sum(a), sum(b),
- * sum(c), sum(d) of system (ix,iy,iw) = (1,1,1).  Printed diagnostics only
- * (Fortran leaves the summation order of SUM to the compiler), so points,
+ * sum(c), sum(d) of system (ix,iy,iw) = (1,1,1), from the FP64 opacity in
+ * either precision mode (the cell records are recomputed here, so the kernel
+ * does not depend on the record layout of the sweep).  Printed diagnostics
+ * only (Fortran leaves the summation order of SUM to the compiler), so points,
  * rows and the sums are done in parallel by one workgroup.  `buf` holds
  * 4 + 5*nz doubles: the sums, then alpha, 1/alpha, alpha^2, albedo, S. */
 template <bool GREY>
@@ -520,7 +604,7 @@ __global__ __launch_bounds__(256) void canary_fp64(const oco_sweep_args a, doubl
         for (int k = 0; k < OCO_WT_STRIDE; k++) w[k] = a.wtab[k];
     for (int z = tid; z < nz; z += blockDim.x) {
         double rec[CR_NFIELD];
-        if (!GREY) load_record<false>(a, 0, z, rec);
+        if (!GREY) oco_cell_record(a.rho[z], a.T[z], a.fully_ionized_T, &smt, rec);
         oco_pt p = point_rec<GREY, false>(a, 0, z, rec, w, &smt);
         double* q = pts + 5 * z;
         q[0] = p.alpha; q[1] = p.alpha1; q[2] = p.alpha2; q[3] = p.albedo; q[4] = p.S;
@@ -555,6 +639,249 @@ __global__ __launch_bounds__(256) void canary_fp64(const oco_sweep_args a, doubl
     if (tid < 4) buf[tid] = red[tid][0];
 }
 
+/* ---- mixed precision, one wavefront per system ----
+ * The lane-per-system sweep keeps every system's elimination state (gam, u,
+ * 1/alpha per row) in an HBM scratch slot: 48 bytes per point of extra
+ * traffic.  The mixed mode is not bound to the reference's elimination
+ * order, so here a whole wavefront solves one (column, wavelength) system
+ * with its state in LDS and registers, in the Rybicki-Hummer difference form
+ * (Rybicki & Hummer 1991, A&A 245, 171, appendix A): every row is written
+ *     -A x[i-1] + (A + C + H) x[i] - C x[i+1] = R,   A, C, H, R >= 0,
+ * with H the row's excess over A + C (zeta; at the boundary rows
+ * alpha^2 (dz + ze)).  The elimination never forms A + C + H and subtracts,
+ * so it keeps full precision where H << A + C (optically thin cells, where
+ * the reference's b = -((a + c) + zeta) rounds zeta away).
+ *   1. points, lane z mod 64: opacity from the column's field-major cell
+ *      records (coalesced loads); alpha, albedo and S go to LDS, alpha also
+ *      to the output;
+ *   2. lane l takes m consecutive rows (m odd: chunk strides hit distinct
+ *      LDS banks) and eliminates them keeping the previous chunk's last
+ *      unknown xL as a parameter: x[i] = D x[i+1] + W xL + Z, with the
+ *      deficit E = 1 - D - W carried as its own positive quantity;
+ *      products along the chunk give its first row in terms of xL and the
+ *      chunk's last unknown;
+ *   3. the chunks' last unknowns G satisfy a 64-row system of the same form,
+ *      solved across lanes by parallel cyclic reduction in difference form
+ *      (H' = H + k1 H[-s] + k2 H[+s]: no subtraction at any step);
+ *   4. back-substitution inside each chunk, ghosts, the 6th-order derivative
+ *      and V, with every output store coalesced.
+ * LDS: alpha [nz], albedo [nz], S -> U [nz + 6]; the chunk's D, W, Z stay in
+ * registers (MM rows at most). */
+__device__ static inline double shfl_up_id(double v, int s, double id)
+{
+    double r = __shfl_up(v, (unsigned)s, OCO_WAVE);
+    return (int)threadIdx.x >= s ? r : id;
+}
+
+__device__ static inline double shfl_down_id(double v, int s, double id)
+{
+    double r = __shfl_down(v, (unsigned)s, OCO_WAVE);
+    return (int)threadIdx.x + s < OCO_WAVE ? r : id;
+}
+
+/* rows per lane of the wavefront sweep for columns of nz cells */
+__host__ __device__ static inline int wave_rows(int nz)
+{
+    return ((nz + OCO_WAVE - 1) / OCO_WAVE) | 1;
+}
+
+/* one system, the whole wavefront; `w` is its wavelength-table row
+ * (wave-uniform), `cc` its column's field-major cell records (field k of
+ * cell z at cc[k*nz + z]) */
+template <int MM>
+__device__ static inline void solve_system_wave(const oco_sys& d, const double* cc, const double* w, double* lds)
+{
+    const int nz = d.nz, lane = threadIdx.x;
+    const double dz = d.dz, dz1 = d.dz1, dz2 = d.dz2;
+    double* LA = lds;              /* alpha */
+    double* LW = lds + nz;         /* albedo */
+    double* LS = lds + 2 * nz;     /* extended, e = z + 3: S, then U with ghosts */
+    const int m = wave_rows(nz);
+    const int z0 = lane * m < nz ? lane * m : nz;
+    const int z1 = z0 + m < nz ? z0 + m : nz;
+
+    wave_sync();   /* the previous system's last reads are done */
+    /* 1. points (source/continuous_opacity.f90:175-201) */
+    for (int z = lane; z < nz; z += OCO_WAVE) {
+        double rec[CR_NFIELD];
+#pragma unroll
+        for (int k = 0; k < CR_NFIELD; k++) rec[k] = cc[k * nz + z];
+        oco_pt p = oco_point_mixed<1>(rec, w);
+        LA[z] = p.alpha;
+        LW[z] = p.albedo;
+        LS[z + 3] = p.S;
+        d.A[z] = p.alpha;
+    }
+    wave_sync();
+    /* 2. rows (source/auxiliary.f90:147-197) in difference form and the
+     * chunk's forward elimination */
+    double Dr[MM], Wr[MM], Zr[MM];
+    double Wp = 1.0, Zp = 0.0, Ep = 0.0;            /* the parameter row x = xL */
+    double pr = 1.0, sP = 0.0, sD = 0.0, sE = 0.0;  /* first row: pr G + sP xL + sD, deficit sE */
+    double DL = 0.0, WL = 0.0, ZL = 0.0, EL = 0.0;  /* last row */
+#pragma unroll
+    for (int j = 0; j < MM; j++) {
+        const int z = z0 + j;
+        Dr[j] = 0.0; Wr[j] = 0.0; Zr[j] = 0.0;
+        if (z < z1) {
+            const double al = LA[z], om = LW[z], S = LS[z + 3];
+            const double al2 = al * al;
+            double Ac, Cc, H, R;
+            oco_pt pc;
+            pc.alpha = al; pc.albedo = om; pc.S = S; pc.alpha2 = al2; pc.alpha1 = 0.0;
+            if (z == 0 || z == nz - 1) {
+                rh_boundary_row(pc, dz, dz2, H, R);
+                Ac = z == 0 ? 0.0 : al;
+                Cc = z == 0 ? al : 0.0;
+            } else {
+                rh_interior_row(LA[z - 1], pc, LA[z + 1], dz2, Ac, Cc, H, R);
+            }
+            const double r = mx_rcp((Ac * (Wp + Ep) + Cc) + H);   /* 1 - D[i-1] = W + E */
+            const double D = Cc * r;
+            const double W = (Ac * Wp) * r;
+            const double Z = (R + Ac * Zp) * r;
+            const double E = (Ac * Ep + H) * r;
+            Dr[j] = D; Wr[j] = W; Zr[j] = Z;
+            if (z < z1 - 1) {
+                sP += pr * W;
+                sD += pr * Z;
+                sE += pr * E;
+                pr *= D;
+            } else {
+                DL = D; WL = W; ZL = Z; EL = E;
+            }
+            Wp = W; Zp = Z; Ep = E;
+        }
+    }
+    /* 3. reduced system in the chunks' last unknowns:
+     * -WL G[k-1] + (WL + DL Gn + EL + DL En) G[k] - DL Gn G[k+1] = ZL + DL Dn
+     * (Gn, Dn, En: the next chunk's first row) */
+    const double Gn = shfl_down_id(pr, 1, 1.0), Dn = shfl_down_id(sD, 1, 0.0);
+    const double En = shfl_down_id(sE, 1, 0.0);
+    double Ar = 0.0, Cr = 0.0, Hr = 1.0, Rr = 0.0;
+    if (z1 > z0) {
+        Ar = WL;
+        Cr = DL * Gn;
+        Hr = EL + DL * En;
+        Rr = ZL + DL * Dn;
+    }
+    (void)sP;
+#pragma unroll
+    for (int sft = 1; sft < OCO_WAVE; sft <<= 1) {
+        const double Am = shfl_up_id(Ar, sft, 0.0), Cm = shfl_up_id(Cr, sft, 0.0);
+        const double Hm = shfl_up_id(Hr, sft, 1.0), Rm = shfl_up_id(Rr, sft, 0.0);
+        const double Ap = shfl_down_id(Ar, sft, 0.0), Cp = shfl_down_id(Cr, sft, 0.0);
+        const double Hp = shfl_down_id(Hr, sft, 1.0), Rp = shfl_down_id(Rr, sft, 0.0);
+        const double k1 = Ar * mx_rcp((Am + Cm) + Hm);
+        const double k2 = Cr * mx_rcp((Ap + Cp) + Hp);
+        Ar = k1 * Am;
+        Cr = k2 * Cp;
+        Hr = (Hr + k1 * Hm) + k2 * Hp;
+        Rr = (Rr + k1 * Rm) + k2 * Rp;
+    }
+    const double G = Rr * mx_rcp((Ar + Cr) + Hr);
+    const double GL = shfl_up_id(G, 1, 0.0);
+    /* 4. back-substitution inside the chunk */
+    double x = G;
+#pragma unroll
+    for (int j = MM - 1; j >= 0; j--) {
+        const int z = z0 + j;
+        if (z < z1) {
+            if (z < z1 - 1) x = (Dr[j] * x + Wr[j] * GL) + Zr[j];
+            LS[z + 3] = x;
+        }
+    }
+    wave_sync();
+    /* ghosts (source/auxiliary.f90:49-62) */
+    if (lane < 3) {
+        const int i = lane + 1;   /* f(n2+i) = 2 f(n2) - f(n2-i) */
+        LS[nz + 2 + i] = 2.0 * LS[nz + 2] - LS[nz + 2 - i];
+    } else if (lane < 6) {
+        const int i = lane - 2;   /* f(n1-i) = 2 f(n1) - f(n1+i) */
+        LS[3 - i] = 2.0 * LS[3] - LS[3 + i];
+    }
+    wave_sync();
+    /* der and calc_flux (:64-78, :96-109) */
+    const double c60 = 1.0 / 60.0;
+    for (int z = lane; z < nz; z += OCO_WAVE) {
+        const double* u = LS + z + 3;
+        double df = c60 * (((45.0 * (u[1] - u[-1])) - (9.0 * (u[2] - u[-2]))) + (u[3] - u[-3]));
+        d.V[z] = ((-1.0 * mx_rcp(LA[z])) * df) * dz1;
+    }
+    for (int e = lane; e < nz + 6; e += OCO_WAVE) d.U[e] = LS[e];
+}
+
+/* Mixed-precision sweep, one wavefront per system; systems in (column,
+ * wavelength) order so the wavefronts in flight share columns in L2. */
+template <int MM>
+__global__ __launch_bounds__(OCO_WAVE) void sweep_mixed_wave(const oco_sweep_args a)
+{
+    extern __shared__ double lds[];
+    const int nw = a.nw;
+    const long nsys = a.ncol * nw;
+    for (long s = blockIdx.x; s < nsys; s += gridDim.x) {
+        const long col = s / nw;
+        const int iw = (int)(s % nw);
+        double w[OCO_WT_STRIDE];
+        oco_cptr wp = (oco_cptr)(a.wtab + (long)iw * OCO_WT_STRIDE);
+#pragma unroll
+        for (int k = 0; k < OCO_WT_STRIDE; k++) w[k] = wp[k];
+        const long osys = (long)iw * a.ncol + col;
+        oco_sys d;
+        d.nz = a.nz;
+        d.dz = a.dz; d.dz1 = a.dz1; d.dz2 = a.dz2; d.sigma_grey = 0.0;
+        d.cells = nullptr; d.rho = nullptr; d.T = nullptr;
+        d.U = a.U + osys * (a.nz + 6);
+        d.V = a.V + osys * a.nz;
+        d.A = a.A + osys * a.nz;
+        d.key = s;
+        solve_system_wave<MM>(d, a.cells + col * (long)CR_NFIELD * a.nz, w, lds);
+    }
+}
+
+static size_t wave_lds_bytes(int nz) { return (3 * (size_t)nz + 6) * sizeof(double); }
+
+#define OCO_WAVE_MAX_ROWS 17
+
+/* 1 when the wavefront-per-system mixed sweep takes columns of nz cells:
+ * at most OCO_WAVE_MAX_ROWS rows per lane and its LDS within a workgroup's */
+int oco_wave_fits(int nz, int device)
+{
+    int lim = 0;
+    if (hipDeviceGetAttribute(&lim, hipDeviceAttributeMaxSharedMemoryPerBlock, device) != hipSuccess || lim <= 0)
+        lim = 65536;
+    return nz >= 4 && wave_rows(nz) <= OCO_WAVE_MAX_ROWS && wave_lds_bytes(nz) <= (size_t)lim;
+}
+
+/* Measured on MI355X (one 64x64-column block, nw = 128): the wavefront sweep
+ * wins while a lane holds few rows (nz = 256: 2.5 ms vs 3.1 ms); at 9 rows
+ * per lane (nz = 512) its cross-lane reduction and the opacity branches that
+ * diverge inside a 64-cell group cost more than the lane sweep's scratch
+ * traffic. */
+#ifndef OCO_WAVE_AUTO_ROWS
+#define OCO_WAVE_AUTO_ROWS 5
+#endif
+int oco_wave_preferred(int nz) { return wave_rows(nz) <= OCO_WAVE_AUTO_ROWS; }
+
+static const void* wave_kernel(int nz)
+{
+    const int m = wave_rows(nz);
+    if (m <= 5) return (const void*)sweep_mixed_wave<5>;
+    if (m <= 9) return (const void*)sweep_mixed_wave<9>;
+    return (const void*)sweep_mixed_wave<17>;
+}
+
+int oco_wave_grid(int nz, int device)
+{
+    int per_cu = 0, cus = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, wave_kernel(nz), OCO_WAVE, wave_lds_bytes(nz)) !=
+            hipSuccess || per_cu < 1)
+        per_cu = 1;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess || cus < 1)
+        cus = 256;
+    return per_cu * cus;
+}
+
 /* ---- batched uneven columns ----
  * Tiles are (column, block of 64 wavelengths), sorted longest column first
  * on the host; a persistent grid draws them from an atomic counter, so
@@ -572,8 +899,8 @@ typedef const __attribute__((address_space(4))) int* oco_cptr_i;
 typedef const __attribute__((address_space(4))) long* oco_cptr_l;
 typedef const __attribute__((address_space(4))) double* oco_cptr_d;
 
-template <int BC, bool MIX>
-__device__ static inline void batch_tiles(const oco_batch_args& a, const oco_mtab* M, int* tile_s, oco_stage* S)
+template <int BC, bool MIX, class STAGE>
+__device__ static inline void batch_tiles(const oco_batch_args& a, const oco_mtab* M, int* tile_s, STAGE* S)
 {
     const int lane = threadIdx.x;
     double* scr = a.scratch + (long)blockIdx.x * a.scr_stride + lane;
@@ -615,7 +942,7 @@ __device__ static inline void batch_tiles(const oco_batch_args& a, const oco_mta
             const double* wr = a.wtab + (woff[col] + iw) * OCO_WT_STRIDE;
 #pragma unroll
             for (int k = 0; k < OCO_WT_STRIDE; k++) w[k] = wr[k];
-            solve_system<false, true, 1, BC, MIX>(d, valid, w, scr, a.fail, M, S);
+            solve_system<false, true, 1, BC, MIX, STAGE>(d, valid, w, scr, a.fail, M, S);
         }
     }
 }
@@ -634,7 +961,7 @@ template <int BC>
 __global__ __launch_bounds__(OCO_WAVE) void batch_mixed(const oco_batch_args a)
 {
     __shared__ int tile_s;
-    __shared__ oco_stage stg;
+    __shared__ oco_stage_mixed stg;
     batch_tiles<BC, true>(a, nullptr, &tile_s, &stg);
 }
 
@@ -661,13 +988,17 @@ hipError_t oco_launch_batch(const oco_batch_args& a, int mixed, int variant, int
 int oco_batch_variants(void) { return 3; }
 
 hipError_t oco_launch_cells(const double* rho, const double* T, long ncells, double fully_ionized_T,
-                            double* cells, hipStream_t stream)
+                            double* cells, int soa_nz, hipStream_t stream)
 {
     if (ncells <= 0) return hipSuccess;
     long blocks = (ncells + 255) / 256;
     if (blocks > 65536) blocks = 65536;
-    hipLaunchKernelGGL(cells_fp64, dim3((unsigned)blocks), dim3(256), 0, stream, rho, T, ncells,
-                       fully_ionized_T, cells);
+    if (soa_nz > 0)
+        hipLaunchKernelGGL(cells_fp64<true>, dim3((unsigned)blocks), dim3(256), 0, stream, rho, T, ncells, soa_nz,
+                           fully_ionized_T, cells);
+    else
+        hipLaunchKernelGGL(cells_fp64<false>, dim3((unsigned)blocks), dim3(256), 0, stream, rho, T, ncells, 0,
+                           fully_ionized_T, cells);
     return hipGetLastError();
 }
 
@@ -688,7 +1019,7 @@ int oco_sweep_grid(int lgrey, int mixed, int uniform, int device)
     return per_cu * cus;
 }
 
-hipError_t oco_launch_sweep(const oco_sweep_args& a, int lgrey, int mixed, int nblocks, hipStream_t stream)
+hipError_t oco_launch_sweep(const oco_sweep_args& a, int lgrey, int mixed, int wave, int nblocks, hipStream_t stream)
 {
     const bool uniform = (a.nw % OCO_WAVE) == 0;
     if (a.canary) {
@@ -701,6 +1032,11 @@ hipError_t oco_launch_sweep(const oco_sweep_args& a, int lgrey, int mixed, int n
     if (lgrey) {
         /* grey opacity has no per-wavelength transcendentals: one kernel for both modes */
         hipLaunchKernelGGL((sweep_fp64<true, false>), g, b, 0, stream, a);
+    } else if (mixed && wave) {
+        const int m = wave_rows(a.nz);
+        if (m <= 5) hipLaunchKernelGGL(sweep_mixed_wave<5>, g, b, wave_lds_bytes(a.nz), stream, a);
+        else if (m <= 9) hipLaunchKernelGGL(sweep_mixed_wave<9>, g, b, wave_lds_bytes(a.nz), stream, a);
+        else hipLaunchKernelGGL(sweep_mixed_wave<17>, g, b, wave_lds_bytes(a.nz), stream, a);
     } else if (mixed) {
         if (uniform) hipLaunchKernelGGL((sweep_mixed<true>), g, b, 0, stream, a);
         else hipLaunchKernelGGL((sweep_mixed<false>), g, b, 0, stream, a);

@@ -346,10 +346,55 @@ void oo_ghosts_flux(int nz, double* U, const double* al1, double dz1, double* V)
     }
 }
 
-/* One (column, wavelength) system: source/flux_feautrier.f90:183-216. */
+/* The same rows in difference form (Rybicki & Hummer 1991, A&A 245, 171,
+ * appendix A): -A x[i-1] + (A + C + H) x[i] - C x[i+1] = R with A, C, H, R
+ * >= 0, i.e. a = A, c = C, b = -(A + C + H), d = -R; H is the row's excess
+ * over A + C (zeta inside, alpha^2 (dz + zeta) at the two boundary rows, where
+ * b = -(1/alpha + dz + zeta) alpha^2 and c or a = alpha).  NOT the reference's
+ * arithmetic: a test reference for the mixed mode, which solves this form. */
+void oo_rh_coefficients(int nz, const double* al, const double* om, const double* S, double dz, double dz2,
+                        double* A, double* C, double* H, double* R)
+{
+    for (int i = 1; i < nz - 1; i++) {
+        double al2 = al[i] * al[i];
+        A[i] = al2 / (0.5 * (al[i] + al[i - 1]));
+        C[i] = al2 / (0.5 * (al[i + 1] + al[i]));
+        H[i] = (dz2 * (al2 * al[i])) * (1.0 - om[i]);
+        R[i] = S[i] * H[i];
+    }
+    for (int k = 0; k < 2; k++) {
+        int i = k ? nz - 1 : 0;
+        double al2 = al[i] * al[i];
+        double ze = ((al[i] * dz2) * (1.0 - om[i])) * 0.25;
+        A[i] = k ? al[i] : 0.0;
+        C[i] = k ? 0.0 : al[i];
+        H[i] = (dz + ze) * al2;
+        R[i] = (S[i] * ze) * al2;
+    }
+}
+
+/* Elimination in difference form: x[i] = D[i] x[i+1] + Z[i] with
+ * D = C / (A (1 - D[i-1]) + C + H) and 1 - D[i] = (A (1 - D[i-1]) + H) / den
+ * carried as its own quantity, so no step subtracts. */
+void oo_tridag_rh(int n, const double* A, const double* C, const double* H, const double* R, double* u,
+                  double* Dw)
+{
+    double om = 1.0, zp = 0.0;   /* 1 - D[i-1], Z[i-1] */
+    for (int i = 0; i < n; i++) {
+        double den = (A[i] * om + C[i]) + H[i];
+        Dw[i] = C[i] / den;
+        u[i] = (R[i] + A[i] * zp) / den;
+        om = (A[i] * om + H[i]) / den;
+        zp = u[i];
+    }
+    for (int i = n - 2; i >= 0; i--) u[i] = Dw[i] * u[i + 1] + u[i];
+}
+
+/* One (column, wavelength) system: source/flux_feautrier.f90:183-216.
+ * rh != 0 solves the rows in difference form instead (see oo_tridag_rh). */
 static int solve_system(int nz, const oo_tables* t, const oo_cell* cells, int iw, int lgrey,
                         double sigma_grey, double dz, double dz1, double dz2, double* work,
-                        double* U, double* V, double* A, int* jfail, double sums[4])
+                        double* U, double* V, double* A, int* jfail, double sums[4], int rh)
 {
     double *al = work, *al1 = al + nz, *al2 = al1 + nz, *om = al2 + nz, *S = om + nz;
     double *a = S + nz, *b = a + nz, *c = b + nz, *d = c + nz, *gam = d + nz;
@@ -364,7 +409,13 @@ static int solve_system(int nz, const oo_tables* t, const oo_cell* cells, int iw
         sums[0] = sums[1] = sums[2] = sums[3] = 0.0;
         for (int i = 0; i < nz; i++) { sums[0] += a[i]; sums[1] += b[i]; sums[2] += c[i]; sums[3] += d[i]; }
     }
-    int st = oo_tridag(nz, a, b, c, d, U + 3, gam, jfail);
+    int st = 0;
+    if (rh) {
+        oo_rh_coefficients(nz, al, om, S, dz, dz2, a, c, b, d);   /* a = A, c = C, b = H, d = R */
+        oo_tridag_rh(nz, a, c, b, d, U + 3, gam);
+    } else {
+        st = oo_tridag(nz, a, b, c, d, U + 3, gam, jfail);
+    }
     if (st) return st;
     oo_ghosts_flux(nz, U, al1, dz1, V);
     memcpy(A, al, sizeof(double) * (size_t)nz);
@@ -373,11 +424,11 @@ static int solve_system(int nz, const oo_tables* t, const oo_cell* cells, int iw
 
 /* The column loop: source/flux_feautrier.f90:160-228.  Columns run in
  * parallel; the first failure is chosen in the reference's (ix, iy, iw) order. */
-int oo_solve_block(int nz, int nyloc, int nxloc, const oo_tables* t,
-                   double dz, double dz1, double dz2,
-                   const double* rho3d, const double* T3d,
-                   double fully_ionized_T, int lgrey, double sigma_grey,
-                   double* U, double* V, double* alpha, oo_diag* diag, int nthreads)
+int oo_solve_block_ex(int nz, int nyloc, int nxloc, const oo_tables* t,
+                      double dz, double dz1, double dz2,
+                      const double* rho3d, const double* T3d,
+                      double fully_ionized_T, int lgrey, double sigma_grey,
+                      double* U, double* V, double* alpha, oo_diag* diag, int nthreads, int rh)
 {
     const int nw = t->nw, mz = nz + 6;
     const long ncol = (long)nxloc * nyloc;
@@ -406,7 +457,7 @@ int oo_solve_block(int nz, int nyloc, int nxloc, const oo_tables* t,
                 int jf = 0;
                 double sums[4];
                 int st = solve_system(nz, t, cells, iw, lgrey, sigma_grey, dz, dz1, dz2, work,
-                                      Us, Vs, As, &jf, (col == 0 && iw == 0) ? sums : NULL);
+                                      Us, Vs, As, &jf, (col == 0 && iw == 0) ? sums : NULL, rh);
                 if (col == 0 && iw == 0) {
                     dg.sum_a = sums[0]; dg.sum_b = sums[1]; dg.sum_c = sums[2]; dg.sum_d = sums[3];
                 }
@@ -442,6 +493,16 @@ int oo_solve_block(int nz, int nyloc, int nxloc, const oo_tables* t,
     }
     if (diag) *diag = dg;
     return dg.status;
+}
+
+int oo_solve_block(int nz, int nyloc, int nxloc, const oo_tables* t,
+                   double dz, double dz1, double dz2,
+                   const double* rho3d, const double* T3d,
+                   double fully_ionized_T, int lgrey, double sigma_grey,
+                   double* U, double* V, double* alpha, oo_diag* diag, int nthreads)
+{
+    return oo_solve_block_ex(nz, nyloc, nxloc, t, dz, dz1, dz2, rho3d, T3d, fully_ionized_T, lgrey, sigma_grey,
+                             U, V, alpha, diag, nthreads, 0);
 }
 
 /* calc_grid: source/disk.f90:42-62 */

@@ -84,6 +84,17 @@ int oo_solve_block(int nz, int nyloc, int nxloc, const oo_tables* t,
                    const double* rho3d, const double* T3d,
                    double fully_ionized_T, int lgrey, double sigma_grey,
                    double* U, double* V, double* alpha, oo_diag* diag, int nthreads);
+/* As oo_solve_block with rh != 0: rows solved in difference form
+ * (oo_rh_coefficients, oo_tridag_rh) -- the accurate solution of the same
+ * discrete problem, the reference for the mixed mode's elimination. */
+int oo_solve_block_ex(int nz, int nyloc, int nxloc, const oo_tables* t, double dz, double dz1, double dz2,
+                      const double* rho3d, const double* T3d, double fully_ionized_T, int lgrey,
+                      double sigma_grey, double* U, double* V, double* alpha, oo_diag* diag, int nthreads,
+                      int rh);
+void oo_rh_coefficients(int nz, const double* al, const double* om, const double* S, double dz, double dz2,
+                        double* A, double* C, double* H, double* R);
+void oo_tridag_rh(int n, const double* A, const double* C, const double* H, const double* R, double* u,
+                  double* Dw);
 
 /* Analytic disk input: source/disk.f90:42-129. */
 void oo_calc_grid(int nz, double z1, double z0, double* z /* mz */, double* dz);

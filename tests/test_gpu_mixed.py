@@ -1,19 +1,26 @@
-"""Mixed-precision mode (OCO_MODE_MIXED) against the FP64 path.
+"""Mixed-precision mode (OCO_MODE_MIXED).
 
 The mixed path evaluates the opacity's exponentials and logarithms with an
 FP32 core (relative error ~1.2e-7 each) and divides by reciprocal + Newton;
-the Saha state, the row assembly and the Thomas elimination stay FP64.
+the Saha state stays FP64, and the rows are eliminated in FP64 in the
+Rybicki-Hummer difference form (-A x[i-1] + (A+C+H) x[i] - C x[i+1] = R, all
+terms >= 0), which keeps the row excess H = zeta that the reference's
+b = -((a + c) + zeta) rounds away in optically thin cells.
 
-Stated bounds, per result:
-  * alpha (absorption coefficient): relative error <= 1e-6 at every point;
-  * emergent flux V(1), V(nz): relative error <= 1e-6 wherever the FP64 answer
-    itself is resolved to that level; elsewhere within a small multiple of the
-    FP64 answer's own resolution q = eps*max|U(stencil)|/|alpha*dz| (one
-    rounding of U seen through the boundary derivative).  In optically thin
-    cells (alpha*dz ~ 1e-13 on the 2000 K, 1e-15 g/cm^3 floor) q/|V| reaches
-    1e-3: there the row sum a+c+zeta loses zeta entirely and dU is a few
-    hundred ulps of U, so no FP64 evaluation order resolves V better;
-  * the batched path in mixed mode returns exactly the block path's bits.
+Reference for the comparison: the oracle with FP64 opacity and the same
+difference-form elimination (oracle solve_block(rh=True), itself pinned
+against exact rational solves in test_oracle_rh.py).  Stated bounds:
+  * alpha: relative error <= 1e-6 at every point (against the FP64 path);
+  * U: relative error <= 1e-6 at every point;
+  * emergent flux V(1), V(nz): relative error <= 1e-6 where FP64 resolves V
+    to 1e-10 (q/|V| <= 1e-10, q = eps*max|U(stencil)|/|alpha*dz|, one
+    rounding of U seen through the boundary derivative); elsewhere within
+    1e-6 |V| + 64 q.  On the 2000 K, 1e-15 g/cm^3 floor q/|V| reaches 1e-3:
+    no FP64 evaluation of the derivative resolves V better;
+  * the batched path in mixed mode returns the block path's bits.
+The FP64 drop-in (the reference's elimination) is itself off by up to ~2e-3
+in U on that floor; its difference from the mixed result is printed, not
+bounded.
 """
 import numpy as np
 import pytest
@@ -23,6 +30,7 @@ from conftest import golden, relerr
 pytestmark = pytest.mark.gpu
 
 ALPHA_TOL = 1e-6
+U_TOL = 1e-6
 FLUX_TOL = 1e-6
 EPS = 2.0 ** -53
 
@@ -88,6 +96,34 @@ def test_mixed_alpha_goldens(ctx):
         assert err <= ALPHA_TOL
 
 
+def rh_oracle(rho, T, dz, waves):
+    from oracle.oracle import Oracle
+    U, V, A, d = Oracle(waves, portable=True).solve_block(rho, T, dz, rh=True)
+    assert d.status == 0
+    return U, V, A
+
+
+def check_against_rh(U, V, A, ref, dz, label):
+    """The stated bounds against the difference-form oracle."""
+    Ur, Vr, Ar = ref
+    ea = relerr(A, Ar).max()
+    eu = relerr(U, Ur).max()
+    print("  %s: alpha %.2e  U %.2e" % (label, ea, eu), flush=True)
+    assert ea <= ALPHA_TOL
+    assert eu <= U_TOL
+    qs = resolution(Ur, Ar, dz)
+    for i, k in enumerate((0, -1)):
+        vr = Vr[..., k]
+        rel = relerr(V[..., k], vr)
+        r = qs[i] / np.abs(vr) <= 1e-10
+        excess = ((np.abs(V[..., k] - vr) - FLUX_TOL * np.abs(vr)) / qs[i]).max()
+        print("    V(%s): resolved %d/%d, max rel err resolved %.3e, all %.3e, max excess/q %.2f"
+              % (k, r.sum(), r.size, rel[r].max() if r.any() else 0.0, rel.max(), excess), flush=True)
+        if r.any():
+            assert rel[r].max() <= FLUX_TOL
+        assert excess <= RES_FACTOR
+
+
 @pytest.mark.parametrize("where", [(0, 0), (30, 30)])
 def test_mixed_c4_slice(ctx, where):
     rho, T, dz, waves = c4_slice(where[0], where[1], 4)
@@ -95,20 +131,14 @@ def test_mixed_c4_slice(ctx, where):
     out = both_modes(ctx, rho, T, dz, waves)
     U64, V64, A64 = out["fp64"]
     Um, Vm, Am = out["mixed"]
-    assert relerr(Am, A64).max() <= ALPHA_TOL
+    ref = rh_oracle(rho, T, dz, waves)
+    check_against_rh(Um, Vm, Am, ref, dz, "mixed")
+    print("  FP64 drop-in vs the same oracle: U %.2e" % relerr(U64, ref[0]).max(), flush=True)
+    # FP64-resolved emergent points: the drop-in agrees with the mixed path too
     res = resolved(U64, V64, A64, dz)
-    qs = resolution(U64, A64, dz)
     for i, k in enumerate((0, -1)):
-        v64, vm = V64[..., k], Vm[..., k]
-        rel = relerr(vm, v64)
-        r = res[i]
-        excess = ((np.abs(vm - v64) - FLUX_TOL * np.abs(v64)) / qs[i]).max()
-        print("  V(%s): resolved %d/%d, max rel err resolved %.3e, all %.3e, max excess/q %.2f"
-              % (k, r.sum(), r.size, rel[r].max() if r.any() else 0.0, rel.max(), excess), flush=True)
-        assert r.sum() > 0.2 * r.size
-        assert rel[r].max() <= FLUX_TOL
-        # unresolved points: inside a small multiple of FP64's own resolution
-        assert excess <= RES_FACTOR
+        assert res[i].sum() > 0.2 * res[i].size
+        assert relerr(Vm[..., k], V64[..., k])[res[i]].max() <= FLUX_TOL
 
 
 def test_mixed_c2_slice(ctx):
@@ -116,21 +146,50 @@ def test_mixed_c2_slice(ctx):
     rho, T, dz, waves = config2(4, 4, 256, 128)
     print("case c2 slice", rho.shape, flush=True)
     out = both_modes(ctx, rho, T, dz, waves)
-    U64, V64, A64 = out["fp64"]
-    Um, Vm, Am = out["mixed"]
-    assert relerr(Am, A64).max() <= ALPHA_TOL
-    res = resolved(U64, V64, A64, dz)
-    for i, k in enumerate((0, -1)):
-        rel = relerr(Vm[..., k], V64[..., k])
-        r = res[i]
-        print("  V(%s): resolved %d/%d, max rel err resolved %.3e" % (k, r.sum(), r.size, rel[r].max()),
-              flush=True)
-        assert rel[r].max() <= FLUX_TOL
+    check_against_rh(*out["mixed"], rh_oracle(rho, T, dz, waves), dz, "mixed")
+
+
+def sweep_context(kind):
+    """A context whose mixed mode uses the given sweep ("wave" or "lane")."""
+    import os
+    import ocotillo_amd as oco
+    old = os.environ.get("OCO_MIXED_SWEEP")
+    os.environ["OCO_MIXED_SWEEP"] = kind
+    try:
+        return oco.Context(0)
+    finally:
+        if old is None:
+            del os.environ["OCO_MIXED_SWEEP"]
+        else:
+            os.environ["OCO_MIXED_SWEEP"] = old
+
+
+@pytest.mark.parametrize("nz", [4, 5, 63, 64, 65, 200, 256, 513, 1100])
+def test_mixed_wave_and_lane_sweeps(nz):
+    """Both mixed sweeps (wavefront per system with the partitioned solve --
+    up to 17 rows per lane, nz <= 1088 -- and lane per system with the
+    sequential one) meet the bounds against the oracle."""
+    import ocotillo_amd as oco
+    from ocotillo_amd.workloads import config2
+    rho, T, dz, waves = config2(2, 2, nz, 40)
+    print("case nz", nz, flush=True)
+    ref = rh_oracle(rho, T, dz, waves)
+    out = {}
+    for kind in ("wave", "lane"):
+        with sweep_context(kind) as c:
+            c.set_wavelengths(waves)
+            c.set_physics(2e4, False, 0.4)
+            U, V, A, d = c.solve_block(rho, T, dz, mode=oco.MODE_MIXED)
+        assert d.status == 0
+        check_against_rh(U, V, A, ref, dz, kind + " sweep")
+        out[kind] = A
+    assert np.array_equal(out["wave"], out["lane"])   # same opacity code
 
 
 def test_batch_mixed_equals_block(ctx):
-    """Batched (persistent, graph-replayed) mixed solve == block mixed solve,
-    bit for bit, on every replay."""
+    """Batched (persistent, graph-replayed) mixed solve == the block mixed
+    solve with the same (lane-per-system) kernel, bit for bit, on every
+    replay."""
     import ocotillo_amd as oco
     from ocotillo_amd.workloads import config2
     rho, T, dz, waves = config2(2, 3, 96, 80)
@@ -138,7 +197,10 @@ def test_batch_mixed_equals_block(ctx):
     nz, nw = rho.shape[2], waves.size
     ctx.set_wavelengths(waves)
     ctx.set_physics(2e4, False, 0.4)
-    U0, V0, A0, _ = ctx.solve_block(rho, T, dz, mode=oco.MODE_MIXED)
+    with sweep_context("lane") as lc:
+        lc.set_wavelengths(waves)
+        lc.set_physics(2e4, False, 0.4)
+        U0, V0, A0, _ = lc.solve_block(rho, T, dz, mode=oco.MODE_MIXED)
     b = oco.Batch(ctx, [nz] * ncol, [nw] * ncol, np.tile(waves, ncol), [dz] * ncol)
     try:
         for rep in range(3):
