@@ -100,14 +100,18 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--config", default="c4", choices=["c4", "c2", "c3"],
+    ap.add_argument("--config", default="c4", choices=["c4", "c2", "c3", "c5"],
                     help="c4: the 256x256x512 x 128-wavelength mixed-precision snapshot (default); "
                          "c2: the 64x64x256 x 128-wavelength FP64 drop-in block; "
-                         "c3: 4096 uneven columns through the batched persistent path")
+                         "c3: 4096 uneven columns through the batched persistent path; "
+                         "c5: the serving loop under Poisson arrivals")
     ap.add_argument("--mode", default=None, choices=["both", "fp64", "mixed"],
                     help="precision modes to time (default: both for c4 and c3, fp64 for c2)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-accuracy", action="store_true", help="skip the mixed-vs-FP64 accuracy pass")
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving leg of the c4 run")
+    ap.add_argument("--jobs", type=int, default=None, help="serving: jobs after warm-up (c5: 300, c4 leg: 40)")
+    ap.add_argument("--pool", type=int, default=None, help="serving: distinct job shapes (c5: 12, c4 leg: 6)")
     ap.add_argument("--nx", type=int, default=None)
     ap.add_argument("--ny", type=int, default=None)
     ap.add_argument("--nz", type=int, default=None)
@@ -134,6 +138,15 @@ def main():
         return
     if args.config == "c3":
         return bench_batch(args)
+    if args.config == "c5":
+        res = bench_serve(args.jobs or 300, args.pool or 12)
+        out = {"metric": METRIC, "value": res["p99_ms"], "unit": "ms (p99 latency)", "n_gpus": 1,
+               "steps": res["jobs"], "warmup": args.pool or 12, "ms_per_step": res["mean_ms"],
+               "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "f64+f32",
+               "data": "synthetic", "config": {"workload": res["workload"]}, "serving": res,
+               "roofline": None, "cpu_baseline": None}
+        print(json.dumps(out), flush=True)
+        return
     return bench_block(args)
 
 
@@ -227,6 +240,10 @@ def bench_block(args):
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(rho, T, dz, waves, args.config)
+    serving = None
+    if args.config == "c4" and not args.no_serve:
+        del d_rho, d_T
+        serving = bench_serve(args.jobs or 40, args.pool or 6)
 
     head = modes[-1]
     out = {
@@ -246,6 +263,8 @@ def bench_block(args):
                        "roofline": res["fp64"]["roofline"]}
         out["speedup_mixed_vs_fp64"] = round(res["mixed"]["value"] / res["fp64"]["value"], 3)
         out["accuracy"] = accuracy
+    if serving is not None:
+        out["serving"] = serving
     print(json.dumps(out), flush=True)
     ctx.close()
 
@@ -345,6 +364,92 @@ def cpu_baseline(rho, T, dz, waves, config):
     tc = time.perf_counter() - tc0
     return {"value": round(pts / tc, 1), "unit": "points/s", "cores": nthreads, "kind": "port",
             "sample": "%s: %.4g points in %.2f s wall" % (sample, pts, tc)}
+
+
+C5_RATE = 50.0          # jobs/s of the serving workload
+C5_MEAN_POINTS = 5.16e9  # mean points per job of its shape distribution
+
+
+def bench_serve(njobs, pool_size, seed_jobs=True, load=0.7):
+    """Serving (the C5 shapes: nx, ny ~ U{16..256}, nz ~ U{64..2048},
+    nw ~ U{16..512}, mixed precision, emergent-flux outputs): Poisson
+    arrivals into the server.  The stated 50 jobs/s offers ~2.6e11 points/s,
+    above one GPU's capacity, so arrivals run at min(50/s, load x measured
+    capacity / mean job points) and both rates are reported.  Jobs are drawn
+    from a seeded pool of `pool_size` shapes whose inputs sit in pinned host
+    memory (generating ~300 MB of inputs per arrival would bound the host,
+    not the server)."""
+    import queue
+    import threading
+    import torch
+    import ocotillo_amd as oco
+    from ocotillo_amd import workloads as W
+    torch.cuda.empty_cache()
+    ctx = oco.Context(0)
+    ctx.set_physics(2e4, False, 0.4)
+    shapes = W.config5_shapes(pool_size)
+    pool = []
+    for k, (nx, ny, nz, nw) in enumerate(shapes):
+        rho, T, dz, waves = W.config5_job(nx, ny, nz, nw, seed=W.SEED + 500 + k)
+        pr = torch.empty(rho.shape, dtype=torch.float64, pin_memory=True).numpy()
+        pt = torch.empty(T.shape, dtype=torch.float64, pin_memory=True).numpy()
+        pr[...] = rho
+        pt[...] = T
+        pool.append((pr, pt, dz, waves, float(nx * ny * nz * nw)))
+        log("serve pool job %d: %dx%dx%d, nw=%d (%.3g points)" % (k, nx, ny, nz, nw, pool[-1][4]))
+    free, _ = torch.cuda.mem_get_info()
+    budget = int(min(200 * 2 ** 30, free - 24 * 2 ** 30))
+    with oco.Server(ctx, budget, max_nz=2048, mode=oco.MODE_MIXED) as srv:
+        # capacity (and warm-up): every pool job once, back to back
+        t0 = time.perf_counter()
+        tickets = [srv.submit(*j[:4]) for j in pool]
+        for t in tickets:
+            srv.wait(t)
+        cap = sum(j[4] for j in pool) / (time.perf_counter() - t0)
+        mean_pts = float(np.mean([j[4] for j in pool]))
+        rate = min(C5_RATE, load * cap / mean_pts)
+        log("serve capacity %.3g points/s -> arrival rate %.2f jobs/s" % (cap, rate))
+        arrivals = W.poisson_arrivals(njobs, rate)
+        pick = np.random.default_rng(W.SEED + 7).integers(0, pool_size, njobs)
+        q = queue.Queue()
+        results = []
+
+        def collect():
+            while True:
+                t = q.get()
+                if t is None:
+                    return
+                _, r = srv.wait(t)
+                results.append((r.latency_ms, r.queue_ms, r.points, time.perf_counter()))
+
+        th = threading.Thread(target=collect)
+        th.start()
+        t_start = time.perf_counter()
+        for k in range(njobs):
+            delay = t_start + arrivals[k] - time.perf_counter()
+            if delay > 0:
+                time.sleep(delay)
+            q.put(srv.submit(*pool[pick[k]][:4]))
+            if (k + 1) % 20 == 0:
+                log("serve: %d/%d submitted" % (k + 1, njobs))
+        q.put(None)
+        th.join()
+        st = srv.stats()
+    ctx.close()
+    lat = np.array([r[0] for r in results])
+    pts = sum(r[2] for r in results)
+    wall = max(r[3] for r in results) - t_start
+    return {"workload": "serving: Poisson arrivals, job shapes nx,ny~U{16..256}, nz~U{64..2048}, "
+                        "nw~U{16..512} (pool of %d seeded shapes), mixed precision, emergent-flux outputs"
+                        % pool_size,
+            "jobs": njobs, "arrival_rate_jobs_per_s": round(rate, 3), "stated_rate_jobs_per_s": C5_RATE,
+            "stated_offered_points_per_s": C5_RATE * C5_MEAN_POINTS,
+            "capacity_points_per_s": round(cap, 1),
+            "p50_ms": round(float(np.percentile(lat, 50)), 2), "p99_ms": round(float(np.percentile(lat, 99)), 2),
+            "mean_ms": round(float(lat.mean()), 2), "max_ms": round(float(lat.max()), 2),
+            "mean_queue_ms": round(float(np.mean([r[1] for r in results])), 2),
+            "points_per_s": round(pts / wall, 1), "launches": int(st.iterations),
+            "hbm_budget_gib": round(budget / 2 ** 30, 1), "hbm_peak_gib": round(st.hbm_peak / 2 ** 30, 2)}
 
 
 def bench_batch(args):

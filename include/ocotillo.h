@@ -120,6 +120,62 @@ int oco_batch_run_host(oco_batch* plan, const double* rho, const double* T, doub
  * whether tuning ran (each mode is tuned and captured on its first run). */
 int oco_batch_tuning(oco_batch* plan, int mode, int* variant, int* nblocks, int* tuned);
 
+/* ---- Serving ----
+ * A request loop over one device: jobs (a snapshot block or column set,
+ * each with its own shape and wavelength grid) are submitted as they arrive
+ * and return the emergent flux V(1), V(nz) of every (column, wavelength)
+ * system.  The server admits queued jobs in arrival order while its HBM
+ * budget and job slots allow, keeps wavelength tables resident, and feeds
+ * one persistent launch per iteration with tiles of every admitted job
+ * (continuous batching: a job that arrives while others run joins at the
+ * next iteration, and its systems share launches with theirs).  Each system
+ * is solved exactly as it would be alone.  No reference counterpart (the
+ * reference runs one block per process, source/flux_feautrier.f90:160-228);
+ * the physics per system is the column loop body, in `mode` precision. */
+typedef struct oco_server oco_server;
+
+typedef struct {
+    int nx, ny, nz, nw;     /* nxloc, nyloc, cells per column, wavelengths */
+    double dz;              /* grid step (1/dz and dz**2 derived as the reference does) */
+    const double* rho3d;    /* (nz, ny, nx) Fortran order; read until the job completes */
+    const double* T3d;      /* (nz, ny, nx) */
+    const double* waves;    /* nw wavelengths in Angstrom */
+    double* emergent;       /* out: (2, ny, nx, nw): V(1), V(nz) of every system */
+} oco_job;
+
+typedef struct {
+    int status;             /* OCO_OK or a pivot status (FP64 mode) */
+    int fail_ix, fail_iy, fail_iw, fail_j;
+    double queue_ms;        /* submit -> admitted */
+    double latency_ms;      /* submit -> outputs in host memory */
+    double points;          /* nz*nx*ny*nw */
+} oco_job_result;
+
+typedef struct {
+    long long submitted, completed, queued, active;
+    double p50_ms, p99_ms, mean_ms, max_ms;   /* latency of completed jobs */
+    double points;                            /* points of completed jobs */
+    double wall_s;                            /* first submit -> last completion */
+    double points_per_s;
+    long long iterations;                     /* persistent launches */
+    size_t hbm_budget, hbm_peak;              /* job arena: budget and high-water mark */
+} oco_serve_stats;
+
+/* hbm_budget: bytes of device memory for admitted jobs and resident tables;
+ * max_nz: longest column accepted; mode: OCO_MODE_FP64 or OCO_MODE_MIXED. */
+int oco_server_create(oco_ctx* ctx, size_t hbm_budget, int max_nz, int mode, oco_server** srv);
+/* Queue a job; returns at once with a ticket.  OCO_E_OOM if the job alone
+ * exceeds the budget. */
+int oco_server_submit(oco_server* srv, const oco_job* job, long long* ticket);
+/* Block until the job has completed (outputs written); then forget it. */
+int oco_server_wait(oco_server* srv, long long ticket, oco_job_result* res);
+/* OCO_OK and the result if the job has completed (then forgotten),
+ * OCO_E_BUSY if not yet. */
+int oco_server_poll(oco_server* srv, long long ticket, oco_job_result* res);
+int oco_server_stats(oco_server* srv, oco_serve_stats* st);
+/* Finish every submitted job, then release the server. */
+int oco_server_destroy(oco_server* srv);
+
 /* Message text for a status code. */
 const char* oco_strerror(int status);
 

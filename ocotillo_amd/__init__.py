@@ -17,8 +17,8 @@ import os
 
 import numpy as np
 
-__all__ = ["OcoError", "Context", "Batch", "Diag", "load_library", "LIB_PATH", "MODE_FP64", "MODE_MIXED",
-           "EXPORTS"]
+__all__ = ["OcoError", "Context", "Batch", "Server", "Diag", "load_library", "LIB_PATH", "MODE_FP64",
+           "MODE_MIXED", "EXPORTS"]
 
 PKG = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(PKG, "libocotillo.so")
@@ -30,7 +30,9 @@ MODE_FP64, MODE_MIXED = 0, 1
 EXPORTS = ["oco_create", "oco_destroy", "oco_set_wavelengths", "oco_set_physics", "oco_solve_block",
            "oco_solve_block_device", "oco_strerror", "oco_version",
            "oco_batch_create", "oco_batch_destroy", "oco_batch_layout", "oco_batch_buffers", "oco_batch_run",
-           "oco_batch_run_host", "oco_batch_tuning"]
+           "oco_batch_run_host", "oco_batch_tuning",
+           "oco_server_create", "oco_server_submit", "oco_server_wait", "oco_server_poll", "oco_server_stats",
+           "oco_server_destroy"]
 
 
 class Diag(C.Structure):
@@ -38,6 +40,23 @@ class Diag(C.Structure):
                 + [(n, C.c_double) for n in ("sum_a", "sum_b", "sum_c", "sum_d", "minU", "maxU",
                                              "minV", "maxV", "minA", "maxA", "cells_ms", "sweep_ms",
                                              "total_ms")])
+
+
+class Job(C.Structure):
+    _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int), ("nw", C.c_int), ("dz", C.c_double),
+                ("rho3d", C.c_void_p), ("T3d", C.c_void_p), ("waves", C.c_void_p), ("emergent", C.c_void_p)]
+
+
+class JobResult(C.Structure):
+    _fields_ = ([(n, C.c_int) for n in ("status", "fail_ix", "fail_iy", "fail_iw", "fail_j")]
+                + [(n, C.c_double) for n in ("queue_ms", "latency_ms", "points")])
+
+
+class ServeStats(C.Structure):
+    _fields_ = ([(n, C.c_longlong) for n in ("submitted", "completed", "queued", "active")]
+                + [(n, C.c_double) for n in ("p50_ms", "p99_ms", "mean_ms", "max_ms", "points", "wall_s",
+                                             "points_per_s")]
+                + [("iterations", C.c_longlong), ("hbm_budget", C.c_size_t), ("hbm_peak", C.c_size_t)])
 
 
 class OcoError(RuntimeError):
@@ -81,6 +100,12 @@ def load_library(path=None):
     L.oco_batch_run_host.argtypes = [C.c_void_p, _DP, _DP, _DP, _DP, _DP, C.c_int, C.POINTER(Diag)]
     L.oco_batch_tuning.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int),
                                    C.POINTER(C.c_int)]
+    L.oco_server_create.argtypes = [C.c_void_p, C.c_size_t, C.c_int, C.c_int, C.POINTER(C.c_void_p)]
+    L.oco_server_submit.argtypes = [C.c_void_p, C.POINTER(Job), C.POINTER(C.c_longlong)]
+    L.oco_server_wait.argtypes = [C.c_void_p, C.c_longlong, C.POINTER(JobResult)]
+    L.oco_server_poll.argtypes = [C.c_void_p, C.c_longlong, C.POINTER(JobResult)]
+    L.oco_server_stats.argtypes = [C.c_void_p, C.POINTER(ServeStats)]
+    L.oco_server_destroy.argtypes = [C.c_void_p]
     L.oco_strerror.restype = C.c_char_p
     L.oco_strerror.argtypes = [C.c_int]
     L.oco_version.restype = C.c_char_p
@@ -258,3 +283,79 @@ class Batch:
             u += (nz + 6) * nw
             v += nz * nw
         return outU, outV, outA
+
+
+class Server:
+    """The serving loop (oco_server_*): submit jobs as they arrive, each a
+    block of columns with its own shape and wavelength grid; a job returns the
+    emergent flux V(1), V(nz) of every system as an array [iw, ix, iy, 2]
+    (Fortran (2, ny, nx, nw)).  Inputs and output arrays are held here until
+    the job is collected."""
+
+    def __init__(self, ctx, hbm_budget, max_nz=2048, mode=MODE_MIXED):
+        self.ctx = ctx
+        self.L = ctx.L
+        h = C.c_void_p()
+        st = self.L.oco_server_create(ctx.h, int(hbm_budget), int(max_nz), int(mode), C.byref(h))
+        if st != OK:
+            raise OcoError(st, "oco_server_create: " + strerror(st))
+        self.h = h
+        self.jobs = {}
+
+    def submit(self, rho3d, T3d, dz, waves):
+        """rho3d, T3d: [ix, iy, z]; returns a ticket."""
+        rho3d = np.ascontiguousarray(rho3d, dtype=np.float64)
+        T3d = np.ascontiguousarray(T3d, dtype=np.float64)
+        waves = np.ascontiguousarray(waves, dtype=np.float64)
+        if rho3d.ndim != 3 or rho3d.shape != T3d.shape:
+            raise OcoError(E_ARG, "rho3d and T3d must both be [nxloc, nyloc, nz]")
+        nx, ny, nz = rho3d.shape
+        out = np.empty((waves.size, nx, ny, 2))
+        job = Job(nx, ny, nz, waves.size, float(dz), rho3d.ctypes.data, T3d.ctypes.data, waves.ctypes.data,
+                  out.ctypes.data)
+        t = C.c_longlong()
+        st = self.L.oco_server_submit(self.h, C.byref(job), C.byref(t))
+        if st != OK:
+            raise OcoError(st, "oco_server_submit: " + strerror(st))
+        self.jobs[t.value] = (rho3d, T3d, waves, out)
+        return t.value
+
+    def wait(self, ticket, raise_on_pivot=True):
+        """Block until the job completes; returns (emergent [iw, ix, iy, 2], JobResult)."""
+        r = JobResult()
+        st = self.L.oco_server_wait(self.h, ticket, C.byref(r))
+        out = self.jobs.pop(ticket)[3]
+        if st != OK and (raise_on_pivot or st not in (E_PIVOT0_STAGE1, E_PIVOT0_STAGE2)):
+            raise OcoError(st, "oco_server_wait: " + strerror(st))
+        return out, r
+
+    def poll(self, ticket):
+        """(emergent, JobResult) if the job has completed, else None."""
+        r = JobResult()
+        st = self.L.oco_server_poll(self.h, ticket, C.byref(r))
+        if st == E_BUSY:
+            return None
+        out = self.jobs.pop(ticket)[3]
+        if st not in (OK, E_PIVOT0_STAGE1, E_PIVOT0_STAGE2):
+            raise OcoError(st, "oco_server_poll: " + strerror(st))
+        return out, r
+
+    def stats(self):
+        s = ServeStats()
+        self.L.oco_server_stats(self.h, C.byref(s))
+        return s
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.oco_server_destroy(self.h)
+            self.h = None
+            self.jobs.clear()
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()

@@ -35,17 +35,17 @@ def build_lib(out=LIB, defines=(), build_dir=BUILD):
     (e.g. OCO_SWEEP_WAVES_PER_EU=4) for the kernels."""
     os.makedirs(build_dir, exist_ok=True)
     objs = []
-    for src, device in (("oco_kernels.hip", True), ("oco_api.cpp", False)):
+    for src, device in (("oco_kernels.hip", True), ("oco_api.cpp", False), ("oco_serve.cpp", False)):
         obj = os.path.join(build_dir, os.path.splitext(src)[0] + ".o")
         if device:
             cmd = [HIPCC] + COMMON + DEVICE + ["-D" + d for d in defines] + ["-c", os.path.join(CSRC, src), "-o", obj]
         else:   # host-only C++ (the C ABI): no device pass
-            cmd = [CXX] + COMMON + ["-D__HIP_PLATFORM_AMD__", "-I" + os.path.join(ROCM, "include"),
+            cmd = [CXX] + COMMON + ["-pthread", "-D__HIP_PLATFORM_AMD__", "-I" + os.path.join(ROCM, "include"),
                                     "-c", os.path.join(CSRC, src), "-o", obj]
         _run(cmd)
         objs.append(obj)
     tmp = out + ".tmp"
-    _run([HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", tmp] + objs)
+    _run([HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-pthread", "-o", tmp] + objs)
     os.replace(tmp, out)
     return out
 

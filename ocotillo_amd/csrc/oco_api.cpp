@@ -106,7 +106,7 @@ static double gaunt(int n, double wave_cm)
 /* pre_calc_opacity_quantities (source/continuous_opacity.f90:64-141) plus the
  * wavelength-only factors of the per-point formulas.  Runs once per grid on
  * the host, with the host libm exactly as the reference does. */
-static void build_wtab(const std::vector<double>& waves, std::vector<double>& wt)
+void oco_build_wtab(const std::vector<double>& waves, std::vector<double>& wt)
 {
     static const double a_coeff[7] = {+1.99654, -1.18267e-6, +2.62423e-7, -4.40524e-11,
                                       +3.23992e-15, -1.39568e-19, +2.78701e-24};
@@ -148,6 +148,9 @@ static void build_wtab(const std::vector<double>& waves, std::vector<double>& wt
         w[WT_SFW5] = sfc * ((((wcm1 * wcm1) * wcm1) * wcm1) * wcm1);
     }
 }
+
+int oco_ctx_device(const oco_ctx* c) { return c->device; }
+double oco_ctx_fully_ionized_T(const oco_ctx* c) { return c->fully_ionized_T; }
 
 extern "C" {
 
@@ -226,7 +229,7 @@ int oco_set_wavelengths(oco_ctx* c, int nw, const double* waves)
     HIPCHK(hipSetDevice(c->device));
     c->waves.assign(waves, waves + nw);
     std::vector<double> wt;
-    build_wtab(c->waves, wt);
+    oco_build_wtab(c->waves, wt);
     int st = ensure(c, c->wtab, wt.size() * sizeof(double));
     if (st) return st;
     HIPCHK(hipMemcpyAsync(c->wtab.p, wt.data(), wt.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
@@ -517,7 +520,7 @@ int oco_batch_create(oco_ctx* c, int ncol, const int* nz, const int* nw, const d
     std::vector<double> wt((size_t)ns * OCO_WT_STRIDE), one;
     for (int i = 0; i < ncol; i++) {
         std::vector<double> g(waves + woff[i], waves + woff[i] + nw[i]);
-        build_wtab(g, one);
+        oco_build_wtab(g, one);
         std::copy(one.begin(), one.end(), wt.begin() + woff[i] * OCO_WT_STRIDE);
     }
     int dev_cus = 256;

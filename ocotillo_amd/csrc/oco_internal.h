@@ -49,7 +49,30 @@ struct oco_batch_args {
     unsigned long long* fail;
 };
 
+/* Serving: one admitted job's device descriptor. */
+struct oco_serve_job {
+    const double* cells;   /* cell records, cell-major (ncol*nz, OCO_CELL_STRIDE) */
+    const double* wtab;    /* (nw, OCO_WT_STRIDE) */
+    double* em;            /* emergent V: (2, ncol, nw) -- V(1), V(nz) per system */
+    long ncol;
+    int nz, nw;
+    double dz, dz1, dz2;
+};
+
+struct oco_serve_args {
+    int ntiles;
+    const int* tiles;             /* ntiles x (job slot, column, first wavelength) */
+    const oco_serve_job* jobs;    /* per slot */
+    double* scratch;
+    long scr_stride;              /* doubles per wavefront slot: max_nz * 3 * 64 */
+    unsigned int* counter;
+    unsigned long long* fail;     /* per slot: first zero pivot key (FP64 mode) */
+};
+
 /* Launchers (enqueue on `stream`, no synchronisation). */
+hipError_t oco_launch_serve(const oco_serve_args& a, int mixed, int nblocks, hipStream_t stream);
+int oco_serve_grid(int mixed, int device);
+hipError_t oco_launch_fail_reset(unsigned long long* fail, int n, hipStream_t stream);
 /* mixed != 0 selects the mixed-precision kernels (non-grey only). */
 hipError_t oco_launch_batch(const oco_batch_args& a, int mixed, int variant, int nblocks, hipStream_t stream);
 int oco_batch_variants(void);
@@ -65,6 +88,15 @@ int oco_wave_fits(int nz, int device);
 int oco_wave_preferred(int nz);
 int oco_wave_grid(int nz, int device);
 int oco_sweep_grid(int lgrey, int mixed, int uniform, int device);
+
+/* host helpers shared by the C-ABI files */
+#ifdef __cplusplus
+#include <vector>
+struct oco_ctx;
+void oco_build_wtab(const std::vector<double>& waves, std::vector<double>& wt);
+int oco_ctx_device(const oco_ctx* c);
+double oco_ctx_fully_ionized_T(const oco_ctx* c);
+#endif
 
 /* failure key layout: system index << 32 | stage << 28 | row */
 #define OCO_FAIL_NONE 0xffffffffffffffffULL

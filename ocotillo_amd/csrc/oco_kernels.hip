@@ -135,12 +135,22 @@ __device__ static inline void record_fail(unsigned long long* fail, long sys, in
 template <int R>
 struct oco_stage_t {
     static constexpr int rows = R;
+    static constexpr bool emergent = false;
     double ub[OCO_WAVE * (R + 1)];   /* U rows; alpha rows during the forward pass */
     double vb[OCO_WAVE * (R + 1)];   /* V rows */
     double* base[3][OCO_WAVE];       /* per lane: U, V, alpha; null = lane stores nothing */
 };
 typedef oco_stage_t<OCO_STAGE_ROWS> oco_stage;
 typedef oco_stage_t<OCO_STAGE_ROWS_MIXED> oco_stage_mixed;
+
+/* Output policy of the serving kernels: only the emergent flux V(1), V(nz)
+ * of each system is stored (d.V[0], d.V[1]); U and alpha are not written. */
+struct oco_stage_emergent {
+    static constexpr int rows = 8;
+    static constexpr bool emergent = true;
+    double ub[1], vb[1];
+    double* base[3][1];
+};
 
 /* LDS traffic of one wavefront is performed in issue order, so ordering the
  * wavefront's own accesses needs only a compiler barrier at wavefront scope */
@@ -259,11 +269,14 @@ __device__ static inline void solve_system(const oco_sys& d, bool valid, const d
     const int nz = d.nz;
     const double dz = d.dz, dz1 = d.dz1, dz2 = d.dz2;
     constexpr int R = STAGE::rows;
+    constexpr bool EM = STAGE::emergent;
     const int lane = threadIdx.x;
-    wave_sync();   /* the previous system's flushes have read the buffers */
-    S->base[0][lane] = valid ? d.U : nullptr;
-    S->base[1][lane] = valid ? d.V : nullptr;
-    S->base[2][lane] = valid ? d.A : nullptr;
+    if constexpr (!EM) {
+        wave_sync();   /* the previous system's flushes have read the buffers */
+        S->base[0][lane] = valid ? d.U : nullptr;
+        S->base[1][lane] = valid ? d.V : nullptr;
+        S->base[2][lane] = valid ? d.A : nullptr;
+    }
 
     /* ---- forward elimination (source/auxiliary.f90:25-41) ----
      * Two-deep pipeline: iteration j eliminates row j (which needs the
@@ -301,7 +314,7 @@ __device__ static inline void solve_system(const oco_sys& d, bool valid, const d
     scr[0 * OCO_WAVE] = 0.0;
     scr[1 * OCO_WAVE] = u;
     scr[2 * OCO_WAVE] = pB.alpha1;
-    stage_put<R>(S->ub, 0, pB.alpha);
+    if constexpr (!EM) stage_put<R>(S->ub, 0, pB.alpha);
     double aA = pB.alpha;   /* alpha of row j-1 */
     pB = pC;                /* row j = 1 */
     pC = point_sys<GREY, WS, MIX>(d, 2, rec, w, M);
@@ -332,11 +345,13 @@ __device__ static inline void solve_system(const oco_sys& d, bool valid, const d
         sp[0] = gam;
         sp[OCO_WAVE] = u;
         sp[2 * OCO_WAVE] = pB.alpha1;
-        stage_put<R>(S->ub, j, pB.alpha);
-        if ((j & (R - 1)) == R - 1) {
-            wave_sync();
-            stage_flush<R>(S->ub, S->base[2], j / R, nz);
-            wave_sync();
+        if constexpr (!EM) {
+            stage_put<R>(S->ub, j, pB.alpha);
+            if ((j & (R - 1)) == R - 1) {
+                wave_sync();
+                stage_flush<R>(S->ub, S->base[2], j / R, nz);
+                wave_sync();
+            }
         }
         /* point j+2 (independent of the chain above) */
         oco_pt pD = point_sys<GREY, WS, MIX>(d, j + 2, rec, w, M);
@@ -369,11 +384,13 @@ __device__ static inline void solve_system(const oco_sys& d, bool valid, const d
         sp[0] = gam;
         sp[OCO_WAVE] = u;
         sp[2 * OCO_WAVE] = pB.alpha1;
-        stage_put<R>(S->ub, j, pB.alpha);
-        if ((j & (R - 1)) == R - 1) {
-            wave_sync();
-            stage_flush<R>(S->ub, S->base[2], j / R, nz);
-            wave_sync();
+        if constexpr (!EM) {
+            stage_put<R>(S->ub, j, pB.alpha);
+            if ((j & (R - 1)) == R - 1) {
+                wave_sync();
+                stage_flush<R>(S->ub, S->base[2], j / R, nz);
+                wave_sync();
+            }
         }
         j = nz - 1;
         if constexpr (MIX) {
@@ -397,9 +414,11 @@ __device__ static inline void solve_system(const oco_sys& d, bool valid, const d
         sp[0] = gam;
         sp[OCO_WAVE] = u;
         sp[2 * OCO_WAVE] = pC.alpha1;
-        stage_put<R>(S->ub, j, pC.alpha);
-        wave_sync();
-        stage_flush<R>(S->ub, S->base[2], j / R, nz);   /* last (possibly partial) block */
+        if constexpr (!EM) {
+            stage_put<R>(S->ub, j, pC.alpha);
+            wave_sync();
+            stage_flush<R>(S->ub, S->base[2], j / R, nz);   /* last (possibly partial) block */
+        }
     }
     (void)om;
     (void)Dprev;
@@ -407,10 +426,13 @@ __device__ static inline void solve_system(const oco_sys& d, bool valid, const d
         /* the reference stops at this system (source/auxiliary.f90:27-39):
          * its U and V are not written (the difference form has no zero pivot) */
         if (valid) record_fail(fail, d.key, fail_stage, fail_row);
-        S->base[0][lane] = nullptr;
-        S->base[1][lane] = nullptr;
+        if constexpr (!EM) {
+            S->base[0][lane] = nullptr;
+            S->base[1][lane] = nullptr;
+        }
+        valid = false;
     }
-    wave_sync();
+    if constexpr (!EM) wave_sync();
 
     /* ---- back-substitution (:43-45) fused with update_ghosts (:49-62),
      *      der (:64-78) and calc_flux (:96-109) ---- */
@@ -435,7 +457,7 @@ __device__ static inline void solve_system(const oco_sys& d, bool valid, const d
     double g1 = 2.0 * t[0] - t[1], g2 = 2.0 * t[0] - t[2], g3 = 2.0 * t[0] - t[3];
     /* U(e) for e = nz+5 down to nz-1, in descending order: a block is
      * flushed when its lowest element is staged */
-    {
+    if constexpr (!EM) {
         const double top[7] = {g3, g2, g1, t[0], t[1], t[2], t[3]};
 #pragma unroll
         for (int k = 0; k < 7; k++) {
@@ -451,11 +473,16 @@ __device__ static inline void solve_system(const oco_sys& d, bool valid, const d
     double w0 = t[3], w1 = t[2], w2 = t[1], w3 = t[0], w4 = g1, w5 = g2, w6 = g3;
     {
         double df = c60 * (((45.0 * (w4 - w2)) - (9.0 * (w5 - w1))) + (w6 - w0));
-        stage_put<R>(S->vb, nz - 1, ((-1.0 * a1q3) * df) * dz1);
-        if (((nz - 1) & (R - 1)) == 0) {
-            wave_sync();
-            stage_flush<R>(S->vb, S->base[1], (nz - 1) / R, nz);
-            wave_sync();
+        const double vtop = ((-1.0 * a1q3) * df) * dz1;
+        if constexpr (EM) {
+            if (valid) d.V[1] = vtop;
+        } else {
+            stage_put<R>(S->vb, nz - 1, vtop);
+            if (((nz - 1) & (R - 1)) == 0) {
+                wave_sync();
+                stage_flush<R>(S->vb, S->base[1], (nz - 1) / R, nz);
+                wave_sync();
+            }
         }
     }
     /* rows nz-5 .. 0 in chunks of BC: each chunk's scratch
@@ -486,15 +513,17 @@ __device__ static inline void solve_system(const oco_sys& d, bool valid, const d
                     gnext = cg[k];
                     a1q3 = a1q2; a1q2 = a1q1; a1q1 = a1q0; a1q0 = ca[k];
                     const int e = j + 3;   /* U(e) and V(z = e) */
-                    stage_put<R>(S->ub, e, Uj);
                     w6 = w5; w5 = w4; w4 = w3; w3 = w2; w2 = w1; w1 = w0; w0 = Uj;
-                    double df = c60 * (((45.0 * (w4 - w2)) - (9.0 * (w5 - w1))) + (w6 - w0));
-                    stage_put<R>(S->vb, e, ((-1.0 * a1q3) * df) * dz1);
-                    if ((e & (R - 1)) == 0) {
-                        wave_sync();
-                        stage_flush<R>(S->ub, S->base[0], e / R, nz + 6);
-                        stage_flush<R>(S->vb, S->base[1], e / R, nz);
-                        wave_sync();
+                    if constexpr (!EM) {
+                        stage_put<R>(S->ub, e, Uj);
+                        double df = c60 * (((45.0 * (w4 - w2)) - (9.0 * (w5 - w1))) + (w6 - w0));
+                        stage_put<R>(S->vb, e, ((-1.0 * a1q3) * df) * dz1);
+                        if ((e & (R - 1)) == 0) {
+                            wave_sync();
+                            stage_flush<R>(S->ub, S->base[0], e / R, nz + 6);
+                            stage_flush<R>(S->vb, S->base[1], e / R, nz);
+                            wave_sync();
+                        }
                     }
                 }
             }
@@ -504,20 +533,31 @@ __device__ static inline void solve_system(const oco_sys& d, bool valid, const d
     }
     /* bottom ghosts (f(n1-i) = 2 f(n1) - f(n1+i)); window now e = 3..9 */
     double u0 = w0, G1 = 2.0 * u0 - w1, G2 = 2.0 * u0 - w2, G3 = 2.0 * u0 - w3;
-    stage_put<R>(S->ub, 2, G1);
-    stage_put<R>(S->ub, 1, G2);
-    stage_put<R>(S->ub, 0, G3);
+    if constexpr (!EM) {
+        stage_put<R>(S->ub, 2, G1);
+        stage_put<R>(S->ub, 1, G2);
+        stage_put<R>(S->ub, 0, G3);
+    }
     const double gh[3] = {G1, G2, G3};
 #pragma unroll
     for (int k = 0; k < 3; k++) {
         w6 = w5; w5 = w4; w4 = w3; w3 = w2; w2 = w1; w1 = w0; w0 = gh[k];
         a1q3 = a1q2; a1q2 = a1q1; a1q1 = a1q0;
-        double df = c60 * (((45.0 * (w4 - w2)) - (9.0 * (w5 - w1))) + (w6 - w0));
-        stage_put<R>(S->vb, 2 - k, ((-1.0 * a1q3) * df) * dz1);
+        if (!EM || k == 2) {
+            double df = c60 * (((45.0 * (w4 - w2)) - (9.0 * (w5 - w1))) + (w6 - w0));
+            const double v = ((-1.0 * a1q3) * df) * dz1;
+            if constexpr (EM) {
+                if (valid) d.V[0] = v;
+            } else {
+                stage_put<R>(S->vb, 2 - k, v);
+            }
+        }
     }
-    wave_sync();
-    stage_flush<R>(S->ub, S->base[0], 0, nz + 6);
-    stage_flush<R>(S->vb, S->base[1], 0, nz);
+    if constexpr (!EM) {
+        wave_sync();
+        stage_flush<R>(S->ub, S->base[0], 0, nz + 6);
+        stage_flush<R>(S->vb, S->base[1], 0, nz);
+    }
 }
 
 /* One wavefront per workgroup, grid-stride over tiles of 64 systems; the
@@ -891,8 +931,23 @@ __global__ void batch_reset(unsigned int* counter, unsigned long long* fail)
 {
     if (threadIdx.x == 0 && blockIdx.x == 0) {
         *counter = 0u;
-        *fail = OCO_FAIL_NONE;
+        if (fail) *fail = OCO_FAIL_NONE;
     }
+}
+
+/* fill n failure keys with OCO_FAIL_NONE (serving: a slot's key at admission) */
+__global__ void fail_reset(unsigned long long* fail, int n)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) fail[i] = OCO_FAIL_NONE;
+}
+
+hipError_t oco_launch_fail_reset(unsigned long long* fail, int n, hipStream_t stream)
+{
+    (void)hipGetLastError();   /* a pending not-ready status from an event query is not this launch's */
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(fail_reset, dim3((n + 63) / 64), dim3(64), 0, stream, fail, n);
+    return hipGetLastError();
 }
 
 typedef const __attribute__((address_space(4))) int* oco_cptr_i;
@@ -965,8 +1020,72 @@ __global__ __launch_bounds__(OCO_WAVE) void batch_mixed(const oco_batch_args a)
     batch_tiles<BC, true>(a, nullptr, &tile_s, &stg);
 }
 
+/* ---- serving: tiles of many jobs in one persistent launch ----
+ * Each tile is (job slot, column, first of 64 wavelengths); the job's
+ * descriptor gives its cell records, wavelength table and emergent-flux
+ * output.  A launch mixes tiles of every job admitted so far, so jobs that
+ * arrive while others run join at the next launch (continuous batching);
+ * each system is solved by one lane exactly as in a launch of its own. */
+template <int BC, bool MIX>
+__global__ __launch_bounds__(OCO_WAVE) void serve_tiles(const oco_serve_args a)
+{
+    __shared__ oco_mtab smt;
+    __shared__ int tile_s;
+    __shared__ oco_stage_emergent none;
+    if (!MIX) load_mtab(&smt);
+    const int lane = threadIdx.x;
+    double* scr = a.scratch + (long)blockIdx.x * a.scr_stride + lane;
+    oco_cptr_i tiles = (oco_cptr_i)a.tiles;
+    for (;;) {
+        if (lane == 0) tile_s = (int)atomicAdd(a.counter, 1u);
+        __syncthreads();
+        const int t = __builtin_amdgcn_readfirstlane(tile_s);
+        __syncthreads();
+        if (t >= a.ntiles) break;
+        const int slot = tiles[3 * t];
+        const long col = tiles[3 * t + 1];
+        const oco_serve_job jb = a.jobs[slot];
+        const bool valid = tiles[3 * t + 2] + lane < jb.nw;
+        const int iw = valid ? tiles[3 * t + 2] + lane : jb.nw - 1;
+        oco_sys d;
+        d.nz = jb.nz;
+        d.dz = jb.dz; d.dz1 = jb.dz1; d.dz2 = jb.dz2;
+        d.sigma_grey = 0.0;
+        d.cells = jb.cells + col * jb.nz * OCO_CELL_STRIDE;
+        d.rho = nullptr; d.T = nullptr;
+        d.U = nullptr; d.A = nullptr;
+        d.V = jb.em + ((long)iw * jb.ncol + col) * 2;
+        d.key = col * jb.nw + iw;
+        double w[OCO_WT_STRIDE];
+        const double* wr = jb.wtab + (long)iw * OCO_WT_STRIDE;
+#pragma unroll
+        for (int k = 0; k < OCO_WT_STRIDE; k++) w[k] = wr[k];
+        solve_system<false, true, 1, BC, MIX, oco_stage_emergent>(d, valid, w, scr, a.fail + slot,
+                                                                  MIX ? nullptr : &smt, &none);
+    }
+}
+
+hipError_t oco_launch_serve(const oco_serve_args& a, int mixed, int nblocks, hipStream_t stream)
+{
+    (void)hipGetLastError();   /* a pending not-ready status from an event query is not this launch's */
+    hipLaunchKernelGGL(batch_reset, dim3(1), dim3(64), 0, stream, a.counter, (unsigned long long*)nullptr);
+    if (mixed) hipLaunchKernelGGL((serve_tiles<2, true>), dim3(nblocks), dim3(OCO_WAVE), 0, stream, a);
+    else hipLaunchKernelGGL((serve_tiles<2, false>), dim3(nblocks), dim3(OCO_WAVE), 0, stream, a);
+    return hipGetLastError();
+}
+
+int oco_serve_grid(int mixed, int device)
+{
+    int per_cu = 0, cus = 0;
+    const void* k = mixed ? (const void*)serve_tiles<2, true> : (const void*)serve_tiles<2, false>;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, OCO_WAVE, 0) != hipSuccess || per_cu < 1) per_cu = 4;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess || cus < 1) cus = 256;
+    return per_cu * cus;
+}
+
 hipError_t oco_launch_batch(const oco_batch_args& a, int mixed, int variant, int nblocks, hipStream_t stream)
 {
+    (void)hipGetLastError();   /* a pending not-ready status from an event query is not this launch's */
     hipLaunchKernelGGL(batch_reset, dim3(1), dim3(64), 0, stream, a.counter, a.fail);
     const dim3 g(nblocks), b(OCO_WAVE);
     if (mixed) {
@@ -990,6 +1109,7 @@ int oco_batch_variants(void) { return 3; }
 hipError_t oco_launch_cells(const double* rho, const double* T, long ncells, double fully_ionized_T,
                             double* cells, int soa_nz, hipStream_t stream)
 {
+    (void)hipGetLastError();
     if (ncells <= 0) return hipSuccess;
     long blocks = (ncells + 255) / 256;
     if (blocks > 65536) blocks = 65536;
@@ -1021,6 +1141,7 @@ int oco_sweep_grid(int lgrey, int mixed, int uniform, int device)
 
 hipError_t oco_launch_sweep(const oco_sweep_args& a, int lgrey, int mixed, int wave, int nblocks, hipStream_t stream)
 {
+    (void)hipGetLastError();   /* a pending not-ready status from an event query is not this launch's */
     const bool uniform = (a.nw % OCO_WAVE) == 0;
     if (a.canary) {
         if (lgrey)

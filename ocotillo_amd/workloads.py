@@ -126,3 +126,28 @@ def config4(nx=256, ny=256, nz=512, nw=128, seed=SEED + 4, sigma_dex=1.0, corr_c
     np.maximum(rho, rho_floor, out=rho)
     T = np.clip(T_MID * (rho / rho0) ** 0.3, T_FLOOR, 2e5)
     return rho, T, dz, log_waves(nw)
+
+
+def config5_shapes(njobs, seed=SEED + 5, nxy=(16, 256), nz=(64, 2048), nw=(16, 512)):
+    """Job shapes of the serving workload: nx, ny ~ U{16..256},
+    nz ~ U{64..2048}, nw ~ U{16..512} (mean 5.16e9 points per job)."""
+    rng = np.random.default_rng(seed)
+    return [(int(rng.integers(nxy[0], nxy[1] + 1)), int(rng.integers(nxy[0], nxy[1] + 1)),
+             int(rng.integers(nz[0], nz[1] + 1)), int(rng.integers(nw[0], nw[1] + 1))) for _ in range(njobs)]
+
+
+def config5_job(nx, ny, nz, nw, seed, sigma_dex=0.5):
+    """One serving job: disk profiles with a lognormal rho perturbation as in
+    config3, nw log-spaced 100 A - 10 um.  Returns rho, T ([nx, ny, nz]), dz,
+    waves."""
+    rng = np.random.default_rng(seed)
+    rho, T, dz = disk_columns(nx, ny, nz, rng=rng)
+    rho *= 10.0 ** (sigma_dex * rng.standard_normal(rho.shape))
+    np.maximum(rho, RHO_FLOOR, out=rho)
+    return rho, T, dz, log_waves(nw)
+
+
+def poisson_arrivals(njobs, rate, seed=SEED + 5):
+    """Arrival times (s) of a Poisson stream of `rate` jobs per second."""
+    rng = np.random.default_rng(seed + 1000)
+    return np.cumsum(rng.exponential(1.0 / rate, njobs))

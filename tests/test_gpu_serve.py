@@ -1,0 +1,148 @@
+"""Serving loop (oco_server_*): continuous batching of jobs of different
+shapes and wavelength grids, HBM-budget admission, emergent-flux outputs.
+
+Bars:
+  * a job's result is bit-identical alone and in a mix of concurrently
+    submitted jobs (every system is solved by one lane, whatever else shares
+    the launch);
+  * the emergent flux V(1), V(nz) equals the block path's V[..., 0] and
+    V[..., -1] bit for bit, in both precision modes (same per-system code);
+    in FP64 mode that is the oracle's result bit for bit;
+  * admission keeps the job arena within its budget; a job larger than the
+    budget is refused at submit; the reference's zero-pivot stop is reported
+    per job with the reference's (ix, iy, iw).
+"""
+import os
+
+import numpy as np
+import pytest
+
+from conftest import golden
+
+pytestmark = pytest.mark.gpu
+
+GiB = 1 << 30
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    import ocotillo_amd as oco
+    with oco.Context(0) as c:
+        c.set_physics(2e4, False, 0.4)
+        yield c
+
+
+def lane_context():
+    """Block solves with the lane-per-system mixed sweep, the serving kernel's."""
+    import ocotillo_amd as oco
+    old = os.environ.get("OCO_MIXED_SWEEP")
+    os.environ["OCO_MIXED_SWEEP"] = "lane"
+    try:
+        c = oco.Context(0)
+    finally:
+        if old is None:
+            del os.environ["OCO_MIXED_SWEEP"]
+        else:
+            os.environ["OCO_MIXED_SWEEP"] = old
+    c.set_physics(2e4, False, 0.4)
+    return c
+
+
+def make_job(seed, nx, ny, nz, nw):
+    from ocotillo_amd.workloads import disk_columns, log_waves
+    rho, T, dz = disk_columns(nx, ny, nz, seed=seed)
+    return rho, T, dz, log_waves(nw)
+
+
+JOBS = [(11, 3, 2, 96, 70), (12, 2, 2, 300, 16), (13, 4, 3, 64, 130), (14, 1, 5, 513, 40),
+        (15, 2, 3, 200, 200), (16, 5, 2, 128, 64), (17, 1, 1, 2048, 33), (18, 3, 3, 150, 90)]
+
+
+@pytest.mark.parametrize("mode_name", ["mixed", "fp64"])
+def test_alone_equals_in_the_mix(ctx, mode_name):
+    import ocotillo_amd as oco
+    mode = oco.MODE_MIXED if mode_name == "mixed" else oco.MODE_FP64
+    probe = make_job(*JOBS[0])
+    print("case alone", mode_name, flush=True)
+    with oco.Server(ctx, 1 * GiB, max_nz=2048, mode=mode) as srv:
+        alone, r = srv.wait(srv.submit(*probe))
+        assert r.status == 0
+    print("case mix", mode_name, flush=True)
+    with oco.Server(ctx, 1 * GiB, max_nz=2048, mode=mode) as srv:
+        tickets = [srv.submit(*make_job(*j)) for j in JOBS[1:4]]
+        tp = srv.submit(*probe)
+        tickets += [srv.submit(*make_job(*j)) for j in JOBS[4:]]
+        mixed_out, r = srv.wait(tp)
+        others = [srv.wait(t) for t in tickets]
+        st = srv.stats()
+    assert r.status == 0 and all(o[1].status == 0 for o in others)
+    assert np.array_equal(alone, mixed_out)
+    assert st.completed == len(JOBS) and st.iterations >= 1
+    print("  iterations %d, p50 %.1f ms, p99 %.1f ms" % (st.iterations, st.p50_ms, st.p99_ms), flush=True)
+
+
+@pytest.mark.parametrize("mode_name", ["mixed", "fp64"])
+def test_emergent_equals_block_path(mode_name):
+    import ocotillo_amd as oco
+    mode = oco.MODE_MIXED if mode_name == "mixed" else oco.MODE_FP64
+    jobs = [make_job(*j) for j in JOBS[:3]]
+    with lane_context() as c:
+        with oco.Server(c, 1 * GiB, max_nz=1024, mode=mode) as srv:
+            tickets = [srv.submit(*j) for j in jobs]
+            outs = [srv.wait(t) for t in tickets]
+        for (rho, T, dz, waves), (em, r) in zip(jobs, outs):
+            print("case", rho.shape, waves.size, flush=True)
+            assert r.status == 0
+            c.set_wavelengths(waves)
+            U, V, A, d = c.solve_block(rho, T, dz, mode=mode)
+            assert np.array_equal(em[..., 0], V[..., 0])
+            assert np.array_equal(em[..., 1], V[..., -1])
+            if mode == oco.MODE_FP64:
+                from oracle.oracle import Oracle
+                _, Vo, _, _ = Oracle(waves, portable=True).solve_block(rho, T, dz)
+                assert np.array_equal(em[..., 0], Vo[..., 0]) and np.array_equal(em[..., 1], Vo[..., -1])
+
+
+def test_admission_within_budget(ctx):
+    """A budget that holds a few jobs at a time: all complete, the arena's
+    high-water mark stays within the budget, later jobs queue."""
+    import ocotillo_amd as oco
+    # a job: 16 columns x 256 cells -> inputs 2 x 32 KiB, cell records 768 KiB,
+    # emergent flux 16 KiB; the arena holds two of them beside the table
+    job_bytes = 2 * 32768 + 16 * 256 * 24 * 8 + 16 * 64 * 2 * 8
+    budget = 2 * job_bytes + (64 << 10)
+    jobs = [make_job(100 + i, 4, 4, 256, 64) for i in range(10)]
+    with oco.Server(ctx, max(budget, 1 << 20), max_nz=256, mode=oco.MODE_MIXED) as srv:
+        budget = srv.stats().hbm_budget
+        tickets = [srv.submit(*j) for j in jobs]
+        res = [srv.wait(t) for t in tickets]
+        st = srv.stats()
+    assert all(r[1].status == 0 for r in res)
+    assert job_bytes <= st.hbm_peak <= budget
+    assert st.hbm_peak < 3 * job_bytes + (64 << 10)   # never three admitted at once
+    print("  peak %.2f MiB of %.2f MiB" % (st.hbm_peak / 2 ** 20, budget / 2 ** 20), flush=True)
+
+
+def test_oversize_job_refused(ctx):
+    import ocotillo_amd as oco
+    with oco.Server(ctx, 4 << 20, max_nz=256, mode=oco.MODE_MIXED) as srv:
+        with pytest.raises(oco.OcoError) as e:
+            srv.submit(*make_job(1, 16, 16, 256, 64))
+        assert e.value.status == oco.E_OOM
+
+
+def test_pivot_reported_per_job(ctx):
+    """The reference stops at its first zero pivot; in FP64 mode the job
+    reports that system, other jobs are unaffected."""
+    import ocotillo_amd as oco
+    g = golden("dump_pivot0.npz")
+    ncol, nz = int(g["ncol"]), int(g["nz"])
+    rho, T = g["rho"].reshape(1, ncol, nz), g["T"].reshape(1, ncol, nz)
+    with oco.Server(ctx, 1 * GiB, max_nz=nz, mode=oco.MODE_FP64) as srv:
+        t_ok = srv.submit(*make_job(*JOBS[0]))
+        t_bad = srv.submit(rho, T, float(g["dz"]), g["waves"])
+        _, r_ok = srv.wait(t_ok)
+        _, r_bad = srv.wait(t_bad, raise_on_pivot=False)
+    assert r_ok.status == 0
+    assert r_bad.status == int(g["ref_stop_stage"])
+    assert (r_bad.fail_ix, r_bad.fail_iy, r_bad.fail_iw) == (1, int(g["ref_stop_col"]), int(g["ref_stop_iw"]))
