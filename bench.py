@@ -51,10 +51,13 @@ W_IONISED = 143 + 7 * (W_DIV - 1) + 2 * W_EXP10 + W_EXP + 3 * W_LOG + 3 * W_EXP
 WM_EXP, WM_EXP10, WM_LOG, WM_OMEXP, WM_EXPM1, WM_DIV = 10, 10, 8, 23, 23, 4
 WM_NEUTRAL = 138 + 7 * (WM_DIV - 1) + WM_OMEXP + WM_EXP10 + WM_EXPM1
 WM_IONISED = 143 + 7 * (WM_DIV - 1) + WM_OMEXP + WM_EXP10 + WM_EXPM1 + 3 * WM_LOG + 2 * WM_EXP + WM_OMEXP
-# HBM bytes per sweep launch on the 64x64x256x128 block from the rocprofv3
-# FETCH_SIZE / WRITE_SIZE passes (x2.0 / x1.0 copy-kernel calibration); None
-# where not measured
-SWEEP_TRAFFIC_BYTES = None
+# HBM bytes per sweep launch, (nx, ny, nz, nw, kernel) -> bytes, from the
+# rocprofv3 FETCH_SIZE / WRITE_SIZE passes over one launch, scaled by the
+# copy-kernel calibration (x2.0 / x1.0); workloads not listed report None
+SWEEP_TRAFFIC_BYTES = {
+    (256, 256, 512, 128, "sweep_mixed"): 3.40102e11,
+    (256, 256, 512, 128, "sweep_fp64"): 3.55494e11,
+}
 
 
 def ionised_fraction(rho, T, fully_ionized_T=2e4):
@@ -222,7 +225,7 @@ def bench_block(args):
         ms = (t1 - t0) * 1e3 / args.steps
         sw = float(np.mean(sweep_ms))
         kern = mixed_kernel_name(nz) if mode == "mixed" else "sweep_fp64"
-        traffic = SWEEP_TRAFFIC_BYTES if (mode == "fp64" and args.config == "c2") else None
+        traffic = SWEEP_TRAFFIC_BYTES.get((nx, ny, nz, nw, kern))
         roof = roofline_of(kern, points, point_weights(mode, fion), sw, traffic)
         roof["cells_kernel_ms"] = round(float(np.mean(cells_ms)), 4)
         gbs = sweep_bytes(nz, ncol, nw) / (sw * 1e-3) / 1e9
