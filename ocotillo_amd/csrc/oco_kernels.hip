@@ -256,6 +256,54 @@ __device__ static inline oco_pt point_sys(const oco_sys& d, int z, const double*
 #define OCO_SWEEP_ATTR
 #endif
 
+/* Elimination scratch of the lane-per-system sweeps, one row per call:
+ *   FP64 drop-in: [gam, u, 1/alpha] as doubles, 24 bytes;
+ *   mixed: [Z double][D as a float pair hi + lo][1/alpha float], 20 bytes,
+ *   each of the three a coalesced line of 64 lanes.
+ * D = hi + lo keeps ~2^-48 relative, so long runs of D -> 1 (optically thin
+ * rows) add no error along the back-substitution; 1/alpha in FP32 costs
+ * 6e-8 on V.  `scr` is the lane's slot pointer (wave slot + lane). */
+constexpr long SCR_ROW_MIXED = 160;   /* doubles per row of a wave's mixed slot: 64 + 64 + 32 */
+
+template <bool MIX>
+__device__ static inline void scr_put(double* scr, int j, double gam, double u, double a1)
+{
+    if constexpr (MIX) {
+        const int lane = threadIdx.x;
+        double* sp = scr + (long)j * SCR_ROW_MIXED;
+        sp[0] = u;
+        const double D = -gam;
+        float2 pk;
+        pk.x = (float)D;
+        pk.y = (float)(D - (double)pk.x);
+        *(float2*)(sp + OCO_WAVE) = pk;
+        ((float*)(sp - lane + 2 * OCO_WAVE))[lane] = (float)a1;
+    } else {
+        double* sp = scr + (long)j * 3 * OCO_WAVE;
+        sp[0] = gam;
+        sp[OCO_WAVE] = u;
+        sp[2 * OCO_WAVE] = a1;
+    }
+}
+
+template <bool MIX>
+__device__ static inline void scr_get(const double* scr, int j, double& gam, double& u, double& a1)
+{
+    if constexpr (MIX) {
+        const int lane = threadIdx.x;
+        const double* sp = scr + (long)j * SCR_ROW_MIXED;
+        u = sp[0];
+        const float2 pk = *(const float2*)(sp + OCO_WAVE);
+        gam = -((double)pk.x + (double)pk.y);
+        a1 = (double)((const float*)(sp - lane + 2 * OCO_WAVE))[lane];
+    } else {
+        const double* sp = scr + (long)j * 3 * OCO_WAVE;
+        gam = sp[0];
+        u = sp[OCO_WAVE];
+        a1 = sp[2 * OCO_WAVE];
+    }
+}
+
 /* One (column, wavelength) system, one lane: forward elimination with the
  * opacity of the cells recomputed in registers, then back-substitution fused
  * with ghosts, der and V.  `w` is the lane's wavelength-table row with
@@ -311,9 +359,7 @@ __device__ static inline void solve_system(const oco_sys& d, bool valid, const d
         bet = bv;
         u = dv / bet;
     }
-    scr[0 * OCO_WAVE] = 0.0;
-    scr[1 * OCO_WAVE] = u;
-    scr[2 * OCO_WAVE] = pB.alpha1;
+    scr_put<MIX>(scr, 0, 0.0, u, pB.alpha1);
     if constexpr (!EM) stage_put<R>(S->ub, 0, pB.alpha);
     double aA = pB.alpha;   /* alpha of row j-1 */
     pB = pC;                /* row j = 1 */
@@ -341,10 +387,7 @@ __device__ static inline void solve_system(const oco_sys& d, bool valid, const d
             fail_stage = first ? 2 : fail_stage;
             u = (dv - av * u) / bet;
         }
-        double* sp = scr + (long)j * 3 * OCO_WAVE;
-        sp[0] = gam;
-        sp[OCO_WAVE] = u;
-        sp[2 * OCO_WAVE] = pB.alpha1;
+        scr_put<MIX>(scr, j, gam, u, pB.alpha1);
         if constexpr (!EM) {
             stage_put<R>(S->ub, j, pB.alpha);
             if ((j & (R - 1)) == R - 1) {
@@ -380,10 +423,7 @@ __device__ static inline void solve_system(const oco_sys& d, bool valid, const d
             fail_stage = first ? 2 : fail_stage;
             u = (dv - av * u) / bet;
         }
-        double* sp = scr + (long)j * 3 * OCO_WAVE;
-        sp[0] = gam;
-        sp[OCO_WAVE] = u;
-        sp[2 * OCO_WAVE] = pB.alpha1;
+        scr_put<MIX>(scr, j, gam, u, pB.alpha1);
         if constexpr (!EM) {
             stage_put<R>(S->ub, j, pB.alpha);
             if ((j & (R - 1)) == R - 1) {
@@ -410,10 +450,7 @@ __device__ static inline void solve_system(const oco_sys& d, bool valid, const d
             fail_stage = first ? 2 : fail_stage;
             u = (dv - av * u) / bet;
         }
-        sp = scr + (long)j * 3 * OCO_WAVE;
-        sp[0] = gam;
-        sp[OCO_WAVE] = u;
-        sp[2 * OCO_WAVE] = pC.alpha1;
+        scr_put<MIX>(scr, j, gam, u, pC.alpha1);
         if constexpr (!EM) {
             stage_put<R>(S->ub, j, pC.alpha);
             wave_sync();
@@ -439,17 +476,18 @@ __device__ static inline void solve_system(const oco_sys& d, bool valid, const d
     const double c60 = 1.0 / 60.0;
     /* window w0..w6 = extended U at e = j .. j+6 (e = z + 3) */
     double Un = u;                       /* U(nz-1) */
-    double gnext = scr[(long)(nz - 1) * 3 * OCO_WAVE];
-    double a1q0 = scr[(long)(nz - 1) * 3 * OCO_WAVE + 2 * OCO_WAVE];   /* 1/alpha of the V row pending */
+    double gnext, a1q0, udum;            /* gam of the row above; 1/alpha of the V row pending */
+    scr_get<MIX>(scr, nz - 1, gnext, udum, a1q0);
+    (void)udum;
     double a1q1 = 0.0, a1q2 = 0.0, a1q3 = 0.0;
     double t[4];
     t[0] = Un;
     for (int k = 1; k < 4; k++) {
         int j = nz - 1 - k;
-        const double* sp = scr + (long)j * 3 * OCO_WAVE;
-        double Uj = sp[OCO_WAVE] - gnext * t[k - 1];
-        gnext = sp[0];
-        double a1 = sp[2 * OCO_WAVE];
+        double gj, uj, a1;
+        scr_get<MIX>(scr, j, gj, uj, a1);
+        double Uj = uj - gnext * t[k - 1];
+        gnext = gj;
         a1q3 = a1q2; a1q2 = a1q1; a1q1 = a1q0; a1q0 = a1;
         t[k] = Uj;
     }
@@ -495,15 +533,13 @@ __device__ static inline void solve_system(const oco_sys& d, bool valid, const d
 #pragma unroll
         for (int k = 0; k < BC; k++) {
             int j = jc - k;
-            const double* sp = scr + (long)(j < 0 ? 0 : j) * 3 * OCO_WAVE;
-            cg[k] = sp[0]; cu[k] = sp[OCO_WAVE]; ca[k] = sp[2 * OCO_WAVE];
+            scr_get<MIX>(scr, j < 0 ? 0 : j, cg[k], cu[k], ca[k]);
         }
         for (; jc >= 0; jc -= BC) {
 #pragma unroll
             for (int k = 0; k < BC; k++) {      /* next chunk, in flight */
                 int j = jc - BC - k;
-                const double* sp = scr + (long)(j < 0 ? 0 : j) * 3 * OCO_WAVE;
-                ng[k] = sp[0]; nu[k] = sp[OCO_WAVE]; na[k] = sp[2 * OCO_WAVE];
+                scr_get<MIX>(scr, j < 0 ? 0 : j, ng[k], nu[k], na[k]);
             }
 #pragma unroll
             for (int k = 0; k < BC; k++) {
