@@ -94,6 +94,32 @@ int oco_solve_block_device(oco_ctx* ctx, int nz, int nyloc, int nxloc, double dz
                            double dz2, const double* d_rho3d, const double* d_T3d, double* d_U,
                            double* d_V, double* d_absorp, int mode, oco_diag* diag);
 
+/* ---- Athena input construction ----
+ * The arithmetic that turns one Athena z slab (the file id<iproc>.<snap>.bin
+ * of one iprocz: raw density, three momenta and total energy in code units,
+ * each (nxloc, nyloc, nzl) in the file's order) into the block's rho3d, T3d
+ * (nz, nyloc, nxloc) at z offset iz0 (0-based): replaces
+ * source/read_athena.f90:163-205 (kinetic and internal energy, with the
+ * momentum-underflow guard of :213-228), :232-258 (density unit and floor,
+ * temperature unit) and the temperature rule source/disk.f90:96-104.  The
+ * host reads the file and provides the units (read_athena_input,
+ * source/read_athena.f90:33-62), switchTemp and the Gaussian profile
+ * tgauss(z) = midTemp exp(-z^2/(2 sigma^2)) (source/disk.f90:28, 89). */
+typedef struct {
+    double unit_density, unit_temperature, gamma1, rho_floor, switch_temp, floor_temp;
+    int avoid_momentum_underflow;   /* lavoid_momentum_underflow */
+} oco_athena_units;
+
+/* Host arrays: the slab's z range of rho3d, T3d is written, the rest kept. */
+int oco_athena_slab(oco_ctx* ctx, int nz, int nyloc, int nxloc, int nzl, int iz0, const double* rho_raw,
+                    const double* mx, const double* my, const double* mz, const double* e, const double* tgauss,
+                    const oco_athena_units* units, double* rho3d, double* T3d);
+/* Device arrays (tgauss too), enqueued and completed before return. */
+int oco_athena_slab_device(oco_ctx* ctx, int nz, int nyloc, int nxloc, int nzl, int iz0, const double* d_rho_raw,
+                           const double* d_mx, const double* d_my, const double* d_mz, const double* d_e,
+                           const double* d_tgauss, const oco_athena_units* units, double* d_rho3d,
+                           double* d_T3d);
+
 /* ---- Batched uneven columns ----
  * Independent columns, each with its own nz, grid step dz and wavelength
  * grid, solved in one persistent launch.  Replaces running the reference

@@ -17,8 +17,8 @@ import os
 
 import numpy as np
 
-__all__ = ["OcoError", "Context", "Batch", "Server", "Diag", "load_library", "LIB_PATH", "MODE_FP64",
-           "MODE_MIXED", "EXPORTS"]
+__all__ = ["OcoError", "Context", "Batch", "Server", "Diag", "AthenaUnits", "load_library", "LIB_PATH",
+           "MODE_FP64", "MODE_MIXED", "EXPORTS"]
 
 PKG = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(PKG, "libocotillo.so")
@@ -28,7 +28,7 @@ MODE_FP64, MODE_MIXED = 0, 1
 
 # every symbol include/ocotillo.h declares
 EXPORTS = ["oco_create", "oco_destroy", "oco_set_wavelengths", "oco_set_physics", "oco_solve_block",
-           "oco_solve_block_device", "oco_strerror", "oco_version",
+           "oco_solve_block_device", "oco_strerror", "oco_version", "oco_athena_slab", "oco_athena_slab_device",
            "oco_batch_create", "oco_batch_destroy", "oco_batch_layout", "oco_batch_buffers", "oco_batch_run",
            "oco_batch_run_host", "oco_batch_tuning",
            "oco_server_create", "oco_server_submit", "oco_server_wait", "oco_server_poll", "oco_server_stats",
@@ -40,6 +40,12 @@ class Diag(C.Structure):
                 + [(n, C.c_double) for n in ("sum_a", "sum_b", "sum_c", "sum_d", "minU", "maxU",
                                              "minV", "maxV", "minA", "maxA", "cells_ms", "sweep_ms",
                                              "total_ms")])
+
+
+class AthenaUnits(C.Structure):
+    _fields_ = ([(n, C.c_double) for n in ("unit_density", "unit_temperature", "gamma1", "rho_floor",
+                                             "switch_temp", "floor_temp")]
+                + [("avoid_momentum_underflow", C.c_int)])
 
 
 class Job(C.Structure):
@@ -91,6 +97,10 @@ def load_library(path=None):
                                          C.c_void_p, C.c_int, C.POINTER(Diag)]
     _SZ = C.POINTER(C.c_size_t)
     _PP = C.POINTER(C.c_void_p)
+    L.oco_athena_slab.argtypes = ([C.c_void_p] + [C.c_int] * 5 + [_DP] * 6 + [C.POINTER(AthenaUnits)]
+                                  + [_DP] * 2)
+    L.oco_athena_slab_device.argtypes = ([C.c_void_p] + [C.c_int] * 5 + [C.c_void_p] * 6
+                                         + [C.POINTER(AthenaUnits)] + [C.c_void_p] * 2)
     L.oco_batch_create.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int), _DP, _DP,
                                    C.POINTER(C.c_void_p)]
     L.oco_batch_destroy.argtypes = [C.c_void_p]
@@ -200,6 +210,37 @@ class Context:
         if st != OK and (raise_on_pivot or st not in (E_PIVOT0_STAGE1, E_PIVOT0_STAGE2)):
             raise OcoError(st, "oco_solve_block_device: " + strerror(st), d)
         return d
+
+
+    def athena_slab(self, slab, tgauss, units, iz0, rho3d, T3d):
+        """Raw Athena fields of one z slab -> the slab's z range of rho3d, T3d
+        ([nxloc, nyloc, nz], written in place; replaces the arithmetic of
+        source/read_athena.f90:163-258 and source/disk.f90:96-104).
+        `slab` is a sequence (rho, Mx, My, Mz, E), each a C-ordered
+        [nzl, nyloc, nxloc] array (the file's Fortran (nxloc, nyloc, nzl))."""
+        f = [np.ascontiguousarray(a, dtype=np.float64) for a in slab]
+        if len(f) != 5 or any(a.ndim != 3 or a.shape != f[0].shape for a in f):
+            raise OcoError(E_ARG, "slab must be five [nzl, nyloc, nxloc] arrays")
+        nzl, nyloc, nxloc = f[0].shape
+        for a in (rho3d, T3d):
+            if a.dtype != np.float64 or not a.flags.c_contiguous or a.shape[:2] != (nxloc, nyloc):
+                raise OcoError(E_ARG, "rho3d and T3d must be C-contiguous float64 [nxloc, nyloc, nz]")
+        nz = rho3d.shape[2]
+        tg = np.ascontiguousarray(tgauss, dtype=np.float64)
+        if tg.size != nz or T3d.shape != rho3d.shape:
+            raise OcoError(E_ARG, "tgauss must hold nz values")
+        st = self.L.oco_athena_slab(self.h, nz, nyloc, nxloc, nzl, iz0, *[_ptr(a) for a in f], _ptr(tg),
+                                    C.byref(units), _ptr(rho3d), _ptr(T3d))
+        if st != OK:
+            raise OcoError(st, "oco_athena_slab: " + strerror(st))
+
+    def athena_slab_device(self, nz, nyloc, nxloc, nzl, iz0, d_fields, d_tgauss, units, d_rho3d, d_T3d):
+        """Device pointers: d_fields = (rho, Mx, My, Mz, E) of the slab."""
+        st = self.L.oco_athena_slab_device(self.h, nz, nyloc, nxloc, nzl, iz0,
+                                           *[C.c_void_p(p) for p in d_fields], C.c_void_p(d_tgauss),
+                                           C.byref(units), C.c_void_p(d_rho3d), C.c_void_p(d_T3d))
+        if st != OK:
+            raise OcoError(st, "oco_athena_slab_device: " + strerror(st))
 
 
 class Batch:

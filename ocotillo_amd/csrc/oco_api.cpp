@@ -36,7 +36,7 @@ struct oco_ctx {
     int lgrey = 0;
     double sigma_grey = 0.4;
     int mixed_sweep = 0;  /* mixed mode: 0 by column length, 1 wavefront per system, 2 lane per system */
-    DevBuf wtab, rho, T, cells, scratch, U, V, A, fail, canary;
+    DevBuf wtab, rho, T, cells, scratch, U, V, A, fail, canary, raw, tg;
 };
 
 static int hip_status(hipError_t e) { return e == hipSuccess ? OCO_OK : OCO_E_HIP; }
@@ -212,7 +212,7 @@ int oco_destroy(oco_ctx* c)
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     for (DevBuf* b : {&c->wtab, &c->rho, &c->T, &c->cells, &c->scratch, &c->U, &c->V, &c->A, &c->fail,
-                      &c->canary})
+                      &c->canary, &c->raw, &c->tg})
         release(c, *b);
     for (auto& e : c->ev)
         if (e) (void)hipEventDestroy(e);
@@ -391,6 +391,82 @@ int oco_solve_block(oco_ctx* c, int nz, int nyloc, int nxloc, double dz, double 
     HIPCHK(hipMemcpyAsync(absorp, c->A.p, nout * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
     return st;
+}
+
+/* ---------------------------------------------------------------------- */
+/* Athena input construction                                              */
+/* ---------------------------------------------------------------------- */
+
+static int athena_check(int nz, int nyloc, int nxloc, int nzl, int iz0, const oco_athena_units* u)
+{
+    if (nz < 1 || nyloc < 1 || nxloc < 1 || nzl < 1 || iz0 < 0 || iz0 + nzl > nz || !u) return OCO_E_ARG;
+    return OCO_OK;
+}
+
+static oco_athena_args athena_args(int nz, int nyloc, int nxloc, int nzl, int iz0, const oco_athena_units* u)
+{
+    oco_athena_args a;
+    a.nz = nz; a.nyloc = nyloc; a.nxloc = nxloc; a.nzl = nzl; a.iz0 = iz0;
+    a.unit_density = u->unit_density;
+    a.unit_temperature = u->unit_temperature;
+    a.gamma1 = u->gamma1;
+    a.rho_floor = u->rho_floor;
+    a.switch_temp = u->switch_temp;
+    a.floor_temp = u->floor_temp;
+    a.avoid_momentum_underflow = u->avoid_momentum_underflow;
+    return a;
+}
+
+int oco_athena_slab_device(oco_ctx* c, int nz, int nyloc, int nxloc, int nzl, int iz0, const double* rho_raw,
+                           const double* mx, const double* my, const double* mz, const double* e,
+                           const double* tgauss, const oco_athena_units* u, double* rho3d, double* T3d)
+{
+    if (!c || !rho_raw || !mx || !my || !mz || !e || !tgauss || !rho3d || !T3d) return OCO_E_ARG;
+    int st = athena_check(nz, nyloc, nxloc, nzl, iz0, u);
+    if (st) return st;
+    HIPCHK(hipSetDevice(c->device));
+    oco_athena_args a = athena_args(nz, nyloc, nxloc, nzl, iz0, u);
+    a.rho = rho_raw; a.mx = mx; a.my = my; a.mz = mz; a.e = e;
+    a.tgauss = tgauss;
+    a.rho3d = rho3d;
+    a.T3d = T3d;
+    HIPCHK(oco_launch_athena(a, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return OCO_OK;
+}
+
+int oco_athena_slab(oco_ctx* c, int nz, int nyloc, int nxloc, int nzl, int iz0, const double* rho_raw,
+                    const double* mx, const double* my, const double* mz, const double* e, const double* tgauss,
+                    const oco_athena_units* u, double* rho3d, double* T3d)
+{
+    if (!c || !rho_raw || !mx || !my || !mz || !e || !tgauss || !rho3d || !T3d) return OCO_E_ARG;
+    int st = athena_check(nz, nyloc, nxloc, nzl, iz0, u);
+    if (st) return st;
+    HIPCHK(hipSetDevice(c->device));
+    const size_t ns = (size_t)nxloc * nyloc * nzl, ncell = (size_t)nxloc * nyloc * nz;
+    if ((st = ensure(c, c->raw, 5 * ns * sizeof(double)))) return st;
+    if ((st = ensure(c, c->tg, (size_t)nz * sizeof(double)))) return st;
+    if ((st = ensure(c, c->rho, ncell * sizeof(double)))) return st;
+    if ((st = ensure(c, c->T, ncell * sizeof(double)))) return st;
+    double* d = (double*)c->raw.p;
+    const double* src[5] = {rho_raw, mx, my, mz, e};
+    for (int f = 0; f < 5; f++)
+        HIPCHK(hipMemcpyAsync(d + f * ns, src[f], ns * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(c->tg.p, tgauss, (size_t)nz * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    oco_athena_args a = athena_args(nz, nyloc, nxloc, nzl, iz0, u);
+    a.rho = d; a.mx = d + ns; a.my = d + 2 * ns; a.mz = d + 3 * ns; a.e = d + 4 * ns;
+    a.tgauss = (const double*)c->tg.p;
+    a.rho3d = (double*)c->rho.p;
+    a.T3d = (double*)c->T.p;
+    HIPCHK(oco_launch_athena(a, c->stream));
+    /* the slab's z range of every column back into the caller's arrays */
+    const size_t pitch = (size_t)nz * sizeof(double), width = (size_t)nzl * sizeof(double);
+    HIPCHK(hipMemcpy2DAsync(rho3d + iz0, pitch, (const double*)c->rho.p + iz0, pitch, width,
+                            (size_t)nxloc * nyloc, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpy2DAsync(T3d + iz0, pitch, (const double*)c->T.p + iz0, pitch, width, (size_t)nxloc * nyloc,
+                            hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return OCO_OK;
 }
 
 /* ---------------------------------------------------------------------- */

@@ -69,6 +69,18 @@ struct oco_serve_args {
     unsigned long long* fail;     /* per slot: first zero pivot key (FP64 mode) */
 };
 
+/* Athena slab to the block's rho, T (see athena_slab in oco_kernels.hip). */
+struct oco_athena_args {
+    int nz, nyloc, nxloc, nzl, iz0;
+    const double *rho, *mx, *my, *mz, *e;   /* slab fields, file order (nxloc, nyloc, nzl) */
+    const double* tgauss;                     /* (nz) */
+    double unit_density, unit_temperature, gamma1, rho_floor, switch_temp, floor_temp;
+    int avoid_momentum_underflow;
+    double* rho3d;                            /* (nz, nyloc, nxloc) */
+    double* T3d;
+};
+hipError_t oco_launch_athena(const oco_athena_args& a, hipStream_t stream);
+
 /* Launchers (enqueue on `stream`, no synchronisation). */
 hipError_t oco_launch_serve(const oco_serve_args& a, int mixed, int nblocks, hipStream_t stream);
 int oco_serve_grid(int mixed, int device);

@@ -1142,6 +1142,58 @@ hipError_t oco_launch_batch(const oco_batch_args& a, int mixed, int variant, int
 
 int oco_batch_variants(void) { return 3; }
 
+/* Athena input construction: one z slab (one id<iproc> file) of raw code-unit
+ * fields into the block's rho, T -- the arithmetic of
+ * source/read_athena.f90:163-205 (kinetic energy with the momentum-underflow
+ * guard of :213-228, internal energy per mass), :232-258 (density unit and
+ * floor, temperature unit) and the temperature rule of
+ * source/disk.f90:96-104 (below switchTemp: the Gaussian profile or the
+ * floor), in the reference's operation order.  One lane per output cell, so
+ * the Fortran-order (z fastest) stores are coalesced. */
+__global__ __launch_bounds__(256) void athena_slab(const oco_athena_args a)
+{
+    const long n = (long)a.nxloc * a.nyloc * a.nzl;
+    const double eps = 0x1.0p-52;   /* epsilon(0.0) under the reference's 8-byte reals */
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+        const int k = (int)(i % a.nzl);
+        const long r = i / a.nzl;
+        const int iy = (int)(r % a.nyloc);
+        const int ix = (int)(r / a.nyloc);
+        const long src = ((long)k * a.nyloc + iy) * a.nxloc + ix;
+        const double rho = a.rho[src];
+        double t = 0.0;
+        const double p3[3] = {a.mx[src], a.my[src], a.mz[src]};
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+            const double p = p3[c];
+            if (!a.avoid_momentum_underflow || __builtin_fabs(p) >= eps) t = t + (0.5 * (p * p)) / rho;
+        }
+        t = (a.e[src] - t) / rho;
+        double rc = rho * a.unit_density;
+        if (rc < a.rho_floor) rc = a.rho_floor;
+        double T = (t * a.gamma1) * a.unit_temperature;
+        const int z = a.iz0 + k;
+        if (T < a.switch_temp) {
+            const double tg = a.tgauss[z];
+            T = tg > a.floor_temp ? tg : a.floor_temp;
+        }
+        const long dst = ((long)ix * a.nyloc + iy) * a.nz + z;
+        a.rho3d[dst] = rc;
+        a.T3d[dst] = T;
+    }
+}
+
+hipError_t oco_launch_athena(const oco_athena_args& a, hipStream_t stream)
+{
+    (void)hipGetLastError();
+    const long n = (long)a.nxloc * a.nyloc * a.nzl;
+    if (n <= 0) return hipSuccess;
+    long blocks = (n + 255) / 256;
+    if (blocks > 65536) blocks = 65536;
+    hipLaunchKernelGGL(athena_slab, dim3((unsigned)blocks), dim3(256), 0, stream, a);
+    return hipGetLastError();
+}
+
 hipError_t oco_launch_cells(const double* rho, const double* T, long ncells, double fully_ionized_T,
                             double* cells, int soa_nz, hipStream_t stream)
 {

@@ -17,7 +17,14 @@ module oco_capi
     real(c_double) :: cells_ms, sweep_ms, total_ms
   end type oco_diag
 
+  ! oco_athena_units of include/ocotillo.h
+  type, bind(C), public :: oco_athena_units
+    real(c_double) :: unit_density, unit_temperature, gamma1, rho_floor, switch_temp, floor_temp
+    integer(c_int) :: avoid_momentum_underflow
+  end type oco_athena_units
+
   public :: oco_create, oco_destroy, oco_set_wavelengths, oco_set_physics, oco_solve_block, oco_message
+  public :: oco_athena_slab
 
   interface
     integer(c_int) function oco_create(device, hbm_budget, ctx) bind(C, name='oco_create')
@@ -57,6 +64,16 @@ module oco_capi
       real(c_double), intent(inout) :: u(*), v(*), absorp(*)
       type(oco_diag), intent(out) :: diag
     end function oco_solve_block
+
+    integer(c_int) function oco_athena_slab(ctx, nz, nyloc, nxloc, nzl, iz0, rho_raw, mx, my, mz, e, tgauss, &
+         units, rho3d, t3d) bind(C, name='oco_athena_slab')
+      import :: c_int, c_ptr, c_double, oco_athena_units
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: nz, nyloc, nxloc, nzl, iz0
+      real(c_double), intent(in) :: rho_raw(*), mx(*), my(*), mz(*), e(*), tgauss(*)
+      type(oco_athena_units), intent(in) :: units
+      real(c_double), intent(inout) :: rho3d(*), t3d(*)
+    end function oco_athena_slab
 
     type(c_ptr) function oco_strerror_c(status) bind(C, name='oco_strerror')
       import :: c_int, c_ptr

@@ -239,22 +239,31 @@ contains
     enddo
   end subroutine calc_grid
 
-  ! T rule; the Gaussian profile is kept from the first call
-  ! (source/disk.f90:64-109).
-  subroutine calc_temperature(T, z, lathena)
-    real(8), intent(inout) :: T(nz)
+  ! The Gaussian temperature profile, built on the first call and kept
+  ! (source/disk.f90:87-90).
+  subroutine gaussian_profile(z, tg)
     real(8), intent(in) :: z(mz)
-    logical, intent(in) :: lathena
-    integer :: i
-    if (.not. lathena) T = midTemp
+    real(8), intent(out) :: tg(nz)
     if (.not. allocated(Tgauss)) then
       allocate(Tgauss(nz))
       Tgauss = midTemp * exp(-0.5d0 * z(n1:n2)**2 / sigma**2)
     endif
+    tg = Tgauss
+  end subroutine gaussian_profile
+
+  ! T rule of the analytic disk (source/disk.f90:64-109).
+  subroutine calc_temperature(T, z, lathena)
+    real(8), intent(inout) :: T(nz)
+    real(8), intent(in) :: z(mz)
+    logical, intent(in) :: lathena
+    real(8) :: tg(nz)
+    integer :: i
+    if (.not. lathena) T = midTemp
+    call gaussian_profile(z, tg)
     do i = 1, nz
       if (T(i) < switchTemp) then
-        if (Tgauss(i) > floorTemp) then
-          T(i) = Tgauss(i)
+        if (tg(i) > floorTemp) then
+          T(i) = tg(i)
         else
           T(i) = floorTemp
         endif
@@ -276,12 +285,16 @@ contains
 end module oco_disk
 
 !------------------------------------------------------------------------------
-! Athena-4.2 .bin reader (file format and unit conversion of
-! source/read_athena.f90:33-291).
+! Athena-4.2 .bin reader (file format and units of source/read_athena.f90:33-291).
+! The host reads the files and sets up the z grid, the units and the Gaussian
+! profile; the field arithmetic (kinetic and internal energy, unit
+! conversion, density floor, temperature rule) runs on the device, one
+! oco_athena_slab call per z slab.
 module oco_athena
   use oco_shapes
-  use oco_disk, only: calc_temperature
-  use, intrinsic :: iso_c_binding, only: c_int, c_double
+  use oco_disk, only: gaussian_profile, switchTemp, floorTemp
+  use oco_capi, only: oco_athena_units, oco_athena_slab, oco_message, OCO_OK
+  use, intrinsic :: iso_c_binding, only: c_int, c_double, c_ptr
   use, intrinsic :: ieee_arithmetic
   implicit none
   character(len=90) :: RunName = ' ', datadir = "./input_athena/distributed"
@@ -317,114 +330,100 @@ contains
     print*, 'unit_temperature = ', unit_temperature, ' K'
   end subroutine read_athena_input
 
-  subroutine read_from_athena(z, dz, rho, temp, iprocx, iprocy, snapshot)
+  function slab_file(iprocx, iprocy, iprocz, snapshot) result(filename)
+    integer, intent(in) :: iprocx, iprocy, iprocz
+    character(len=*), intent(in) :: snapshot
+    character(len=256) :: filename, head, sproc
+    integer :: iproc
+    iproc = modulo(iprocz, nprocz) * nprocx * nprocy + modulo(iprocy, nprocy) * nprocx + modulo(iprocx, nprocx)
+    sproc = itoa(iproc)
+    head = trim(datadir) // "/id" // trim(sproc) // "/" // trim(RunName)
+    if (iproc == 0) then
+      filename = trim(head) // "." // trim(snapshot) // ".bin"
+    else
+      filename = trim(head) // "-id" // trim(sproc) // "." // trim(snapshot) // ".bin"
+    endif
+    if (.not. file_exists(filename)) then
+      print*, "iproc=", iproc
+      print*, "File ", trim(filename), " does not exist"
+      stop 1
+    endif
+  end function slab_file
+
+  ! Header of one slab file up to and including the z coordinates
+  ! (source/read_athena.f90:123-150); leaves the unit positioned at the fields.
+  subroutine read_header(u, nzl, gamma1, zloc)
+    integer, intent(in) :: u
+    integer(c_int), intent(out) :: nzl
+    real(c_double), intent(out) :: gamma1
+    real(c_double), allocatable, intent(out) :: zloc(:)
+    integer(c_int) :: hdr(4), nxl, nyl, coordsys
+    real(c_double) :: misc(3)
+    real(c_double), allocatable :: xloc(:), yloc(:)
+    read(u) coordsys
+    read(u) nxl
+    if (nxl /= nxloc) then
+      print*, "Resolution in x from Athena = ", nxl
+      print*, "RT x-dimensionality from resolution.in = ", nxloc
+      print*, "The Athena x dimensionality does not match the chosen for the RT post-processing. Fix."
+      stop 1
+    endif
+    read(u) nyl
+    if (nyl /= nyloc) then
+      print*, "Resolution in y from Athena = ", nyl
+      print*, "RT y-dimensionality from resolution.in = ", nyloc
+      print*, "The Athena y dimensionality does not match the chosen for the RT post-processing. Fix."
+      stop 1
+    endif
+    read(u) nzl
+    read(u) hdr
+    read(u) gamma1
+    read(u) misc
+    allocate(xloc(nxloc), yloc(nyloc), zloc(nzl))
+    read(u) xloc
+    read(u) yloc
+    read(u) zloc
+  end subroutine read_header
+
+  subroutine read_from_athena(ctx, z, dz, rho, temp, iprocx, iprocy, snapshot)
+    type(c_ptr), intent(in) :: ctx
     real(8), intent(out) :: z(mz), dz, rho(nz, nyloc, nxloc), temp(nz, nyloc, nxloc)
     integer, intent(in) :: iprocx, iprocy
     character(len=*), intent(in) :: snapshot
-    integer(c_int) :: hdr(8), nxl, nyl, nzl, coordsys
-    real(c_double) :: gamma1, misc(3)
-    real(c_double), allocatable :: xloc(:), yloc(:), zloc(:), tmp(:,:,:)
-    real(8) :: zn(nz), z0
-    integer :: iprocz, iproc, u, iz0, iz1, ix, iy, k, comp
-    character(len=256) :: filename, head, sproc
+    integer(c_int) :: nzl, st
+    real(c_double) :: gamma1
+    real(c_double), allocatable :: zloc(:), f(:,:,:,:)
+    real(8) :: zn(nz), z0, tg(nz)
+    integer :: iprocz, u, iz0(0:nprocz - 1), nzls(0:nprocz - 1), i
+    type(oco_athena_units) :: units
 
+    ! pass 1: headers -> the block's z grid and each slab's place in it
     z0 = 0; dz = 0
     do iprocz = 0, nprocz - 1
-      iproc = modulo(iprocz, nprocz) * nprocx * nprocy + modulo(iprocy, nprocy) * nprocx + modulo(iprocx, nprocx)
-      sproc = itoa(iproc)
-      head = trim(datadir) // "/id" // trim(sproc) // "/" // trim(RunName)
-      if (iproc == 0) then
-        filename = trim(head) // "." // trim(snapshot) // ".bin"
-      else
-        filename = trim(head) // "-id" // trim(sproc) // "." // trim(snapshot) // ".bin"
-      endif
-      if (.not. file_exists(filename)) then
-        print*, "iproc=", iproc
-        print*, "File ", trim(filename), " does not exist"
-        stop 1
-      endif
-      open(newunit=u, file=trim(filename), form='unformatted', access='stream', status='old', action='read')
-      read(u) coordsys
-      read(u) nxl
-      if (nxl /= nxloc) then
-        print*, "Resolution in x from Athena = ", nxl
-        print*, "RT x-dimensionality from resolution.in = ", nxloc
-        print*, "The Athena x dimensionality does not match the chosen for the RT post-processing. Fix."
-        stop 1
-      endif
-      read(u) nyl
-      if (nyl /= nyloc) then
-        print*, "Resolution in y from Athena = ", nyl
-        print*, "RT y-dimensionality from resolution.in = ", nyloc
-        print*, "The Athena y dimensionality does not match the chosen for the RT post-processing. Fix."
-        stop 1
-      endif
-      read(u) nzl
-      read(u) hdr(1:4)
-      read(u) gamma1
-      read(u) misc
-      allocate(xloc(nxloc), yloc(nyloc), zloc(nzl), tmp(nxloc, nyloc, nzl))
-      read(u) xloc
-      read(u) yloc
-      read(u) zloc
+      open(newunit=u, file=trim(slab_file(iprocx, iprocy, iprocz, snapshot)), form='unformatted', &
+           access='stream', status='old', action='read')
+      call read_header(u, nzl, gamma1, zloc)
+      close(u)
       if (iprocz == 0) then
         z0 = zloc(1)
         dz = zloc(2) - zloc(1)
       endif
-      iz0 = nint((zloc(1) - z0) / dz) + 1
-      iz1 = iz0 + nzl - 1
-      zn(iz0:iz1) = zloc
-      read(u) tmp
-      do ix = 1, nxloc
-        do iy = 1, nyloc
-          do k = 1, nzl
-            rho(iz0 + k - 1, iy, ix) = tmp(ix, iy, k)
-          enddo
-        enddo
-      enddo
-      temp(iz0:iz1, :, :) = 0
-      ! kinetic energy (rux**2+ruy**2+ruz**2)/(2 rho)
-      do comp = 1, 3
-        read(u) tmp
-        do ix = 1, nxloc
-          do iy = 1, nyloc
-            do k = 1, nzl
-              if (.not. lavoid_momentum_underflow .or. abs(tmp(ix, iy, k)) >= epsilon(0d0)) &
-                   temp(iz0 + k - 1, iy, ix) = temp(iz0 + k - 1, iy, ix) + &
-                   0.5d0 * tmp(ix, iy, k)**2 / rho(iz0 + k - 1, iy, ix)
-            enddo
-          enddo
-        enddo
-      enddo
-      ! internal energy per mass: (E - ekin)/rho
-      read(u) tmp
-      do ix = 1, nxloc
-        do iy = 1, nyloc
-          do k = 1, nzl
-            temp(iz0 + k - 1, iy, ix) = (tmp(ix, iy, k) - temp(iz0 + k - 1, iy, ix)) / rho(iz0 + k - 1, iy, ix)
-          enddo
-        enddo
-      enddo
-      deallocate(xloc, yloc, zloc, tmp)
-      close(u)
+      iz0(iprocz) = nint((zloc(1) - z0) / dz) + 1
+      nzls(iprocz) = nzl
+      if (iz0(iprocz) < 1 .or. iz0(iprocz) + nzl - 1 > nz) then
+        print*, "Athena slab", iprocz, " does not fit nz =", nz
+        stop 1
+      endif
+      zn(iz0(iprocz):iz0(iprocz) + nzl - 1) = zloc
     enddo
-    call postprocess(rho, temp, zn, z, dz, gamma1)
-  end subroutine read_from_athena
-
-  subroutine postprocess(rho, temp, zn, z, dz, gamma1)
-    real(8), intent(inout) :: rho(nz, nyloc, nxloc), temp(nz, nyloc, nxloc), zn(nz)
-    real(8), intent(out) :: z(mz), dz
-    real(8), intent(in) :: gamma1
-    integer :: i, ix, iy
-    rho = rho * unit_density
+    ! postprocess_athena_values (source/read_athena.f90:232-268): the grid
+    ! and the scalars here, the fields on the device
     if (ieee_is_nan(rho_floor)) then
       print*, ""
       print*, "Must set `rho_floor` (cgs) value in namelist: `input.in/athena_input`."
       print*, ""
       stop 1
     endif
-    where (rho < rho_floor) rho = rho_floor
-    temp = temp * gamma1 * unit_temperature
     zn = zn * unit_length
     dz = (zn(nz) - zn(1)) / (nz - 1)
     z(n1:n2) = zn
@@ -432,16 +431,34 @@ contains
       z(n1 - i) = z(n1) - i * dz
       z(n2 + i) = z(n2) + i * dz
     enddo
-    do ix = 1, nxloc
-      do iy = 1, nyloc
-        call calc_temperature(temp(:, iy, ix), z, .true.)
+    call gaussian_profile(z, tg)
+    ! gamma1 is the last slab's, as in the reference
+    units = oco_athena_units(unit_density, unit_temperature, gamma1, rho_floor, switchTemp, floorTemp, &
+         merge(1_c_int, 0_c_int, lavoid_momentum_underflow))
+    ! pass 2: the fields of each slab, converted on the device
+    do iprocz = 0, nprocz - 1
+      open(newunit=u, file=trim(slab_file(iprocx, iprocy, iprocz, snapshot)), form='unformatted', &
+           access='stream', status='old', action='read')
+      call read_header(u, nzl, gamma1, zloc)
+      allocate(f(nxloc, nyloc, nzl, 5))
+      do i = 1, 5
+        read(u) f(:, :, :, i)
       enddo
+      close(u)
+      st = oco_athena_slab(ctx, int(nz, c_int), int(nyloc, c_int), int(nxloc, c_int), nzl, &
+           int(iz0(iprocz) - 1, c_int), f(:, :, :, 1), f(:, :, :, 2), f(:, :, :, 3), f(:, :, :, 4), &
+           f(:, :, :, 5), tg, units, rho, temp)
+      deallocate(f)
+      if (st /= OCO_OK) then
+        print*, "ocotillo GPU library: ", oco_message(st)
+        stop 2
+      endif
     enddo
     if (lroot) then
       print*, 'minval(rho),maxval(rho),mean(rho)', minval(rho), maxval(rho), sum(rho) / (nxloc * nyloc * nz)
       print*, 'minval(T),maxval(T),mean(T)', minval(temp), maxval(temp), sum(temp) / (nxloc * nyloc * nz)
     endif
-  end subroutine postprocess
+  end subroutine read_from_athena
 end module oco_athena
 
 !------------------------------------------------------------------------------
@@ -615,7 +632,7 @@ program flux_feautrier_gpu
     do iprocy = 0, nprocy - 1
       lroot = (iprocx == 0) .and. (iprocy == 0)
       if (lread_athena) then
-        call read_from_athena(z, dz, rho3d, temp3d, iprocx, iprocy, snapshot)
+        call read_from_athena(ctx, z, dz, rho3d, temp3d, iprocx, iprocy, snapshot)
       else
         if (lroot) call calc_grid(z1, z0, z, dz)
         call calc_density(rho, z)

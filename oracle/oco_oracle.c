@@ -539,3 +539,85 @@ void oo_calc_temperature(int nz, const double* z, double midTemp, double sigma, 
         if (T[i] < switchTemp) T[i] = (tg > floorTemp) ? tg : floorTemp;
     }
 }
+
+/* ---------------------------------------------------------------------- */
+/* Athena snapshot input                                                   */
+/* ---------------------------------------------------------------------- */
+
+/* source/common.f90:33, 76-77 (8-byte reals); k1_cgs and c_light_cgs are
+ * OO_K1_CGS and OO_C_LIGHT above. */
+#define OO_AMU 1.66053906660e-24
+#define OO_G_NEWTON 6.6742e-8
+#define OO_SOLAR_MASS 1.988409870698051e+33
+
+/* read_athena_input, source/read_athena.f90:42-56. */
+void oo_athena_units(double Mbh_SolarMasses, double r0ref_rg, double aspect_ratio, double mean_molecular_weight,
+                     double rho0, double* unit_length, double* unit_density, double* unit_temperature)
+{
+    double Mbh = Mbh_SolarMasses * OO_SOLAR_MASS;
+    double rg = (2.0 * (OO_G_NEWTON / (OO_C_LIGHT * OO_C_LIGHT))) * Mbh;
+    double rr = r0ref_rg * rg;
+    double g0 = OO_G_NEWTON * Mbh;
+    double Omega = sqrt(g0) * pow(rr, -1.5);
+    double H = aspect_ratio * rr;
+    double unit_time = 1.0 / Omega;
+    double uv = H / unit_time;
+    *unit_length = H;
+    *unit_density = rho0;
+    *unit_temperature = ((mean_molecular_weight * OO_AMU) * OO_K1_CGS) * (uv * uv);
+}
+
+/* One slab of read_from_athena, source/read_athena.f90:163-205: the
+ * transpose to (z, y, x), temp = 0, the three kinetic-energy passes (each
+ * guarded by fix_momentum_underflow, :213-228, when asked) and the
+ * internal-energy pass, as separate sweeps over the slab like the
+ * reference's array statements. */
+void oo_athena_slab_raw(int nz, int nyloc, int nxloc, int nzl, int iz0, const double* f_rho, const double* f_mx,
+                        const double* f_my, const double* f_mz, const double* f_e, int avoid, double* rho,
+                        double* temp)
+{
+#define OO_SRC(ix, iy, k) ((((size_t)(k) * nyloc) + (iy)) * nxloc + (ix))
+#define OO_DST(ix, iy, z) ((((size_t)(ix) * nyloc) + (iy)) * nz + (z))
+    const double eps = 2.220446049250313080847263336181640625e-16;   /* epsilon(0.0), 8-byte reals */
+    for (int ix = 0; ix < nxloc; ix++)
+        for (int iy = 0; iy < nyloc; iy++)
+            for (int k = 0; k < nzl; k++) {
+                rho[OO_DST(ix, iy, iz0 + k)] = f_rho[OO_SRC(ix, iy, k)];
+                temp[OO_DST(ix, iy, iz0 + k)] = 0.0;
+            }
+    const double* mom[3] = {f_mx, f_my, f_mz};
+    for (int c = 0; c < 3; c++)
+        for (int ix = 0; ix < nxloc; ix++)
+            for (int iy = 0; iy < nyloc; iy++)
+                for (int k = 0; k < nzl; k++) {
+                    double p = mom[c][OO_SRC(ix, iy, k)];
+                    size_t d = OO_DST(ix, iy, iz0 + k);
+                    if (!avoid || fabs(p) >= eps) temp[d] = temp[d] + (0.5 * (p * p)) / rho[d];
+                }
+    for (int ix = 0; ix < nxloc; ix++)
+        for (int iy = 0; iy < nyloc; iy++)
+            for (int k = 0; k < nzl; k++) {
+                size_t d = OO_DST(ix, iy, iz0 + k);
+                temp[d] = (f_e[OO_SRC(ix, iy, k)] - temp[d]) / rho[d];
+            }
+#undef OO_SRC
+#undef OO_DST
+}
+
+/* postprocess_athena_values, source/read_athena.f90:232-268, on the whole
+ * block; the temperature rule is calc_temperature's (source/disk.f90:96-104). */
+void oo_athena_postprocess(int nz, int nyloc, int nxloc, double unit_density, double rho_floor, double gamma1,
+                           double unit_temperature, const double* tgauss, double switchTemp, double floorTemp,
+                           double* rho, double* temp)
+{
+    const size_t n = (size_t)nz * nyloc * nxloc;
+    for (size_t i = 0; i < n; i++) rho[i] = rho[i] * unit_density;
+    for (size_t i = 0; i < n; i++)
+        if (rho[i] < rho_floor) rho[i] = rho_floor;
+    for (size_t i = 0; i < n; i++) temp[i] = (temp[i] * gamma1) * unit_temperature;
+    for (size_t col = 0; col < (size_t)nyloc * nxloc; col++)
+        for (int z = 0; z < nz; z++) {
+            double* t = temp + col * nz + z;
+            if (*t < switchTemp) *t = (tgauss[z] > floorTemp) ? tgauss[z] : floorTemp;
+        }
+}

@@ -102,6 +102,24 @@ void oo_calc_density(int nz, const double* z, double rho0, double H, double rho_
 void oo_calc_temperature(int nz, const double* z, double midTemp, double sigma, double floorTemp,
                          double switchTemp, const double* Tgauss_or_null, int from_athena, double* T);
 
+/* Athena snapshot input: source/read_athena.f90:33-268.
+ * oo_athena_units: read_athena_input's code units (:42-56) -> unit_length,
+ * unit_density, unit_temperature.
+ * oo_athena_slab_raw: one slab file's fields (file order, (nxloc, nyloc,
+ * nzl)) into the block's code-unit rho, temp (nz, nyloc, nxloc) at z rows
+ * iz0..iz0+nzl-1 (0-based), :163-205 and :213-228.
+ * oo_athena_postprocess: the whole block's unit conversion, density floor and
+ * temperature rule, :232-268 with source/disk.f90:96-104 (tgauss = the
+ * Gaussian profile of the first calc_temperature call). */
+void oo_athena_units(double Mbh_SolarMasses, double r0ref_rg, double aspect_ratio, double mean_molecular_weight,
+                     double rho0, double* unit_length, double* unit_density, double* unit_temperature);
+void oo_athena_slab_raw(int nz, int nyloc, int nxloc, int nzl, int iz0, const double* f_rho, const double* f_mx,
+                        const double* f_my, const double* f_mz, const double* f_e, int avoid_momentum_underflow,
+                        double* rho, double* temp);
+void oo_athena_postprocess(int nz, int nyloc, int nxloc, double unit_density, double rho_floor, double gamma1,
+                           double unit_temperature, const double* tgauss, double switchTemp, double floorTemp,
+                           double* rho, double* temp);
+
 #ifdef __cplusplus
 }
 #endif
