@@ -103,11 +103,12 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--config", default="c4", choices=["c4", "c2", "c3", "c5"],
+    ap.add_argument("--config", default="c4", choices=["c4", "c2", "c3", "c5", "athena"],
                     help="c4: the 256x256x512 x 128-wavelength mixed-precision snapshot (default); "
                          "c2: the 64x64x256 x 128-wavelength FP64 drop-in block; "
                          "c3: 4096 uneven columns through the batched persistent path; "
-                         "c5: the serving loop under Poisson arrivals")
+                         "c5: the serving loop under Poisson arrivals; "
+                         "athena: Athena input construction of the c4 block only")
     ap.add_argument("--mode", default=None, choices=["both", "fp64", "mixed"],
                     help="precision modes to time (default: both for c4 and c3, fp64 for c2)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
@@ -141,6 +142,8 @@ def main():
         return
     if args.config == "c3":
         return bench_batch(args)
+    if args.config == "athena":
+        return bench_athena_only(args)
     if args.config == "c5":
         res = bench_serve(args.jobs or 300, args.pool or 12)
         out = {"metric": METRIC, "value": res["p99_ms"], "unit": "ms (p99 latency)", "n_gpus": 1,
@@ -235,6 +238,7 @@ def bench_block(args):
         res[mode] = {"value": points / (ms * 1e-3), "ms_per_step": ms, "roofline": roof, "roofline_hbm": roof_hbm}
     del d_U, d_V, d_A
     torch.cuda.empty_cache()
+    athena = bench_athena(ctx, d_rho, d_T, args.steps)
 
     accuracy = None
     if len(modes) == 2 and not args.no_accuracy:
@@ -268,8 +272,79 @@ def bench_block(args):
         out["accuracy"] = accuracy
     if serving is not None:
         out["serving"] = serving
+    out["input_construction"] = athena
     print(json.dumps(out), flush=True)
     ctx.close()
+
+
+def bench_athena_only(args):
+    import torch
+    import ocotillo_amd as oco
+    from ocotillo_amd import workloads
+    dims = dict(nx=args.nx or 256, ny=args.ny or 256, nz=args.nz or 512, nw=1)
+    rho, T, dz, waves = workloads.config4(**dims)
+    d_rho = torch.from_numpy(rho).to("cuda:0")
+    d_T = torch.from_numpy(T).to("cuda:0")
+    torch.cuda.synchronize()
+    ctx = oco.Context(0)
+    r = bench_athena(ctx, d_rho, d_T, args.steps)
+    ncell = r["cells"]
+    out = {"metric": "Athena input construction (raw code-unit fields -> block rho, T)",
+           "value": round(ncell / (r["ms_per_call"] * 1e-3), 1), "unit": "cells/s", "n_gpus": 1,
+           "steps": max(args.steps, 10), "warmup": 1, "ms_per_step": r["ms_per_call"], "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+           "config": {"workload": "one %dx%dx%d Athena slab (the c4 block's cells)" % (dims["nx"], dims["ny"],
+                                                                                     dims["nz"])},
+           "roofline": {k: r[k] for k in ("bound", "achieved", "peak", "unit", "frac")}, "input_construction": r,
+           "cpu_baseline": None}
+    print(json.dumps(out), flush=True)
+    ctx.close()
+
+
+def bench_athena(ctx, d_rho, d_T, reps):
+    """Athena input construction of the same block (oco_athena_slab_device,
+    the whole block as one z slab): raw code-unit fields in file order
+    (x fastest) made from the workload's rho and T with random momenta, to
+    the block's rho, T.  56 algorithmic bytes per cell (5 fields read, 2
+    written); the call synchronises, so the rate is wall time per call."""
+    import torch
+    import ocotillo_amd as oco
+    from ocotillo_amd.athena import code_units
+    nx, ny, nz = d_rho.shape
+    ul, ud, ut = code_units(1e8, 1e3, 0.005, 1.0079764, 1e-9)
+    gamma1 = 0.4
+    g = torch.Generator(device=d_rho.device).manual_seed(11)
+    rho_c = (d_rho / ud).permute(2, 1, 0).contiguous()                   # [nz, ny, nx]
+    mom = [torch.randn(rho_c.shape, generator=g, device=d_rho.device, dtype=torch.float64) * rho_c * 0.1
+           for _ in range(3)]
+    eint = (d_T / (gamma1 * ut)).permute(2, 1, 0).contiguous()
+    e = sum(0.5 * m * m / rho_c for m in mom) + rho_c * eint
+    fields = [rho_c] + mom + [e]
+    tg = torch.full((nz,), 2000.0, dtype=torch.float64, device=d_rho.device)
+    out_rho = torch.empty_like(d_rho)
+    out_T = torch.empty_like(d_T)
+    torch.cuda.synchronize()
+    units = oco.AthenaUnits(ud, ut, gamma1, 1e-15, (1 + 2.0 ** -52) * 64857.04298198191, 2000.0, 1)
+
+    def call():
+        ctx.athena_slab_device(nz, ny, nx, nz, 0, [f.data_ptr() for f in fields], tg.data_ptr(), units,
+                               out_rho.data_ptr(), out_T.data_ptr())
+
+    call()
+    reps = max(reps, 10)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        call()
+    ms = (time.perf_counter() - t0) * 1e3 / reps
+    ncell = nx * ny * nz
+    gbs = 56.0 * ncell / (ms * 1e-3) / 1e9
+    close = torch.allclose(out_rho, d_rho, rtol=1e-12, atol=0)
+    del fields, mom, rho_c, eint, e, out_rho, out_T
+    torch.cuda.empty_cache()
+    log("athena_slab: %.3f ms per call, %.0f GB/s" % (ms, gbs))
+    return {"kernel": "athena_slab", "cells": ncell, "ms_per_call": round(ms, 4), "bound": "hbm",
+            "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4),
+            "bytes_per_cell": 56, "timing": "wall per synchronous call", "rho_roundtrip_1e-12": bool(close)}
 
 
 def accuracy_pass(ctx, d_rho, d_T, dz, nx, ny, nz, nw, nchunks=4):

@@ -1148,49 +1148,100 @@ int oco_batch_variants(void) { return 3; }
  * guard of :213-228, internal energy per mass), :232-258 (density unit and
  * floor, temperature unit) and the temperature rule of
  * source/disk.f90:96-104 (below switchTemp: the Gaussian profile or the
- * floor), in the reference's operation order.  One lane per output cell, so
- * the Fortran-order (z fastest) stores are coalesced. */
-__global__ __launch_bounds__(256) void athena_slab(const oco_athena_args a)
+ * floor), in the reference's operation order.
+ *
+ * The file holds x fastest, the block z fastest, so the kernel is a
+ * transpose with arithmetic in it.  A workgroup takes a tile of ATH_TX x
+ * cells by ATH_TK z cells of one y row: it reads the five fields along x
+ * (coalesced), computes rho and T per cell in registers, parks the two
+ * results in LDS (rows padded by one double, so the column-wise reads hit
+ * distinct banks) and writes them along z (coalesced).  Every cell is read
+ * once and written once: 56 bytes per cell. */
+#ifndef OCO_ATH_TX
+#define OCO_ATH_TX 32
+#endif
+#ifndef OCO_ATH_TK
+#define OCO_ATH_TK 64
+#endif
+constexpr int ATH_TX = OCO_ATH_TX, ATH_TK = OCO_ATH_TK, ATH_THREADS = 256;
+static_assert(ATH_THREADS % ATH_TX == 0 && ATH_THREADS % ATH_TK == 0, "athena tile must divide the workgroup");
+
+__global__ __launch_bounds__(ATH_THREADS) void athena_slab(const oco_athena_args a)
 {
-    const long n = (long)a.nxloc * a.nyloc * a.nzl;
+    __shared__ double s_rho[ATH_TK][ATH_TX + 1];
+    __shared__ double s_T[ATH_TK][ATH_TX + 1];
     const double eps = 0x1.0p-52;   /* epsilon(0.0) under the reference's 8-byte reals */
-    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
-        const int k = (int)(i % a.nzl);
-        const long r = i / a.nzl;
-        const int iy = (int)(r % a.nyloc);
-        const int ix = (int)(r / a.nyloc);
-        const long src = ((long)k * a.nyloc + iy) * a.nxloc + ix;
-        const double rho = a.rho[src];
-        double t = 0.0;
-        const double p3[3] = {a.mx[src], a.my[src], a.mz[src]};
+    const int ntx = (a.nxloc + ATH_TX - 1) / ATH_TX, ntk = (a.nzl + ATH_TK - 1) / ATH_TK;
+    const long ntiles = (long)ntx * ntk * a.nyloc;
+    const long plane = (long)a.nyloc * a.nxloc;
+    for (long t = blockIdx.x; t < ntiles; t += gridDim.x) {
+        const int tx = (int)(t % ntx);
+        const long r = t / ntx;
+        const int tk = (int)(r % ntk);
+        const int iy = (int)(r / ntk);
+        const int ix0 = tx * ATH_TX, k0 = tk * ATH_TK;
+        {   /* read phase: lanes along x; every load of the thread's rows is
+               issued before the arithmetic (the divides) of the first */
+            constexpr int STEP = ATH_THREADS / ATH_TX, ROWS = ATH_TK / STEP;
+            const int lx = threadIdx.x % ATH_TX, ix = ix0 + lx, kk0 = threadIdx.x / ATH_TX;
+            double f[ROWS][5];
+            bool ok[ROWS];
 #pragma unroll
-        for (int c = 0; c < 3; c++) {
-            const double p = p3[c];
-            if (!a.avoid_momentum_underflow || __builtin_fabs(p) >= eps) t = t + (0.5 * (p * p)) / rho;
+            for (int j = 0; j < ROWS; j++) {
+                const int k = k0 + kk0 + j * STEP;
+                ok[j] = ix < a.nxloc && k < a.nzl;
+                const long src = ok[j] ? (long)k * plane + (long)iy * a.nxloc + ix : 0;
+                f[j][0] = ok[j] ? a.rho[src] : 1.0;
+                f[j][1] = ok[j] ? a.mx[src] : 0.0;
+                f[j][2] = ok[j] ? a.my[src] : 0.0;
+                f[j][3] = ok[j] ? a.mz[src] : 0.0;
+                f[j][4] = ok[j] ? a.e[src] : 0.0;
+            }
+#pragma unroll
+            for (int j = 0; j < ROWS; j++) {
+                if (!ok[j]) continue;
+                const int kk = kk0 + j * STEP;
+                const double rho = f[j][0];
+                double e = 0.0;
+#pragma unroll
+                for (int c = 1; c <= 3; c++) {
+                    const double p = f[j][c];
+                    if (!a.avoid_momentum_underflow || __builtin_fabs(p) >= eps) e = e + (0.5 * (p * p)) / rho;
+                }
+                e = (f[j][4] - e) / rho;
+                double rc = rho * a.unit_density;
+                if (rc < a.rho_floor) rc = a.rho_floor;
+                double T = (e * a.gamma1) * a.unit_temperature;
+                if (T < a.switch_temp) {
+                    const double tg = a.tgauss[a.iz0 + k0 + kk];
+                    T = tg > a.floor_temp ? tg : a.floor_temp;
+                }
+                s_rho[kk][lx] = rc;
+                s_T[kk][lx] = T;
+            }
         }
-        t = (a.e[src] - t) / rho;
-        double rc = rho * a.unit_density;
-        if (rc < a.rho_floor) rc = a.rho_floor;
-        double T = (t * a.gamma1) * a.unit_temperature;
-        const int z = a.iz0 + k;
-        if (T < a.switch_temp) {
-            const double tg = a.tgauss[z];
-            T = tg > a.floor_temp ? tg : a.floor_temp;
+        __syncthreads();
+        {   /* write phase: lanes along z */
+            const int wk = threadIdx.x % ATH_TK, k = k0 + wk;
+            for (int xx = threadIdx.x / ATH_TK; xx < ATH_TX; xx += ATH_THREADS / ATH_TK) {
+                const int ix = ix0 + xx;
+                if (ix >= a.nxloc || k >= a.nzl) continue;
+                const long dst = ((long)ix * a.nyloc + iy) * a.nz + a.iz0 + k;
+                a.rho3d[dst] = s_rho[wk][xx];
+                a.T3d[dst] = s_T[wk][xx];
+            }
         }
-        const long dst = ((long)ix * a.nyloc + iy) * a.nz + z;
-        a.rho3d[dst] = rc;
-        a.T3d[dst] = T;
+        __syncthreads();
     }
 }
 
 hipError_t oco_launch_athena(const oco_athena_args& a, hipStream_t stream)
 {
     (void)hipGetLastError();
-    const long n = (long)a.nxloc * a.nyloc * a.nzl;
-    if (n <= 0) return hipSuccess;
-    long blocks = (n + 255) / 256;
-    if (blocks > 65536) blocks = 65536;
-    hipLaunchKernelGGL(athena_slab, dim3((unsigned)blocks), dim3(256), 0, stream, a);
+    const long ntiles = (long)((a.nxloc + ATH_TX - 1) / ATH_TX) * ((a.nzl + ATH_TK - 1) / ATH_TK) * a.nyloc;
+    if (ntiles <= 0) return hipSuccess;
+    const long blocks = ntiles < 65536 ? ntiles : 65536;
+    hipLaunchKernelGGL(athena_slab, dim3((unsigned)blocks), dim3(ATH_THREADS), 0, stream, a);
     return hipGetLastError();
 }
 
