@@ -33,7 +33,8 @@ enum {
     OCO_E_HIP = 5,
     OCO_E_STATE = 6,
     OCO_E_NODEVICE = 7,
-    OCO_E_BUSY = 8
+    OCO_E_BUSY = 8,
+    OCO_E_IO = 9
 };
 
 /* Precision modes of oco_solve_block*. */
@@ -93,6 +94,35 @@ int oco_solve_block(oco_ctx* ctx, int nz, int nyloc, int nxloc, double dz, doubl
 int oco_solve_block_device(oco_ctx* ctx, int nz, int nyloc, int nxloc, double dz, double dz1,
                            double dz2, const double* d_rho3d, const double* d_T3d, double* d_U,
                            double* d_V, double* d_absorp, int mode, oco_diag* diag);
+
+/* ---- Block outputs ----
+ * One block solved and written as the reference's three output files
+ * (source/io.f90:76-101, output_binary):
+ *   <dir>/mean_intensity_<snapshot>.bin           U       (nz+6, nyloc, nxloc, nw)
+ *   <dir>/absorption_coefficients_<snapshot>.bin  absorp  (nz, nyloc, nxloc, nw)
+ *   <dir>/flux_<snapshot>.bin                     V       (nz, nyloc, nxloc, nw)
+ * each one Fortran unformatted sequential record in the layout of the
+ * reference's compiler, gfortran (records over 2147483639 bytes as
+ * subrecords with negated markers).  Replaces the column loop plus
+ * output_binary (source/flux_feautrier.f90:160-228, 243) without holding
+ * the outputs in host memory: the block is solved in wavelength chunks of
+ * about `stage_bytes` (0: 512 MiB), and the solve of chunk k+1, the
+ * device-to-host copy of chunk k and the file writes of chunk k-1 overlap.
+ * The files appear under their names only when complete; on a zero pivot
+ * (the reference stops before output_binary) none is written.  `dir` must
+ * exist.  U_col (nz+6, nw) and A_col (nz, nw), may be NULL, receive the
+ * column (ix, iy) = (nxloc, nyloc) the reference prints to diagnostics.txt
+ * (source/flux_feautrier.f90:241).  diag as for oco_solve_block, except
+ * sweep_ms (device time from the first chunk's solve to the last's) and
+ * total_ms (host wall time of the whole call, files closed). */
+int oco_solve_block_files(oco_ctx* ctx, int nz, int nyloc, int nxloc, double dz, double dz1, double dz2,
+                          const double* rho3d, const double* T3d, int mode, const char* dir, const char* snapshot,
+                          size_t stage_bytes, double* U_col, double* A_col, oco_diag* diag);
+
+/* One array as a one-record Fortran unformatted sequential file, the
+ * layout above (what `write(u) array` gives under gfortran,
+ * source/io.f90:57-61, 71-73, 88-99).  No GPU involved. */
+int oco_write_record(const char* path, const void* data, size_t nbytes);
 
 /* ---- Athena input construction ----
  * The arithmetic that turns one Athena z slab (the file id<iproc>.<snap>.bin

@@ -23,12 +23,13 @@ __all__ = ["OcoError", "Context", "Batch", "Server", "Diag", "AthenaUnits", "loa
 PKG = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(PKG, "libocotillo.so")
 
-OK, E_PIVOT0_STAGE1, E_PIVOT0_STAGE2, E_ARG, E_OOM, E_HIP, E_STATE, E_NODEVICE, E_BUSY = range(9)
+OK, E_PIVOT0_STAGE1, E_PIVOT0_STAGE2, E_ARG, E_OOM, E_HIP, E_STATE, E_NODEVICE, E_BUSY, E_IO = range(10)
 MODE_FP64, MODE_MIXED = 0, 1
 
 # every symbol include/ocotillo.h declares
 EXPORTS = ["oco_create", "oco_destroy", "oco_set_wavelengths", "oco_set_physics", "oco_solve_block",
            "oco_solve_block_device", "oco_strerror", "oco_version", "oco_athena_slab", "oco_athena_slab_device",
+           "oco_solve_block_files", "oco_write_record",
            "oco_batch_create", "oco_batch_destroy", "oco_batch_layout", "oco_batch_buffers", "oco_batch_run",
            "oco_batch_run_host", "oco_batch_tuning",
            "oco_server_create", "oco_server_submit", "oco_server_wait", "oco_server_poll", "oco_server_stats",
@@ -97,6 +98,10 @@ def load_library(path=None):
                                          C.c_void_p, C.c_int, C.POINTER(Diag)]
     _SZ = C.POINTER(C.c_size_t)
     _PP = C.POINTER(C.c_void_p)
+    L.oco_solve_block_files.argtypes = ([C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double,
+                                         _DP, _DP, C.c_int, C.c_char_p, C.c_char_p, C.c_size_t, _DP, _DP,
+                                         C.POINTER(Diag)])
+    L.oco_write_record.argtypes = [C.c_char_p, C.c_void_p, C.c_size_t]
     L.oco_athena_slab.argtypes = ([C.c_void_p] + [C.c_int] * 5 + [_DP] * 6 + [C.POINTER(AthenaUnits)]
                                   + [_DP] * 2)
     L.oco_athena_slab_device.argtypes = ([C.c_void_p] + [C.c_int] * 5 + [C.c_void_p] * 6
@@ -126,6 +131,15 @@ def load_library(path=None):
 
 def strerror(status):
     return load_library().oco_strerror(status).decode()
+
+
+def write_record(path, array):
+    """One array as a one-record Fortran unformatted sequential file in
+    gfortran's layout (oco_write_record; no GPU involved)."""
+    a = np.ascontiguousarray(array)
+    st = load_library().oco_write_record(os.fsencode(path), a.ctypes.data_as(C.c_void_p), a.nbytes)
+    if st != OK:
+        raise OcoError(st, "oco_write_record: " + strerror(st))
 
 
 def _ptr(a):
@@ -211,6 +225,29 @@ class Context:
             raise OcoError(st, "oco_solve_block_device: " + strerror(st), d)
         return d
 
+
+    def solve_block_files(self, rho3d, T3d, dz, outdir, snapshot="0000", dz1=None, dz2=None, mode=MODE_FP64,
+                          stage_bytes=0, raise_on_pivot=True):
+        """Solve a block and write its three output files into `outdir` (which
+        must exist) as the reference's output_binary does.  Returns
+        (U_col [nw, nz+6], A_col [nw, nz], Diag): the column (nxloc, nyloc)
+        the reference prints to diagnostics.txt."""
+        rho3d = np.ascontiguousarray(rho3d, dtype=np.float64)
+        T3d = np.ascontiguousarray(T3d, dtype=np.float64)
+        if rho3d.ndim != 3 or rho3d.shape != T3d.shape:
+            raise OcoError(E_ARG, "rho3d and T3d must both be [nxloc, nyloc, nz]")
+        nxloc, nyloc, nz = rho3d.shape
+        U_col = np.zeros((self.nw, nz + 6))
+        A_col = np.zeros((self.nw, nz))
+        d = Diag()
+        dz1 = 1.0 / dz if dz1 is None else dz1
+        dz2 = dz * dz if dz2 is None else dz2
+        st = self.L.oco_solve_block_files(self.h, nz, nyloc, nxloc, dz, dz1, dz2, _ptr(rho3d), _ptr(T3d), mode,
+                                          os.fsencode(outdir), snapshot.encode(), stage_bytes, _ptr(U_col),
+                                          _ptr(A_col), C.byref(d))
+        if st != OK and (raise_on_pivot or st not in (E_PIVOT0_STAGE1, E_PIVOT0_STAGE2)):
+            raise OcoError(st, "oco_solve_block_files: " + strerror(st), d)
+        return U_col, A_col, d
 
     def athena_slab(self, slab, tgauss, units, iz0, rho3d, T3d):
         """Raw Athena fields of one z slab -> the slab's z range of rho3d, T3d

@@ -35,7 +35,8 @@ def build_lib(out=LIB, defines=(), build_dir=BUILD):
     (e.g. OCO_SWEEP_WAVES_PER_EU=4) for the kernels."""
     os.makedirs(build_dir, exist_ok=True)
     objs = []
-    for src, device in (("oco_kernels.hip", True), ("oco_api.cpp", False), ("oco_serve.cpp", False)):
+    for src, device in (("oco_kernels.hip", True), ("oco_api.cpp", False), ("oco_serve.cpp", False),
+                        ("oco_output.cpp", False)):
         obj = os.path.join(build_dir, os.path.splitext(src)[0] + ".o")
         if device:
             cmd = [HIPCC] + COMMON + DEVICE + ["-D" + d for d in defines] + ["-c", os.path.join(CSRC, src), "-o", obj]

@@ -3,12 +3,18 @@
  * wavelength tables, device buffers and the block solve.
  */
 #include <algorithm>
+#include <chrono>
 #include <cmath>
+#include <condition_variable>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <deque>
+#include <mutex>
 #include <new>
 #include <numeric>
+#include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/ocotillo.h"
@@ -37,6 +43,11 @@ struct oco_ctx {
     double sigma_grey = 0.4;
     int mixed_sweep = 0;  /* mixed mode: 0 by column length, 1 wavefront per system, 2 lane per system */
     DevBuf wtab, rho, T, cells, scratch, U, V, A, fail, canary, raw, tg;
+    /* block outputs to files: copy stream, per-slot events, pinned staging */
+    hipStream_t copy_stream = nullptr;
+    hipEvent_t ev_sweep[2] = {nullptr, nullptr}, ev_copy[2] = {nullptr, nullptr};
+    void* pinned = nullptr;
+    size_t pinned_n = 0;
 };
 
 static int hip_status(hipError_t e) { return e == hipSuccess ? OCO_OK : OCO_E_HIP; }
@@ -168,6 +179,7 @@ const char* oco_strerror(int s)
     case OCO_E_STATE: return "context not ready (wavelengths not set)";
     case OCO_E_NODEVICE: return "no HIP device";
     case OCO_E_BUSY: return "context busy";
+    case OCO_E_IO: return "output file could not be written";
     default: return "unknown status";
     }
 }
@@ -216,6 +228,15 @@ int oco_destroy(oco_ctx* c)
         release(c, *b);
     for (auto& e : c->ev)
         if (e) (void)hipEventDestroy(e);
+    for (int i = 0; i < 2; i++) {
+        if (c->ev_sweep[i]) (void)hipEventDestroy(c->ev_sweep[i]);
+        if (c->ev_copy[i]) (void)hipEventDestroy(c->ev_copy[i]);
+    }
+    if (c->copy_stream) {
+        (void)hipStreamSynchronize(c->copy_stream);
+        (void)hipStreamDestroy(c->copy_stream);
+    }
+    if (c->pinned) (void)hipHostFree(c->pinned);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return OCO_OK;
@@ -247,46 +268,49 @@ int oco_set_physics(oco_ctx* c, double fully_ionized_T, int lgrey, double sigma_
     return OCO_OK;
 }
 
-int oco_solve_block_device(oco_ctx* c, int nz, int nyloc, int nxloc, double dz, double dz1, double dz2,
-                           const double* d_rho, const double* d_T, double* d_U, double* d_V, double* d_A,
-                           int mode, oco_diag* diag)
+}  /* extern "C" */
+
+/* Kernel and grid for sweeping nw_run wavelengths of ncol columns. */
+struct SweepPlan {
+    int wave = 0, nblocks = 0;
+};
+
+static SweepPlan sweep_plan(const oco_ctx* c, int nz, long ncol, int nw_run, int mixed)
 {
-    if (!c) return OCO_E_ARG;
-    if (nz < 4 || nyloc < 1 || nxloc < 1 || !d_rho || !d_T || !d_U || !d_V || !d_A) return OCO_E_ARG;
-    if (mode != OCO_MODE_FP64 && mode != OCO_MODE_MIXED) return OCO_E_ARG;
-    if (c->nw <= 0) return OCO_E_STATE;
-    if (c->lgrey && c->nw != 1) return OCO_E_ARG;   /* source/flux_feautrier.f90:84-87 */
-    HIPCHK(hipSetDevice(c->device));
-    const long ncol = (long)nxloc * nyloc;
-    const long ncells = ncol * nz;
-    const long nsys = ncol * c->nw;
+    SweepPlan p;
+    const long nsys = ncol * nw_run;
     const long ntiles = (nsys + 63) / 64;
-    const int mixed = mode == OCO_MODE_MIXED;
-    const int uniform = (c->nw % 64) == 0;
     /* mixed, non-grey: one wavefront per system when its LDS fits */
-    const int wave = mixed && !c->lgrey && c->mixed_sweep != 2 && oco_wave_fits(nz, c->device) &&
-                     (c->mixed_sweep == 1 || oco_wave_preferred(nz));
-    int nblocks;
-    if (wave) {
+    p.wave = mixed && !c->lgrey && c->mixed_sweep != 2 && oco_wave_fits(nz, c->device) &&
+             (c->mixed_sweep == 1 || oco_wave_preferred(nz));
+    if (p.wave) {
         const int grid = oco_wave_grid(nz, c->device);
-        nblocks = (int)(nsys < grid ? nsys : grid);
+        p.nblocks = (int)(nsys < grid ? nsys : grid);
     } else {
-        const int grid = oco_sweep_grid(c->lgrey, mixed, uniform, c->device);
-        nblocks = (int)(ntiles < grid ? ntiles : grid);
+        const int grid = oco_sweep_grid(c->lgrey, mixed, (nw_run % 64) == 0, c->device);
+        p.nblocks = (int)(ntiles < grid ? ntiles : grid);
     }
+    return p;
+}
+
+/* Cell records, scratch, pivot key and canary buffers for a plan. */
+static int sweep_buffers(oco_ctx* c, const SweepPlan& p, int nz, long ncells)
+{
     int st;
     if (!c->lgrey && (st = ensure(c, c->cells, (size_t)ncells * OCO_CELL_STRIDE * sizeof(double)))) return st;
-    if (!wave) {
+    if (!p.wave) {
         /* one scratch slot per wavefront; multi-wavefront workgroups round up */
-        const size_t slots = ((size_t)nblocks + 7) / 8 * 8;
+        const size_t slots = ((size_t)p.nblocks + 7) / 8 * 8;
         if ((st = ensure(c, c->scratch, slots * nz * 3 * 64 * sizeof(double)))) return st;
     }
     if ((st = ensure(c, c->fail, sizeof(unsigned long long)))) return st;
     if ((st = ensure(c, c->canary, (4 + 5 * (size_t)nz) * sizeof(double)))) return st;
+    return OCO_OK;
+}
 
-    const unsigned long long none = OCO_FAIL_NONE;
-    HIPCHK(hipMemcpyAsync(c->fail.p, &none, sizeof(none), hipMemcpyHostToDevice, c->stream));
-
+static oco_sweep_args sweep_args(const oco_ctx* c, int nz, long ncol, double dz, double dz1, double dz2,
+                                 const double* d_rho, const double* d_T)
+{
     oco_sweep_args a;
     a.nz = nz;
     a.nw = c->nw;
@@ -300,19 +324,83 @@ int oco_solve_block_device(oco_ctx* c, int nz, int nyloc, int nxloc, double dz, 
     a.T = d_T;
     a.cells = (const double*)c->cells.p;
     a.wtab = (const double*)c->wtab.p;
-    a.U = d_U;
-    a.V = d_V;
-    a.A = d_A;
+    a.U = a.V = a.A = nullptr;
     a.scratch = (double*)c->scratch.p;
     a.fail = (unsigned long long*)c->fail.p;
     a.canary = (double*)c->canary.p;
+    return a;
+}
+
+/* First zero pivot of a sweep over wavelengths [iw0, iw0+nw_run) as a key
+ * in the reference's loop order over the whole block (column-major, then
+ * wavelength, then stage and row), or OCO_FAIL_NONE. */
+static unsigned long long global_key(unsigned long long key, int iw0, int nw_run, int nw)
+{
+    if (key == OCO_FAIL_NONE) return key;
+    const unsigned long long sys = key >> 32;
+    const unsigned long long gsys = (sys / nw_run) * nw + iw0 + sys % nw_run;
+    return (gsys << 32) | (key & 0xffffffffULL);
+}
+
+static void decode_key(unsigned long long key, int nw, int nyloc, oco_diag* d)
+{
+    const long sys = (long)(key >> 32);
+    d->status = (int)((key >> 28) & 0xf);
+    d->fail_j = (int)(key & 0x0fffffffULL);
+    const long col = sys / nw;
+    d->fail_iw = (int)(sys % nw) + 1;
+    d->fail_ix = (int)(col / nyloc) + 1;
+    d->fail_iy = (int)(col % nyloc) + 1;
+}
+
+/* min/max of the first pillar's U (interior), V, absorp
+ * (source/flux_feautrier.f90:222-224). */
+static void pillar_minmax(const double* U, const double* V, const double* A, int nz, oco_diag* d)
+{
+    const double* Uh = U + 3;
+    d->minU = d->maxU = Uh[0];
+    d->minV = d->maxV = V[0];
+    d->minA = d->maxA = A[0];
+    for (int i = 0; i < nz; i++) {
+        d->minU = fmin(d->minU, Uh[i]); d->maxU = fmax(d->maxU, Uh[i]);
+        d->minV = fmin(d->minV, V[i]); d->maxV = fmax(d->maxV, V[i]);
+        d->minA = fmin(d->minA, A[i]); d->maxA = fmax(d->maxA, A[i]);
+    }
+}
+
+extern "C" {
+
+int oco_solve_block_device(oco_ctx* c, int nz, int nyloc, int nxloc, double dz, double dz1, double dz2,
+                           const double* d_rho, const double* d_T, double* d_U, double* d_V, double* d_A,
+                           int mode, oco_diag* diag)
+{
+    if (!c) return OCO_E_ARG;
+    if (nz < 4 || nyloc < 1 || nxloc < 1 || !d_rho || !d_T || !d_U || !d_V || !d_A) return OCO_E_ARG;
+    if (mode != OCO_MODE_FP64 && mode != OCO_MODE_MIXED) return OCO_E_ARG;
+    if (c->nw <= 0) return OCO_E_STATE;
+    if (c->lgrey && c->nw != 1) return OCO_E_ARG;   /* source/flux_feautrier.f90:84-87 */
+    HIPCHK(hipSetDevice(c->device));
+    const long ncol = (long)nxloc * nyloc;
+    const long ncells = ncol * nz;
+    const int mixed = mode == OCO_MODE_MIXED;
+    const SweepPlan p = sweep_plan(c, nz, ncol, c->nw, mixed);
+    int st;
+    if ((st = sweep_buffers(c, p, nz, ncells))) return st;
+
+    const unsigned long long none = OCO_FAIL_NONE;
+    HIPCHK(hipMemcpyAsync(c->fail.p, &none, sizeof(none), hipMemcpyHostToDevice, c->stream));
+
+    oco_sweep_args a = sweep_args(c, nz, ncol, dz, dz1, dz2, d_rho, d_T);
+    a.U = d_U;
+    a.V = d_V;
+    a.A = d_A;
 
     HIPCHK(hipEventRecord(c->ev[0], c->stream));
     if (!c->lgrey)
-        HIPCHK(oco_launch_cells(d_rho, d_T, ncells, c->fully_ionized_T, (double*)c->cells.p, wave ? nz : 0,
+        HIPCHK(oco_launch_cells(d_rho, d_T, ncells, c->fully_ionized_T, (double*)c->cells.p, p.wave ? nz : 0,
                                 c->stream));
     HIPCHK(hipEventRecord(c->ev[1], c->stream));
-    HIPCHK(oco_launch_sweep(a, c->lgrey, mixed, wave, nblocks, c->stream));
+    HIPCHK(oco_launch_sweep(a, c->lgrey, mixed, p.wave, p.nblocks, c->stream));
     HIPCHK(hipEventRecord(c->ev[2], c->stream));
 
     unsigned long long key = none;
@@ -332,30 +420,14 @@ int oco_solve_block_device(oco_ctx* c, int nz, int nyloc, int nxloc, double dz, 
     d.sum_c = can[2];
     d.sum_d = can[3];
     if (key != none) {
-        long sys = (long)(key >> 32);
-        d.status = (int)((key >> 28) & 0xf);
-        d.fail_j = (int)(key & 0x0fffffffULL);
-        long col = sys / c->nw;
-        d.fail_iw = (int)(sys % c->nw) + 1;
-        d.fail_ix = (int)(col / nyloc) + 1;
-        d.fail_iy = (int)(col % nyloc) + 1;
+        decode_key(key, c->nw, nyloc, &d);
     } else {
         /* first-pillar canaries: system (ix,iy,iw) = (1,1,1) is the first of each array */
         std::vector<double> h((size_t)(nz + 6) + 2 * (size_t)nz);
         HIPCHK(hipMemcpy(h.data(), d_U, (nz + 6) * sizeof(double), hipMemcpyDeviceToHost));
         HIPCHK(hipMemcpy(h.data() + nz + 6, d_V, nz * sizeof(double), hipMemcpyDeviceToHost));
         HIPCHK(hipMemcpy(h.data() + 2 * nz + 6, d_A, nz * sizeof(double), hipMemcpyDeviceToHost));
-        const double* Uh = h.data() + 3;
-        const double* Vh = h.data() + nz + 6;
-        const double* Ah = h.data() + 2 * nz + 6;
-        d.minU = d.maxU = Uh[0];
-        d.minV = d.maxV = Vh[0];
-        d.minA = d.maxA = Ah[0];
-        for (int i = 0; i < nz; i++) {
-            d.minU = fmin(d.minU, Uh[i]); d.maxU = fmax(d.maxU, Uh[i]);
-            d.minV = fmin(d.minV, Vh[i]); d.maxV = fmax(d.maxV, Vh[i]);
-            d.minA = fmin(d.minA, Ah[i]); d.maxA = fmax(d.maxA, Ah[i]);
-        }
+        pillar_minmax(h.data(), h.data() + nz + 6, h.data() + 2 * nz + 6, nz, &d);
     }
     if (diag) *diag = d;
     return d.status;
@@ -390,6 +462,265 @@ int oco_solve_block(oco_ctx* c, int nz, int nyloc, int nxloc, double dz, double 
     HIPCHK(hipMemcpyAsync(V, c->V.p, nout * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipMemcpyAsync(absorp, c->A.p, nout * sizeof(double), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
+    return st;
+}
+
+/* ---------------------------------------------------------------------- */
+/* Block outputs to files                                                  */
+/* ---------------------------------------------------------------------- */
+
+}  /* extern "C" */
+
+static int ensure_copy_stream(oco_ctx* c)
+{
+    if (c->copy_stream) return OCO_OK;
+    if (hipStreamCreateWithFlags(&c->copy_stream, hipStreamNonBlocking) != hipSuccess) return OCO_E_HIP;
+    for (int i = 0; i < 2; i++) {
+        if (hipEventCreateWithFlags(&c->ev_sweep[i], hipEventDisableTiming) != hipSuccess ||
+            hipEventCreateWithFlags(&c->ev_copy[i], hipEventDisableTiming) != hipSuccess)
+            return OCO_E_HIP;
+    }
+    return OCO_OK;
+}
+
+static int ensure_pinned(oco_ctx* c, size_t bytes)
+{
+    if (c->pinned_n >= bytes) return OCO_OK;
+    if (c->pinned) (void)hipHostFree(c->pinned);
+    c->pinned = nullptr;
+    c->pinned_n = 0;
+    if (hipHostMalloc(&c->pinned, bytes, hipHostMallocDefault) != hipSuccess) {
+        (void)hipGetLastError();
+        c->pinned = nullptr;
+        return OCO_E_OOM;
+    }
+    c->pinned_n = bytes;
+    return OCO_OK;
+}
+
+namespace {
+
+/* Hand-off between the launching thread and the file writer: chunk indices
+ * whose copies are enqueued, in order; -1 ends.  `written` counts chunks the
+ * writer is done with (written, or skipped after an error). */
+struct ChunkQueue {
+    std::mutex m;
+    std::condition_variable cv;
+    std::deque<int> q;
+    int written = 0;
+    int status = OCO_OK;
+
+    void push(int k)
+    {
+        {
+            std::lock_guard<std::mutex> lk(m);
+            q.push_back(k);
+        }
+        cv.notify_all();
+    }
+};
+
+}  // namespace
+
+extern "C" {
+
+int oco_solve_block_files(oco_ctx* c, int nz, int nyloc, int nxloc, double dz, double dz1, double dz2,
+                          const double* rho3d, const double* T3d, int mode, const char* dir, const char* snapshot,
+                          size_t stage_bytes, double* U_col, double* A_col, oco_diag* diag)
+{
+    const auto t_start = std::chrono::steady_clock::now();
+    if (!c || !rho3d || !T3d || !dir || !snapshot) return OCO_E_ARG;
+    if (nz < 4 || nyloc < 1 || nxloc < 1) return OCO_E_ARG;
+    if (mode != OCO_MODE_FP64 && mode != OCO_MODE_MIXED) return OCO_E_ARG;
+    if (c->nw <= 0) return OCO_E_STATE;
+    if (c->lgrey && c->nw != 1) return OCO_E_ARG;   /* source/flux_feautrier.f90:84-87 */
+    HIPCHK(hipSetDevice(c->device));
+    const int nw = c->nw, mz = nz + 6, mixed = mode == OCO_MODE_MIXED;
+    const long ncol = (long)nxloc * nyloc;
+    const size_t ncells = (size_t)ncol * nz;
+    const size_t bU = (size_t)ncol * mz * sizeof(double), bV = (size_t)ncol * nz * sizeof(double);
+    const size_t per_w = bU + 2 * bV;
+    if (stage_bytes == 0) stage_bytes = (size_t)512 << 20;
+    const int nwc = (int)std::min<size_t>((size_t)nw, std::max<size_t>(1, stage_bytes / per_w));
+    const int nch = (nw + nwc - 1) / nwc;
+    const int nlast = nw - (nch - 1) * nwc;
+    const SweepPlan p_full = sweep_plan(c, nz, ncol, nwc, mixed);
+    const SweepPlan p_last = sweep_plan(c, nz, ncol, nlast, mixed);
+    SweepPlan p_max = p_full;
+    p_max.nblocks = std::max(p_full.nblocks, p_last.nblocks);
+    const size_t slot_bytes = (size_t)nwc * per_w;
+    int st;
+    if ((st = sweep_buffers(c, p_max, nz, (long)ncells))) return st;
+    if ((st = ensure(c, c->rho, ncells * sizeof(double)))) return st;
+    if ((st = ensure(c, c->T, ncells * sizeof(double)))) return st;
+    if ((st = ensure(c, c->U, 2 * (size_t)nwc * bU))) return st;
+    if ((st = ensure(c, c->V, 2 * (size_t)nwc * bV))) return st;
+    if ((st = ensure(c, c->A, 2 * (size_t)nwc * bV))) return st;
+    /* pinned: two chunk slots [U][absorp][V], then per-chunk pivot keys, the
+     * canary sums and the "no pivot" value */
+    if ((st = ensure_pinned(c, 2 * slot_bytes + ((size_t)nch + 5) * sizeof(double)))) return st;
+    if ((st = ensure_copy_stream(c))) return st;
+    char* pin = (char*)c->pinned;
+    unsigned long long* keys = (unsigned long long*)(pin + 2 * slot_bytes);
+    double* can = (double*)(keys + nch);
+    unsigned long long* none = (unsigned long long*)(can + 4);
+    *none = OCO_FAIL_NONE;
+
+    oco_record_file* F[3] = {nullptr, nullptr, nullptr};
+    auto close_all = [&](int keep) {
+        int r = OCO_OK;
+        for (auto& f : F) {
+            const int q = oco_record_close(f, keep);
+            if (r == OCO_OK) r = q;
+            f = nullptr;
+        }
+        return r;
+    };
+    {
+        const char* names[3] = {"mean_intensity_", "absorption_coefficients_", "flux_"};
+        const unsigned long long sizes[3] = {nw * (unsigned long long)bU, nw * (unsigned long long)bV,
+                                             nw * (unsigned long long)bV};
+        for (int i = 0; i < 3; i++) {
+            const std::string path = std::string(dir) + "/" + names[i] + snapshot + ".bin";
+            if ((st = oco_record_open(path.c_str(), sizes[i], &F[i]))) {
+                close_all(0);
+                return st;
+            }
+        }
+    }
+
+    hipError_t he = hipMemcpyAsync(c->rho.p, rho3d, ncells * sizeof(double), hipMemcpyHostToDevice, c->stream);
+    if (he == hipSuccess) he = hipMemcpyAsync(c->T.p, T3d, ncells * sizeof(double), hipMemcpyHostToDevice, c->stream);
+    if (he == hipSuccess) he = hipEventRecord(c->ev[0], c->stream);
+    if (he == hipSuccess && !c->lgrey)
+        he = oco_launch_cells((const double*)c->rho.p, (const double*)c->T.p, (long)ncells, c->fully_ionized_T,
+                              (double*)c->cells.p, p_full.wave ? nz : 0, c->stream);
+    if (he == hipSuccess) he = hipEventRecord(c->ev[1], c->stream);
+    if (he != hipSuccess) {
+        fprintf(stderr, "ocotillo: block outputs: %s\n", hipGetErrorString(he));
+        close_all(0);
+        return OCO_E_HIP;
+    }
+
+    oco_diag d;
+    memset(&d, 0, sizeof(d));
+    ChunkQueue w;
+    /* writer: waits for each chunk's copy, writes its byte ranges of the three
+     * records, picks out the printed column and the first pillar */
+    std::thread writer([&]() {
+        (void)hipSetDevice(c->device);
+        for (;;) {
+            int k;
+            {
+                std::unique_lock<std::mutex> lk(w.m);
+                w.cv.wait(lk, [&] { return !w.q.empty(); });
+                k = w.q.front();
+                w.q.pop_front();
+            }
+            if (k < 0) break;
+            const int slot = k & 1, iw0 = k * nwc, n = std::min(nwc, nw - iw0);
+            int r = hipEventSynchronize(c->ev_copy[slot]) == hipSuccess ? OCO_OK : OCO_E_HIP;
+            bool skip;
+            {
+                std::lock_guard<std::mutex> lk(w.m);
+                skip = w.status != OCO_OK;
+            }
+            const char* sb = pin + slot * slot_bytes;
+            const double* sU = (const double*)sb;
+            const double* sA = (const double*)(sb + (size_t)nwc * bU);
+            const double* sV = (const double*)(sb + (size_t)nwc * (bU + bV));
+            if (r == OCO_OK && !skip) {
+                r = oco_record_write(F[0], (unsigned long long)iw0 * bU, sU, (size_t)n * bU);
+                if (r == OCO_OK) r = oco_record_write(F[1], (unsigned long long)iw0 * bV, sA, (size_t)n * bV);
+                if (r == OCO_OK) r = oco_record_write(F[2], (unsigned long long)iw0 * bV, sV, (size_t)n * bV);
+            }
+            if (r == OCO_OK && !skip) {
+                for (int j = 0; j < n; j++) {
+                    if (U_col)
+                        memcpy(U_col + (size_t)(iw0 + j) * mz, sU + ((size_t)j * ncol + ncol - 1) * mz,
+                               (size_t)mz * sizeof(double));
+                    if (A_col)
+                        memcpy(A_col + (size_t)(iw0 + j) * nz, sA + ((size_t)j * ncol + ncol - 1) * nz,
+                               (size_t)nz * sizeof(double));
+                }
+                if (k == 0) pillar_minmax(sU, sV, sA, nz, &d);
+            }
+            {
+                std::lock_guard<std::mutex> lk(w.m);
+                if (r != OCO_OK && w.status == OCO_OK) w.status = r;
+                w.written = k + 1;
+            }
+            w.cv.notify_all();
+        }
+    });
+
+    for (int k = 0; k < nch; k++) {
+        const int slot = k & 1, iw0 = k * nwc, n = std::min(nwc, nw - iw0);
+        if (k >= 2) {   /* slot reuse: the writer is done with chunk k-2 */
+            std::unique_lock<std::mutex> lk(w.m);
+            w.cv.wait(lk, [&] { return w.written >= k - 1; });
+            if (w.status != OCO_OK) break;
+        }
+        oco_sweep_args a = sweep_args(c, nz, ncol, dz, dz1, dz2, (const double*)c->rho.p, (const double*)c->T.p);
+        a.nw = n;
+        a.wtab = (const double*)c->wtab.p + (size_t)iw0 * OCO_WT_STRIDE;
+        a.U = (double*)c->U.p + (size_t)slot * nwc * ncol * mz;
+        a.A = (double*)c->A.p + (size_t)slot * nwc * ncol * nz;
+        a.V = (double*)c->V.p + (size_t)slot * nwc * ncol * nz;
+        a.canary = k == 0 ? (double*)c->canary.p : nullptr;
+        const SweepPlan& pk = n == nwc ? p_full : p_last;
+        char* sb = pin + slot * slot_bytes;
+        he = hipMemcpyAsync(c->fail.p, none, sizeof(*none), hipMemcpyHostToDevice, c->stream);
+        if (he == hipSuccess) he = oco_launch_sweep(a, c->lgrey, mixed, pk.wave, pk.nblocks, c->stream);
+        if (he == hipSuccess)
+            he = hipMemcpyAsync(keys + k, c->fail.p, sizeof(*keys), hipMemcpyDeviceToHost, c->stream);
+        if (he == hipSuccess && k == 0)
+            he = hipMemcpyAsync(can, c->canary.p, 4 * sizeof(double), hipMemcpyDeviceToHost, c->stream);
+        if (he == hipSuccess) he = hipEventRecord(c->ev_sweep[slot], c->stream);
+        if (he == hipSuccess && k == nch - 1) he = hipEventRecord(c->ev[2], c->stream);
+        if (he == hipSuccess) he = hipStreamWaitEvent(c->copy_stream, c->ev_sweep[slot], 0);
+        if (he == hipSuccess) he = hipMemcpyAsync(sb, a.U, (size_t)n * bU, hipMemcpyDeviceToHost, c->copy_stream);
+        if (he == hipSuccess)
+            he = hipMemcpyAsync(sb + (size_t)nwc * bU, a.A, (size_t)n * bV, hipMemcpyDeviceToHost, c->copy_stream);
+        if (he == hipSuccess)
+            he = hipMemcpyAsync(sb + (size_t)nwc * (bU + bV), a.V, (size_t)n * bV, hipMemcpyDeviceToHost,
+                                c->copy_stream);
+        if (he == hipSuccess) he = hipEventRecord(c->ev_copy[slot], c->copy_stream);
+        if (he != hipSuccess) {
+            fprintf(stderr, "ocotillo: block outputs: %s\n", hipGetErrorString(he));
+            st = OCO_E_HIP;
+            break;
+        }
+        w.push(k);
+    }
+    w.push(-1);
+    writer.join();
+    if (hipStreamSynchronize(c->stream) != hipSuccess && st == OCO_OK) st = OCO_E_HIP;
+    if (hipStreamSynchronize(c->copy_stream) != hipSuccess && st == OCO_OK) st = OCO_E_HIP;
+    if (st == OCO_OK) st = w.status;
+
+    unsigned long long best = OCO_FAIL_NONE;
+    if (st == OCO_OK) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, c->ev[0], c->ev[1]) == hipSuccess) d.cells_ms = ms;
+        if (hipEventElapsedTime(&ms, c->ev[1], c->ev[2]) == hipSuccess) d.sweep_ms = ms;
+        d.sum_a = can[0];
+        d.sum_b = can[1];
+        d.sum_c = can[2];
+        d.sum_d = can[3];
+        for (int k = 0; k < nch; k++)
+            best = std::min(best, global_key(keys[k], k * nwc, std::min(nwc, nw - k * nwc), nw));
+        if (best != OCO_FAIL_NONE) {
+            decode_key(best, nw, nyloc, &d);
+            d.minU = d.maxU = d.minV = d.maxV = d.minA = d.maxA = 0.0;
+        }
+    }
+    const int keep = st == OCO_OK && best == OCO_FAIL_NONE;
+    const int cst = close_all(keep);
+    if (st == OCO_OK && keep) st = cst;
+    if (st == OCO_OK && !keep) st = d.status;
+    d.total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
+    if (diag) *diag = d;
     return st;
 }
 

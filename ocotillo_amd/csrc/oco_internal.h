@@ -69,6 +69,14 @@ struct oco_serve_args {
     unsigned long long* fail;     /* per slot: first zero pivot key (FP64 mode) */
 };
 
+/* One-record Fortran unformatted sequential file in gfortran's layout
+ * (oco_output.cpp): created as <path>.partial, data written at any record
+ * offset, renamed to <path> by close(keep=1), removed by close(keep=0). */
+struct oco_record_file;
+int oco_record_open(const char* path, unsigned long long nbytes, oco_record_file** f);
+int oco_record_write(oco_record_file* f, unsigned long long off, const void* data, size_t n);
+int oco_record_close(oco_record_file* f, int keep);
+
 /* Athena slab to the block's rho, T (see athena_slab in oco_kernels.hip). */
 struct oco_athena_args {
     int nz, nyloc, nxloc, nzl, iz0;

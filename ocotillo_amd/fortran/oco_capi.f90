@@ -7,7 +7,8 @@ module oco_capi
   private
 
   integer(c_int), parameter, public :: OCO_OK = 0, OCO_E_PIVOT0_STAGE1 = 1, OCO_E_PIVOT0_STAGE2 = 2, &
-       OCO_E_ARG = 3, OCO_E_OOM = 4, OCO_E_HIP = 5, OCO_E_STATE = 6, OCO_E_NODEVICE = 7
+       OCO_E_ARG = 3, OCO_E_OOM = 4, OCO_E_HIP = 5, OCO_E_STATE = 6, OCO_E_NODEVICE = 7, OCO_E_BUSY = 8, &
+       OCO_E_IO = 9
   integer(c_int), parameter, public :: OCO_MODE_FP64 = 0, OCO_MODE_MIXED = 1
 
   type, bind(C), public :: oco_diag
@@ -24,7 +25,7 @@ module oco_capi
   end type oco_athena_units
 
   public :: oco_create, oco_destroy, oco_set_wavelengths, oco_set_physics, oco_solve_block, oco_message
-  public :: oco_athena_slab
+  public :: oco_athena_slab, oco_solve_block_files, oco_write_record
 
   interface
     integer(c_int) function oco_create(device, hbm_budget, ctx) bind(C, name='oco_create')
@@ -64,6 +65,26 @@ module oco_capi
       real(c_double), intent(inout) :: u(*), v(*), absorp(*)
       type(oco_diag), intent(out) :: diag
     end function oco_solve_block
+
+    integer(c_int) function oco_solve_block_files(ctx, nz, nyloc, nxloc, dz, dz1, dz2, rho3d, t3d, mode, dir, &
+         snapshot, stage_bytes, u_col, a_col, diag) bind(C, name='oco_solve_block_files')
+      import :: c_int, c_ptr, c_double, c_char, c_size_t, oco_diag
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: nz, nyloc, nxloc, mode
+      real(c_double), value :: dz, dz1, dz2
+      real(c_double), intent(in) :: rho3d(*), t3d(*)
+      character(kind=c_char), intent(in) :: dir(*), snapshot(*)
+      integer(c_size_t), value :: stage_bytes
+      real(c_double), intent(out) :: u_col(*), a_col(*)
+      type(oco_diag), intent(out) :: diag
+    end function oco_solve_block_files
+
+    integer(c_int) function oco_write_record(path, data, nbytes) bind(C, name='oco_write_record')
+      import :: c_int, c_double, c_char, c_size_t
+      character(kind=c_char), intent(in) :: path(*)
+      real(c_double), intent(in) :: data(*)
+      integer(c_size_t), value :: nbytes
+    end function oco_write_record
 
     integer(c_int) function oco_athena_slab(ctx, nz, nyloc, nxloc, nzl, iz0, rho_raw, mx, my, mz, e, tgauss, &
          units, rho3d, t3d) bind(C, name='oco_athena_slab')

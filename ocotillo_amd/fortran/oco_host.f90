@@ -1,8 +1,11 @@
 ! Drop-in host for the GPU path: keeps ocotillo's user surface -- input.in
 ! namelists, resolution.in, argv snapshot tag, the analytic disk or the
 ! Athena snapshot reader, diagnostics.txt and the per-block binary outputs --
-! and replaces the column loop (source/flux_feautrier.f90:160-228) by one
-! call per processor block into the HIP library (oco_solve_block).
+! and replaces the column loop and the block's binary outputs
+! (source/flux_feautrier.f90:160-228, 243) by one call per processor block
+! into the HIP library (oco_solve_block_files), which streams U, V and
+! absorp from the device into the three files; the Athena field arithmetic
+! runs on the device too (oco_athena_slab).
 !
 ! Differences from the reference, all additive:
 !   * resolution.in is read at run time (the reference compiles it in,
@@ -462,9 +465,13 @@ contains
 end module oco_athena
 
 !------------------------------------------------------------------------------
-! Writers (source/io.f90:31-101).
+! Writers (source/io.f90:31-73).  The per-block binary outputs
+! (output_binary, :76-101) are written by the library as the block is solved
+! (oco_solve_block_files).
 module oco_output
   use oco_shapes
+  use oco_capi, only: oco_write_record, oco_message, OCO_OK
+  use, intrinsic :: iso_c_binding, only: c_int, c_size_t, c_null_char
   implicit none
 contains
   subroutine output_ascii(U, absorp)
@@ -487,40 +494,28 @@ contains
       write(u, *) i, z(n1 + i - 1)
     enddo
     close(u)
-    open(newunit=u, file="output/zgrid.bin", form="unformatted", status="replace", action="write")
-    write(u) z
-    close(u)
+    call write_record("output/zgrid.bin", z, size(z))
     open(newunit=u, file="output/wavegrid.txt", status="replace", action="write")
     do i = 1, nw
       write(u, *) i, waves(i)
     enddo
     close(u)
-    open(newunit=u, file="output/wavegrid.bin", form="unformatted", status="replace", action="write")
-    write(u) waves
-    close(u)
+    call write_record("output/wavegrid.bin", waves, size(waves))
   end subroutine output_grid
 
-  subroutine output_binary(U, absorp, V, iprocx, iprocy, snapshot)
-    real(8), intent(in) :: U(:,:,:,:), absorp(:,:,:,:), V(:,:,:,:)
-    integer, intent(in) :: iprocx, iprocy
-    character(len=*), intent(in) :: snapshot
-    character(len=256) :: outdir
-    integer :: iu
-    outdir = 'output/procx' // trim(itoa(iprocx)) // '_procy' // trim(itoa(iprocy))
-    call execute_command_line('mkdir -p ' // trim(outdir))
-    open(newunit=iu, file=trim(outdir) // '/mean_intensity_' // trim(snapshot) // '.bin', form='unformatted', &
-         status='replace', action='write')
-    write(iu) U
-    close(iu)
-    open(newunit=iu, file=trim(outdir) // '/absorption_coefficients_' // trim(snapshot) // '.bin', &
-         form='unformatted', status='replace', action='write')
-    write(iu) absorp
-    close(iu)
-    open(newunit=iu, file=trim(outdir) // '/flux_' // trim(snapshot) // '.bin', form='unformatted', &
-         status='replace', action='write')
-    write(iu) V
-    close(iu)
-  end subroutine output_binary
+  ! One array as a one-record file in the reference's (gfortran's) layout.
+  subroutine write_record(path, a, n)
+    character(len=*), intent(in) :: path
+    integer, intent(in) :: n
+    real(8), intent(in) :: a(n)
+    integer(c_int) :: st
+    st = oco_write_record(trim(path) // c_null_char, a, int(8 * n, c_size_t))
+    if (st /= OCO_OK) then
+      print*, "cannot write ", trim(path), ": ", oco_message(st)
+      stop 2
+    endif
+  end subroutine write_record
+
 end module oco_output
 
 !------------------------------------------------------------------------------
@@ -532,7 +527,7 @@ program flux_feautrier_gpu
   use oco_athena, only: read_athena_input, read_from_athena
   use oco_output
   implicit none
-  real(8), allocatable :: U(:,:,:,:), V(:,:,:,:), absorp(:,:,:,:), rho3d(:,:,:), temp3d(:,:,:)
+  real(8), allocatable :: rho3d(:,:,:), temp3d(:,:,:), Ucol(:,:), Acol(:,:)
   real(8), allocatable :: z(:), waves(:), rho(:), T(:)
   real(8) :: z0, z1, w0 = 3000, w1 = 5000, sigma_grey, dz, dz1 = 0, dz2 = 0, dw
   real(8) :: fully_ionized_T, hbm_budget_gb = 0
@@ -542,6 +537,7 @@ program flux_feautrier_gpu
   integer :: oco_mode = 0, oco_device = 0, iprocx, iprocy, ix, iy, i, err, iu
   integer(c_int) :: st
   character(len=90) :: snapshot, inputfile = './input.in'
+  character(len=256) :: outdir
   type(c_ptr) :: ctx
   type(oco_diag) :: diag
   real(8) :: loop_wall = 0, dev_ms = 0
@@ -574,20 +570,12 @@ program flux_feautrier_gpu
     stop
   endif
 
-  allocate(V(nz, nyloc, nxloc, nw), stat=err)
+  ! U, V and absorp stay on the device and go to the files in chunks; the
+  ! host holds the inputs and the column printed to diagnostics.txt
+  allocate(rho3d(nz, nyloc, nxloc), temp3d(nz, nyloc, nxloc), z(mz), waves(nw), rho(nz), T(nz), &
+       Ucol(mz, nw), Acol(nz, nw), stat=err)
   if (err /= 0) then
-    print*, "Could not allocate array of size ", nz, nyloc, nxloc, nw, " for array V."
-    stop
-  endif
-  allocate(absorp(nz, nyloc, nxloc, nw), stat=err)
-  if (err /= 0) then
-    print*, "Could not allocate array of size ", nz, nyloc, nxloc, nw, " for array absorp_coeff."
-    stop
-  endif
-  allocate(U(mz, nyloc, nxloc, nw), rho3d(nz, nyloc, nxloc), temp3d(nz, nyloc, nxloc), z(mz), waves(nw), &
-       rho(nz), T(nz), stat=err)
-  if (err /= 0) then
-    print*, "Could not allocate the mean intensity and input arrays."
+    print*, "Could not allocate the input arrays."
     stop
   endif
 
@@ -649,9 +637,12 @@ program flux_feautrier_gpu
         dz2 = dz**2
       endif
       if (lroot) call cpu_time(start_loop)
+      outdir = 'output/procx' // trim(itoa(iprocx)) // '_procy' // trim(itoa(iprocy))
+      call execute_command_line('mkdir -p ' // trim(outdir))
       call system_clock(c0, crate)
-      st = oco_solve_block(ctx, int(nz, c_int), int(nyloc, c_int), int(nxloc, c_int), dz, dz1, dz2, &
-           rho3d, temp3d, U, V, absorp, int(oco_mode, c_int), diag)
+      st = oco_solve_block_files(ctx, int(nz, c_int), int(nyloc, c_int), int(nxloc, c_int), dz, dz1, dz2, &
+           rho3d, temp3d, int(oco_mode, c_int), trim(outdir) // c_null_char, trim(snapshot) // c_null_char, &
+           0_c_size_t, Ucol, Acol, diag)
       call system_clock(c1)
       if (st == OCO_E_PIVOT0_STAGE1) then
         print*, "ERROR tridiag stage 1: bet=b(1)) = ", 0d0
@@ -668,15 +659,14 @@ program flux_feautrier_gpu
       if (lroot) then
         call cpu_time(finish_loop)
         loop_wall = real(c1 - c0, 8) / real(crate, 8)
-        dev_ms = diag%total_ms
+        dev_ms = diag%cells_ms + diag%sweep_ms
         print*, 'sum(a), sum(b), sum(c), sum(d)=', diag%sum_a, diag%sum_b, diag%sum_c, diag%sum_d
         print*, 'min/max(U)', diag%minU, diag%maxU
         print*, 'min/max(V)', diag%minV, diag%maxV
         print*, 'min/max(absorp_coeff)', diag%minA, diag%maxA
         call output_grid(z, waves)
-        call output_ascii(U(:, nyloc, nxloc, :), absorp(:, nyloc, nxloc, :))
+        call output_ascii(Ucol, Acol)
       endif
-      call output_binary(U, absorp, V, iprocx, iprocy, snapshot)
     enddo
   enddo
   st = oco_destroy(ctx)
@@ -686,7 +676,7 @@ program flux_feautrier_gpu
   print*, "processor column time = ", finish_loop - start_loop, " seconds."
   print*, "Execution time =", (finish_loop - start_loop) / (1d0 * nz * nyloc * nxloc * nw) * 1e6, &
        " micro-seconds per wavelength point per mesh point."
-  print*, "GPU block (wall incl. transfers) =", loop_wall, " s; device kernels =", dev_ms * 1d-3, " s"
+  print*, "GPU block and its output files (wall) =", loop_wall, " s; device kernels =", dev_ms * 1d-3, " s"
   print*, ''
   print*, "Wall time = ", finish - start, " seconds."
   print*, "Execution time =", (finish - start) / (1d0 * nz * ny * nx * nw) * 1e6, &
