@@ -103,12 +103,14 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--config", default="c4", choices=["c4", "c2", "c3", "c5", "athena"],
+    ap.add_argument("--config", default="c4", choices=["c4", "c2", "c3", "c5", "athena", "files"],
                     help="c4: the 256x256x512 x 128-wavelength mixed-precision snapshot (default); "
                          "c2: the 64x64x256 x 128-wavelength FP64 drop-in block; "
                          "c3: 4096 uneven columns through the batched persistent path; "
                          "c5: the serving loop under Poisson arrivals; "
-                         "athena: Athena input construction of the c4 block only")
+                         "athena: Athena input construction of the c4 block only; "
+                         "files: a block solved and written to its three output files (default: the c2 "
+                         "block), streamed vs solve-then-write")
     ap.add_argument("--mode", default=None, choices=["both", "fp64", "mixed"],
                     help="precision modes to time (default: both for c4 and c3, fp64 for c2)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
@@ -144,6 +146,8 @@ def main():
         return bench_batch(args)
     if args.config == "athena":
         return bench_athena_only(args)
+    if args.config == "files":
+        return bench_files(args)
     if args.config == "c5":
         res = bench_serve(args.jobs or 300, args.pool or 12)
         out = {"metric": METRIC, "value": res["p99_ms"], "unit": "ms (p99 latency)", "n_gpus": 1,
@@ -273,6 +277,74 @@ def bench_block(args):
     if serving is not None:
         out["serving"] = serving
     out["input_construction"] = athena
+    print(json.dumps(out), flush=True)
+    ctx.close()
+
+
+def bench_files(args):
+    """A block solved and written as the reference's three output files, in
+    a scratch directory under $TMPDIR: oco_solve_block_files (chunked solve,
+    copies and writes overlapped) against solving to host arrays and then
+    writing the three records, and against writing the same bytes alone
+    (the disk's rate, the ceiling of either)."""
+    import shutil
+    import tempfile
+    import ocotillo_amd as oco
+    from ocotillo_amd import workloads
+    dims = dict(nx=args.nx or 64, ny=args.ny or 64, nz=args.nz or 256, nw=args.nw or 128)
+    rho, T, dz, waves = workloads.config2(**dims)
+    nx, ny, nz = rho.shape
+    nw = waves.size
+    mode = oco.MODE_MIXED if args.mode == "mixed" else oco.MODE_FP64
+    out_bytes = nw * nx * ny * (3 * nz + 6) * 8
+    ctx = oco.Context(0)
+    ctx.set_wavelengths(waves)
+    ctx.set_physics(2e4, False, 0.4)
+    work = tempfile.mkdtemp(prefix="oco_files_")
+    reps = max(1, min(args.steps, 3))
+    try:
+        log("block %dx%dx%d nw=%d: %.2f GB of outputs, %d reps, in %s" % (nx, ny, nz, nw, out_bytes / 1e9, reps, work))
+        ctx.solve_block_files(rho, T, dz, work, "warm", mode=mode)
+        t = []
+        for i in range(reps):
+            t0 = time.perf_counter()
+            _, _, d = ctx.solve_block_files(rho, T, dz, work, "%04d" % i, mode=mode)
+            t.append(time.perf_counter() - t0)
+            log("streamed %d: %.3f s (kernels %.1f ms)" % (i, t[-1], d.cells_ms + d.sweep_ms))
+        streamed = min(t)
+        t = []
+        for i in range(reps):
+            t0 = time.perf_counter()
+            U, V, A, d = ctx.solve_block(rho, T, dz, mode=mode)
+            t1 = time.perf_counter()
+            for name, arr in (("mean_intensity", U), ("absorption_coefficients", A), ("flux", V)):
+                oco.write_record(os.path.join(work, "%s_s%04d.bin" % (name, i)), arr)
+            t.append(time.perf_counter() - t0)
+            log("solve then write %d: %.3f s (solve and copy back %.3f s)" % (i, t[-1], t1 - t0))
+        serial = min(t)
+        t = []
+        for i in range(reps):
+            t0 = time.perf_counter()
+            for name, arr in (("mean_intensity", U), ("absorption_coefficients", A), ("flux", V)):
+                oco.write_record(os.path.join(work, "%s_w%04d.bin" % (name, i)), arr)
+            t.append(time.perf_counter() - t0)
+        write_only = min(t)
+        log("write only: %.3f s" % write_only)
+    finally:
+        shutil.rmtree(work, ignore_errors=True)
+    points = float(nx * ny * nz * nw)
+    out = {"metric": "block solved and written to its output files", "value": round(points / streamed, 1),
+           "unit": "points/s", "n_gpus": 1, "steps": reps, "warmup": 1, "ms_per_step": round(streamed * 1e3, 2),
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+           "dtype": "f64" if mode == oco.MODE_FP64 else "f64+f32", "data": "synthetic",
+           "config": {"workload": "FP64 drop-in block %dx%d columns, nz=%d, nw=%d, files in $TMPDIR" % (nx, ny, nz, nw),
+                      "output_bytes": out_bytes},
+           "files": {"streamed_s": round(streamed, 4), "solve_then_write_s": round(serial, 4),
+                     "write_only_s": round(write_only, 4),
+                     "streamed_GBps": round(out_bytes / streamed / 1e9, 3),
+                     "write_only_GBps": round(out_bytes / write_only / 1e9, 3),
+                     "streamed_vs_write_only": round(write_only / streamed, 3)},
+           "roofline": None, "cpu_baseline": None}
     print(json.dumps(out), flush=True)
     ctx.close()
 
