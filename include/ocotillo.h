@@ -95,6 +95,21 @@ int oco_solve_block_device(oco_ctx* ctx, int nz, int nyloc, int nxloc, double dz
                            double dz2, const double* d_rho3d, const double* d_T3d, double* d_U,
                            double* d_V, double* d_absorp, int mode, oco_diag* diag);
 
+/* ---- Opacity query ----
+ * Absorption coefficient alpha (1/cm), single-scattering albedo and source
+ * function S of npts (rho, T) points at the context's wavelengths, in `mode`
+ * precision: wavelength_independent_pillars then calc_opacity_and_albedo per
+ * point (source/gas_state.f90:30-110, source/continuous_opacity.f90:143-341,
+ * 392-409), or grey_parameters (:377-390) on a grey context.  This is the
+ * query tools/ocotillo-py/solve_opacity.py:63-226 answers in Python, with the
+ * Fortran code's semantics instead of the Python's (its divergences: the
+ * Saha mass and threshold, k, the H bound-free sum to n = 5, the clamp of
+ * negative H- bound-free, astropy constants, a division by zero when fully
+ * ionised).  Host arrays; outputs (nw, npts) in Fortran order (wavelength
+ * fastest); albedo and S may be NULL. */
+int oco_opacity_points(oco_ctx* ctx, int npts, const double* rho, const double* T, int mode, double* alpha,
+                       double* albedo, double* S);
+
 /* ---- Block outputs ----
  * One block solved and written as the reference's three output files
  * (source/io.f90:76-101, output_binary):

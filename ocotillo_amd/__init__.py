@@ -29,7 +29,7 @@ MODE_FP64, MODE_MIXED = 0, 1
 # every symbol include/ocotillo.h declares
 EXPORTS = ["oco_create", "oco_destroy", "oco_set_wavelengths", "oco_set_physics", "oco_solve_block",
            "oco_solve_block_device", "oco_strerror", "oco_version", "oco_athena_slab", "oco_athena_slab_device",
-           "oco_solve_block_files", "oco_write_record",
+           "oco_solve_block_files", "oco_write_record", "oco_opacity_points",
            "oco_batch_create", "oco_batch_destroy", "oco_batch_layout", "oco_batch_buffers", "oco_batch_run",
            "oco_batch_run_host", "oco_batch_tuning",
            "oco_server_create", "oco_server_submit", "oco_server_wait", "oco_server_poll", "oco_server_stats",
@@ -102,6 +102,7 @@ def load_library(path=None):
                                          _DP, _DP, C.c_int, C.c_char_p, C.c_char_p, C.c_size_t, _DP, _DP,
                                          C.POINTER(Diag)])
     L.oco_write_record.argtypes = [C.c_char_p, C.c_void_p, C.c_size_t]
+    L.oco_opacity_points.argtypes = [C.c_void_p, C.c_int, _DP, _DP, C.c_int, _DP, _DP, _DP]
     L.oco_athena_slab.argtypes = ([C.c_void_p] + [C.c_int] * 5 + [_DP] * 6 + [C.POINTER(AthenaUnits)]
                                   + [_DP] * 2)
     L.oco_athena_slab_device.argtypes = ([C.c_void_p] + [C.c_int] * 5 + [C.c_void_p] * 6
@@ -225,6 +226,20 @@ class Context:
             raise OcoError(st, "oco_solve_block_device: " + strerror(st), d)
         return d
 
+
+    def opacity(self, rho, T, mode=MODE_FP64):
+        """Absorption coefficient (1/cm), albedo and source function of (rho,
+        T) points at the context's wavelengths (oco_opacity_points): arrays
+        [npts, nw]."""
+        rho = np.ascontiguousarray(np.atleast_1d(rho), dtype=np.float64).reshape(-1)
+        T = np.ascontiguousarray(np.atleast_1d(T), dtype=np.float64).reshape(-1)
+        if rho.shape != T.shape:
+            raise OcoError(E_ARG, "rho and T must have the same number of points")
+        out = [np.empty((rho.size, self.nw)) for _ in range(3)]
+        st = self.L.oco_opacity_points(self.h, rho.size, _ptr(rho), _ptr(T), mode, *[_ptr(o) for o in out])
+        if st != OK:
+            raise OcoError(st, "oco_opacity_points: " + strerror(st))
+        return tuple(out)
 
     def solve_block_files(self, rho3d, T3d, dz, outdir, snapshot="0000", dz1=None, dz2=None, mode=MODE_FP64,
                           stage_bytes=0, raise_on_pivot=True):

@@ -466,6 +466,51 @@ int oco_solve_block(oco_ctx* c, int nz, int nyloc, int nxloc, double dz, double 
 }
 
 /* ---------------------------------------------------------------------- */
+/* Opacity query                                                           */
+/* ---------------------------------------------------------------------- */
+
+int oco_opacity_points(oco_ctx* c, int npts, const double* rho, const double* T, int mode, double* alpha,
+                       double* albedo, double* S)
+{
+    if (!c || npts < 1 || !rho || !T || !alpha) return OCO_E_ARG;
+    if (mode != OCO_MODE_FP64 && mode != OCO_MODE_MIXED) return OCO_E_ARG;
+    if (c->nw <= 0) return OCO_E_STATE;
+    for (int i = 0; i < npts; i++)
+        if (!(rho[i] > 0.0) || !(T[i] > 0.0)) return OCO_E_ARG;
+    HIPCHK(hipSetDevice(c->device));
+    const size_t n = (size_t)npts * c->nw;
+    int st;
+    if ((st = ensure(c, c->rho, (size_t)npts * sizeof(double)))) return st;
+    if ((st = ensure(c, c->T, (size_t)npts * sizeof(double)))) return st;
+    if (!c->lgrey && (st = ensure(c, c->cells, (size_t)npts * OCO_CELL_STRIDE * sizeof(double)))) return st;
+    if ((st = ensure(c, c->U, n * sizeof(double)))) return st;
+    if (albedo && (st = ensure(c, c->V, n * sizeof(double)))) return st;
+    if (S && (st = ensure(c, c->A, n * sizeof(double)))) return st;
+    HIPCHK(hipMemcpyAsync(c->rho.p, rho, (size_t)npts * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(c->T.p, T, (size_t)npts * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    if (!c->lgrey)
+        HIPCHK(oco_launch_cells((const double*)c->rho.p, (const double*)c->T.p, npts, c->fully_ionized_T,
+                                (double*)c->cells.p, 0, c->stream));
+    oco_points_args a;
+    a.npts = npts;
+    a.nw = c->nw;
+    a.rho = (const double*)c->rho.p;
+    a.T = (const double*)c->T.p;
+    a.cells = (const double*)c->cells.p;
+    a.wtab = (const double*)c->wtab.p;
+    a.sigma_grey = c->sigma_grey;
+    a.alpha = (double*)c->U.p;
+    a.albedo = albedo ? (double*)c->V.p : nullptr;
+    a.S = S ? (double*)c->A.p : nullptr;
+    HIPCHK(oco_launch_points(a, c->lgrey, mode == OCO_MODE_MIXED, c->stream));
+    HIPCHK(hipMemcpyAsync(alpha, a.alpha, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    if (albedo) HIPCHK(hipMemcpyAsync(albedo, a.albedo, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    if (S) HIPCHK(hipMemcpyAsync(S, a.S, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return OCO_OK;
+}
+
+/* ---------------------------------------------------------------------- */
 /* Block outputs to files                                                  */
 /* ---------------------------------------------------------------------- */
 

@@ -77,6 +77,20 @@ int oco_record_open(const char* path, unsigned long long nbytes, oco_record_file
 int oco_record_write(oco_record_file* f, unsigned long long off, const void* data, size_t n);
 int oco_record_close(oco_record_file* f, int keep);
 
+/* Opacity query of (point, wavelength) pairs (see opacity_points). */
+struct oco_points_args {
+    int npts, nw;
+    const double* rho;    /* (npts) */
+    const double* T;
+    const double* cells;  /* cell records of the points (non-grey) */
+    const double* wtab;   /* (nw, OCO_WT_STRIDE) */
+    double sigma_grey;
+    double* alpha;        /* (nw, npts) Fortran order: wavelength fastest */
+    double* albedo;       /* may be null */
+    double* S;            /* may be null */
+};
+hipError_t oco_launch_points(const oco_points_args& a, int lgrey, int mixed, hipStream_t stream);
+
 /* Athena slab to the block's rho, T (see athena_slab in oco_kernels.hip). */
 struct oco_athena_args {
     int nz, nyloc, nxloc, nzl, iz0;

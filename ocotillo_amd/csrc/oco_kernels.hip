@@ -1245,6 +1245,52 @@ hipError_t oco_launch_athena(const oco_athena_args& a, hipStream_t stream)
     return hipGetLastError();
 }
 
+/* Opacity query: alpha, albedo, S of (point, wavelength) pairs, one lane
+ * each, wavelength fastest in the outputs (coalesced stores).  Non-grey:
+ * the point's cell record (cells_fp64) and the wavelength's table row, the
+ * sweeps' per-point code (oco_point / oco_point_mixed); grey: the grey
+ * parameters of the point's rho, T. */
+template <bool MIX, bool GREY>
+__global__ __launch_bounds__(256) void opacity_points(const oco_points_args a)
+{
+    __shared__ oco_mtab smt;
+    if (!MIX || GREY) load_mtab(&smt);
+    const long n = (long)a.npts * a.nw;
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+        const long pt = i / a.nw;
+        const int iw = (int)(i - pt * a.nw);
+        oco_pt p;
+        if (GREY) {
+            p = oco_grey_point(a.rho[pt], a.T[pt], a.sigma_grey, &smt);
+        } else {
+            double c[CR_NFIELD], w[OCO_WT_STRIDE];
+#pragma unroll
+            for (int k = 0; k < CR_NFIELD; k++) c[k] = a.cells[pt * OCO_CELL_STRIDE + k];
+#pragma unroll
+            for (int k = 0; k < OCO_WT_STRIDE; k++) w[k] = a.wtab[(long)iw * OCO_WT_STRIDE + k];
+            if (MIX) p = oco_point_mixed<1>(c, w);
+            else p = oco_point(c, w, &smt);
+        }
+        a.alpha[i] = p.alpha;
+        if (a.albedo) a.albedo[i] = p.albedo;
+        if (a.S) a.S[i] = p.S;
+    }
+}
+
+hipError_t oco_launch_points(const oco_points_args& a, int lgrey, int mixed, hipStream_t stream)
+{
+    (void)hipGetLastError();
+    const long n = (long)a.npts * a.nw;
+    if (n <= 0) return hipSuccess;
+    long blocks = (n + 255) / 256;
+    if (blocks > 65536) blocks = 65536;
+    const dim3 g((unsigned)blocks), b(256);
+    if (lgrey) hipLaunchKernelGGL((opacity_points<false, true>), g, b, 0, stream, a);
+    else if (mixed) hipLaunchKernelGGL((opacity_points<true, false>), g, b, 0, stream, a);
+    else hipLaunchKernelGGL((opacity_points<false, false>), g, b, 0, stream, a);
+    return hipGetLastError();
+}
+
 hipError_t oco_launch_cells(const double* rho, const double* T, long ncells, double fully_ionized_T,
                             double* cells, int soa_nz, hipStream_t stream)
 {
