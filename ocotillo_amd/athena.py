@@ -156,6 +156,24 @@ class SlabHeader:
         return [mm[i] for i in range(5)]
 
 
+def write_slab(path, fields, x, y, z, gamma1, coordsys=-1, header_ints=(5, 0, 0, 0), cs=0.0, t=0.0, dt=0.0):
+    """An Athena .bin file in the layout SlabHeader reads: `fields` are the
+    five C-ordered [nzl, nyloc, nxloc] arrays (rho, Mx, My, Mz, E)."""
+    f0 = np.asarray(fields[0])
+    nzl, nyloc, nxloc = f0.shape
+    if len(fields) != 5 or any(np.shape(a) != f0.shape for a in fields):
+        raise ValueError("five fields of one shape are needed")
+    if (len(x), len(y), len(z)) != (nxloc, nyloc, nzl):
+        raise ValueError("coordinates do not match the fields")
+    with open(path, "wb") as f:
+        np.array([coordsys, nxloc, nyloc, nzl] + list(header_ints), "<i4").tofile(f)
+        np.array([gamma1, cs, t, dt], "<f8").tofile(f)
+        for a in (x, y, z):
+            np.asarray(a, "<f8").tofile(f)
+        for a in fields:
+            np.ascontiguousarray(a, "<f8").tofile(f)
+
+
 class AthenaRun:
     """A run directory laid out as the reference expects (input.in,
     resolution.in, <datadir>/id<iproc>/<RunName>[-id<iproc>].<snap>.bin)."""
@@ -179,6 +197,14 @@ class AthenaRun:
         self.floor_temp = float(t["floortemp"])
         self.switch_temp = (1 + EPSI) * float(t["isotemp"])   # source/disk.f90:28
         self._tgauss = None
+
+    def snapshots(self):
+        """Tags of the snapshots present for process 0, sorted."""
+        d = os.path.join(self.dir, self.datadir, "id0")
+        pre, suf = self.run_name + ".", ".bin"
+        if not os.path.isdir(d):
+            return []
+        return sorted(f[len(pre):-len(suf)] for f in os.listdir(d) if f.startswith(pre) and f.endswith(suf))
 
     def slab_path(self, iprocx, iprocy, iprocz, snapshot="0000"):
         """source/read_athena.f90:102-117."""

@@ -41,3 +41,32 @@ def read_block(outdir, iprocx, iprocy, snapshot, nw, nxloc, nyloc, nz):
     A = read_array("%s/absorption_coefficients_%s.bin" % (d, snapshot), (nw, nxloc, nyloc, nz))
     V = read_array("%s/flux_%s.bin" % (d, snapshot), (nw, nxloc, nyloc, nz))
     return U, A, V
+
+
+def format_e(x, w=15, d=5):
+    """Fortran Ew.d output of one value: 0.ddddd scaled, exponent E+xx, or
+    +xxx (no E) when it needs three digits, right-justified in w columns."""
+    x = float(x)
+    if x == 0.0:
+        digits, e = "0" * d, 0
+    else:
+        s = "%.*e" % (d - 1, abs(x))            # d.ddddde+XX, correctly rounded
+        mant, ex = s.split("e")
+        digits = mant.replace(".", "")
+        e = int(ex) + 1
+    body = "0." + digits + ("E%+03d" % e if -99 <= e <= 99 else "%+04d" % e)
+    if x < 0 or (x == 0.0 and np.signbit(x)):
+        body = "-" + body
+    return body.rjust(w)
+
+
+def diagnostics_text(U_col, A_col):
+    """diagnostics.txt (source/io.f90:31-46): for each cell i, I6 then the nw
+    mean intensities U(n1+i-1, :) and the nw absorption coefficients, each
+    E15.5.  U_col is [nw, nz+6], A_col [nw, nz]."""
+    nw, nz = A_col.shape
+    lines = []
+    for i in range(nz):
+        vals = [U_col[iw, 3 + i] for iw in range(nw)] + [A_col[iw, i] for iw in range(nw)]
+        lines.append("%6d" % (i + 1) + "".join(format_e(v) for v in vals))
+    return "\n".join(lines) + "\n"

@@ -12,7 +12,10 @@
 !     source/common.f90:9), so one executable serves every shape;
 !   * optional namelist &oco_input: lwave_log (log-spaced wavelengths,
 !     default .false. = the reference's linear grid), oco_mode (0 FP64,
-!     1 mixed), oco_device, hbm_budget_gb.
+!     1 mixed), oco_device, hbm_budget_gb;
+!   * several snapshot tags or ranges a:b on the command line run each
+!     snapshot as its own run would, into output_snapshot<tag>/ (what
+!     samples/multiInTest/snapshot_looper.sh meant to do), in one process.
 
 !------------------------------------------------------------------------------
 module oco_shapes
@@ -241,6 +244,12 @@ contains
       z(n2 + i) = z(n2) + i * dz
     enddo
   end subroutine calc_grid
+
+  ! Forget the kept profile: each snapshot of a multi-snapshot run starts
+  ! as the reference's separate process would.
+  subroutine reset_profile()
+    if (allocated(Tgauss)) deallocate(Tgauss)
+  end subroutine reset_profile
 
   ! The Gaussian temperature profile, built on the first call and kept
   ! (source/disk.f90:87-90).
@@ -473,13 +482,16 @@ module oco_output
   use oco_capi, only: oco_write_record, oco_message, OCO_OK
   use, intrinsic :: iso_c_binding, only: c_int, c_size_t, c_null_char
   implicit none
+  ! directory of the run's outputs: output/ (the reference's) for one
+  ! snapshot, output_snapshot<tag>/ per snapshot when several are given
+  character(len=256) :: outroot = 'output'
 contains
   subroutine output_ascii(U, absorp)
     real(8), intent(in) :: U(mz, nw), absorp(nz, nw)
     character(len=32) :: fmt
     integer :: i, iw, u10
     write(fmt, '("(I6,",I0,"(2e15.5))")') nw
-    open(newunit=u10, file="output/diagnostics.txt", status="replace", action='write')
+    open(newunit=u10, file=trim(outroot) // "/diagnostics.txt", status="replace", action='write')
     do i = 1, nz
       write(u10, fmt) i, (U(n1 + i - 1, iw), iw = 1, nw), (absorp(i, iw), iw = 1, nw)
     enddo
@@ -489,18 +501,18 @@ contains
   subroutine output_grid(z, waves)
     real(8), intent(in) :: z(mz), waves(nw)
     integer :: i, u
-    open(newunit=u, file="output/zgrid.txt", status="replace", action='write')
+    open(newunit=u, file=trim(outroot) // "/zgrid.txt", status="replace", action='write')
     do i = 1, nz
       write(u, *) i, z(n1 + i - 1)
     enddo
     close(u)
-    call write_record("output/zgrid.bin", z, size(z))
-    open(newunit=u, file="output/wavegrid.txt", status="replace", action="write")
+    call write_record(trim(outroot) // "/zgrid.bin", z, size(z))
+    open(newunit=u, file=trim(outroot) // "/wavegrid.txt", status="replace", action="write")
     do i = 1, nw
       write(u, *) i, waves(i)
     enddo
     close(u)
-    call write_record("output/wavegrid.bin", waves, size(waves))
+    call write_record(trim(outroot) // "/wavegrid.bin", waves, size(waves))
   end subroutine output_grid
 
   ! One array as a one-record file in the reference's (gfortran's) layout.
@@ -536,8 +548,10 @@ program flux_feautrier_gpu
   logical :: lgrey = .false., lread_athena = .true., lwave_log = .false.
   integer :: oco_mode = 0, oco_device = 0, iprocx, iprocy, ix, iy, i, err, iu
   integer(c_int) :: st
-  character(len=90) :: snapshot, inputfile = './input.in'
+  character(len=90) :: snapshot, inputfile = './input.in', arg
+  character(len=90), allocatable :: snaps(:)
   character(len=256) :: outdir
+  integer :: isnap, nsnap
   type(c_ptr) :: ctx
   type(oco_diag) :: diag
   real(8) :: loop_wall = 0, dev_ms = 0
@@ -548,6 +562,9 @@ program flux_feautrier_gpu
   call cpu_time(start)
   call get_command_argument(1, snapshot)
   if (snapshot == '') snapshot = "0000"
+  if (snapshot /= '--print-config') call snapshot_list(snaps)
+  nsnap = 1
+  if (allocated(snaps)) nsnap = size(snaps)
 
   call read_resolution('./resolution.in')
   open(newunit=iu, file=trim(inputfile))
@@ -564,7 +581,6 @@ program flux_feautrier_gpu
     print '(A,4ES24.16)', 'z0 z1 w0 w1:', z0, z1, w0, w1
     stop
   endif
-  if (lread_athena) print*, "snapshot: ", snapshot
   if (lgrey .and. (nw /= 1)) then
     print*, "For Grey RT use only one wavelength. Switch nw=1 in resolution.in"
     stop
@@ -616,6 +632,17 @@ program flux_feautrier_gpu
 
   if (lread_athena) call read_athena_input(inputfile)
 
+  ! one snapshot: the reference's run; several (the replacement of
+  ! samples/multiInTest/snapshot_looper.sh): each as its own run would be,
+  ! into output_snapshot<tag>/, with one context and resident tables
+  do isnap = 1, nsnap
+  snapshot = snaps(isnap)
+  if (nsnap > 1) then
+    outroot = 'output_snapshot' // trim(snapshot)
+    call execute_command_line('mkdir -p ' // trim(outroot))
+    call reset_profile()
+  endif
+  if (lread_athena) print*, "snapshot: ", snapshot
   do iprocx = 0, nprocx - 1
     do iprocy = 0, nprocy - 1
       lroot = (iprocx == 0) .and. (iprocy == 0)
@@ -637,7 +664,7 @@ program flux_feautrier_gpu
         dz2 = dz**2
       endif
       if (lroot) call cpu_time(start_loop)
-      outdir = 'output/procx' // trim(itoa(iprocx)) // '_procy' // trim(itoa(iprocy))
+      outdir = trim(outroot) // '/procx' // trim(itoa(iprocx)) // '_procy' // trim(itoa(iprocy))
       call execute_command_line('mkdir -p ' // trim(outdir))
       call system_clock(c0, crate)
       st = oco_solve_block_files(ctx, int(nz, c_int), int(nyloc, c_int), int(nxloc, c_int), dz, dz1, dz2, &
@@ -669,6 +696,8 @@ program flux_feautrier_gpu
       endif
     enddo
   enddo
+  if (nsnap > 1) print*, "snapshot ", trim(snapshot), " done: ", trim(outroot)
+  enddo
   st = oco_destroy(ctx)
 
   call cpu_time(finish)
@@ -681,4 +710,46 @@ program flux_feautrier_gpu
   print*, "Wall time = ", finish - start, " seconds."
   print*, "Execution time =", (finish - start) / (1d0 * nz * ny * nx * nw) * 1e6, &
        " micro-seconds per wavelength point per mesh point."
+contains
+
+  ! Snapshot tags from the command line: none -> 0000; each argument a tag
+  ! or a range a:b of snapshot numbers (tags zero-padded to four digits, as
+  ! Athena names its dumps).
+  subroutine snapshot_list(tags)
+    character(len=90), allocatable, intent(out) :: tags(:)
+    character(len=90), allocatable :: tmp(:)
+    integer :: iarg, narg, n, a0, a1, k, colon, ios
+    narg = command_argument_count()
+    allocate(tags(0))
+    if (narg == 0) then
+      tags = [character(len=90) :: "0000"]
+      return
+    endif
+    do iarg = 1, narg
+      call get_command_argument(iarg, arg)
+      colon = index(arg, ':')
+      if (colon > 0) then
+        read(arg(1:colon - 1), *, iostat=ios) a0
+        if (ios == 0) read(arg(colon + 1:), *, iostat=ios) a1
+        if (ios /= 0 .or. a1 < a0 .or. a0 < 0) then
+          print*, "bad snapshot range: ", trim(arg)
+          stop 1
+        endif
+        n = size(tags)
+        allocate(tmp(n + a1 - a0 + 1))
+        tmp(1:n) = tags
+        do k = a0, a1
+          if (k < 10000) then
+            write(tmp(n + k - a0 + 1), '(I4.4)') k
+          else
+            write(tmp(n + k - a0 + 1), '(I0)') k
+          endif
+        enddo
+        call move_alloc(tmp, tags)
+      else
+        tags = [tags, arg]
+      endif
+    enddo
+  end subroutine snapshot_list
+
 end program flux_feautrier_gpu

@@ -74,6 +74,8 @@ def lib(portable=False):
         L.oo_calc_wavelength.argtypes = [C.c_double, C.c_double, C.c_int, DP]
         L.oo_calc_grid.argtypes = [C.c_int, C.c_double, C.c_double, DP, C.POINTER(C.c_double)]
         L.oo_calc_density.argtypes = [C.c_int, DP, C.c_double, C.c_double, C.c_double, DP]
+        L.oo_calc_temperature.argtypes = [C.c_int, DP, C.c_double, C.c_double, C.c_double, C.c_double,
+                                          C.c_void_p, C.c_int, DP]
         L.oo_athena_units.argtypes = [C.c_double] * 5 + [C.POINTER(C.c_double)] * 3
         L.oo_athena_slab_raw.argtypes = [C.c_int] * 5 + [DP] * 5 + [C.c_int, DP, DP]
         L.oo_athena_postprocess.argtypes = ([C.c_int] * 3 + [C.c_double] * 4 + [DP, C.c_double, C.c_double]
