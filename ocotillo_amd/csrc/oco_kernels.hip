@@ -28,6 +28,7 @@
 __constant__ oco_mtab oco_mtab_dev = OCO_MTAB_INIT;
 
 #define OCO_WAVE 64
+#define OCO_XCDS 8   /* accelerator complexes (each with its own L2) of an MI355X */
 #ifndef OCO_BACK_CHUNK
 #define OCO_BACK_CHUNK 2   /* back-substitution rows whose scratch loads are batched */
 #endif
@@ -166,6 +167,10 @@ __device__ static inline void stage_put(double* buf, int idx, double v)
     buf[threadIdx.x * (R + 1) + (idx & (R - 1))] = v;
 }
 
+/* Outputs are written once and not read back by the kernel: nontemporal
+ * stores keep them from displacing the scratch and cell records in L2. */
+__device__ static inline void out_store(double* p, double v) { __builtin_nontemporal_store(v, p); }
+
 /* write block b (elements b*R .. b*R+R-1, clipped to [0, len)) of every lane:
  * thread t handles element r = t mod R of lanes t/R + k*(64/R), k = 0..R-1,
  * so R consecutive threads store one lane's run */
@@ -183,7 +188,7 @@ __device__ static inline void stage_flush(const double* buf, double* const* base
     for (int k = 0; k < R; k++) {
         double* p = pp[k * L];
         double v = bp[k * L * (R + 1)];
-        if (p != nullptr && inb) p[idx] = v;
+        if (p != nullptr && inb) out_store(p + idx, v);
     }
 }
 
@@ -743,16 +748,19 @@ __global__ __launch_bounds__(256) void canary_fp64(const oco_sweep_args a, doubl
  *      and V, with every output store coalesced.
  * LDS: alpha [nz], albedo [nz], S -> U [nz + 6]; the chunk's D, W, Z stay in
  * registers (MM rows at most). */
-__device__ static inline double shfl_up_id(double v, int s, double id)
-{
-    double r = __shfl_up(v, (unsigned)s, OCO_WAVE);
-    return (int)threadIdx.x >= s ? r : id;
-}
+/* Word 3 of a raw buffer descriptor on gfx9-family GPUs (32-bit data
+ * format; the layout of the public ISA documentation). */
+#define OCO_BUF_DWORD3 0x00020000
 
-__device__ static inline double shfl_down_id(double v, int s, double id)
+/* v of lane (byte address addr / 4) of the wavefront: a rotation when addr
+ * is ((lane -+ s) & 63) * 4. */
+__device__ static inline double lane_get(double v, int addr)
 {
-    double r = __shfl_down(v, (unsigned)s, OCO_WAVE);
-    return (int)threadIdx.x + s < OCO_WAVE ? r : id;
+    const int2 w = __builtin_bit_cast(int2, v);
+    int2 r;
+    r.x = __builtin_amdgcn_ds_bpermute(addr, w.x);
+    r.y = __builtin_amdgcn_ds_bpermute(addr, w.y);
+    return __builtin_bit_cast(double, r);
 }
 
 /* rows per lane of the wavefront sweep for columns of nz cells */
@@ -777,16 +785,24 @@ __device__ static inline void solve_system_wave(const oco_sys& d, const double* 
     const int z1 = z0 + m < nz ? z0 + m : nz;
 
     wave_sync();   /* the previous system's last reads are done */
-    /* 1. points (source/continuous_opacity.f90:175-201) */
+    /* 1. points (source/continuous_opacity.f90:175-201); the column's
+     * records are read through a buffer descriptor: the field offset is a
+     * scalar, the lane offset a vector register, no per-field address
+     * arithmetic (loads past the column's records return 0) */
+    const __amdgpu_buffer_rsrc_t crs =
+        __builtin_amdgcn_make_buffer_rsrc((void*)cc, (short)0, CR_NFIELD * nz * (int)sizeof(double), OCO_BUF_DWORD3);
     for (int z = lane; z < nz; z += OCO_WAVE) {
         double rec[CR_NFIELD];
+        const int zo = z * (int)sizeof(double);
 #pragma unroll
-        for (int k = 0; k < CR_NFIELD; k++) rec[k] = cc[k * nz + z];
+        for (int k = 0; k < CR_NFIELD; k++)
+            rec[k] = __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(crs, zo,
+                                                                                    k * nz * (int)sizeof(double), 0));
         oco_pt p = oco_point_mixed<1>(rec, w);
         LA[z] = p.alpha;
         LW[z] = p.albedo;
         LS[z + 3] = p.S;
-        d.A[z] = p.alpha;
+        out_store(d.A + z, p.alpha);
     }
     wave_sync();
     /* 2. rows (source/auxiliary.f90:147-197) in difference form and the
@@ -795,6 +811,10 @@ __device__ static inline void solve_system_wave(const oco_sys& d, const double* 
     double Wp = 1.0, Zp = 0.0, Ep = 0.0;            /* the parameter row x = xL */
     double pr = 1.0, sP = 0.0, sD = 0.0, sE = 0.0;  /* first row: pr G + sP xL + sD, deficit sE */
     double DL = 0.0, WL = 0.0, ZL = 0.0, EL = 0.0;  /* last row */
+    /* 1 / mean alpha of the interface below the current row, shared by this
+     * row's A and the previous row's C (the same sum, so the same bits as
+     * rh_interior_row) */
+    double rlo = z0 > 0 && z0 < z1 ? mx_rcp(0.5 * (LA[z0] + LA[z0 - 1])) : 0.0;
 #pragma unroll
     for (int j = 0; j < MM; j++) {
         const int z = z0 + j;
@@ -809,8 +829,14 @@ __device__ static inline void solve_system_wave(const oco_sys& d, const double* 
                 rh_boundary_row(pc, dz, dz2, H, R);
                 Ac = z == 0 ? 0.0 : al;
                 Cc = z == 0 ? al : 0.0;
+                if (z == 0) rlo = mx_rcp(0.5 * (LA[1] + al));
             } else {
-                rh_interior_row(LA[z - 1], pc, LA[z + 1], dz2, Ac, Cc, H, R);
+                const double rhi = mx_rcp(0.5 * (LA[z + 1] + al));
+                Ac = al2 * rlo;
+                Cc = al2 * rhi;
+                H = (dz2 * (al2 * al)) * (1.0 - om);
+                R = S * H;
+                rlo = rhi;
             }
             const double r = mx_rcp((Ac * (Wp + Ep) + Cc) + H);   /* 1 - D[i-1] = W + E */
             const double D = Cc * r;
@@ -832,8 +858,14 @@ __device__ static inline void solve_system_wave(const oco_sys& d, const double* 
     /* 3. reduced system in the chunks' last unknowns:
      * -WL G[k-1] + (WL + DL Gn + EL + DL En) G[k] - DL Gn G[k+1] = ZL + DL Dn
      * (Gn, Dn, En: the next chunk's first row) */
-    const double Gn = shfl_down_id(pr, 1, 1.0), Dn = shfl_down_id(sD, 1, 0.0);
-    const double En = shfl_down_id(sE, 1, 0.0);
+    /* Neighbours are read by rotation, without masking the lanes whose
+     * neighbour does not exist: those lanes' couplings are exactly zero (the
+     * first chunk's W, the last row's D, and after each reduction step A = 0
+     * for lanes < s, C = 0 for lanes > 63 - s), and every row's A + C + H is
+     * positive and finite, so the wrapped-around values are multiplied by
+     * zero.  Empty chunks are identity rows. */
+    const int dn1 = ((lane + 1) & (OCO_WAVE - 1)) * 4, up1 = ((lane - 1) & (OCO_WAVE - 1)) * 4;
+    const double Gn = lane_get(pr, dn1), Dn = lane_get(sD, dn1), En = lane_get(sE, dn1);
     double Ar = 0.0, Cr = 0.0, Hr = 1.0, Rr = 0.0;
     if (z1 > z0) {
         Ar = WL;
@@ -844,10 +876,9 @@ __device__ static inline void solve_system_wave(const oco_sys& d, const double* 
     (void)sP;
 #pragma unroll
     for (int sft = 1; sft < OCO_WAVE; sft <<= 1) {
-        const double Am = shfl_up_id(Ar, sft, 0.0), Cm = shfl_up_id(Cr, sft, 0.0);
-        const double Hm = shfl_up_id(Hr, sft, 1.0), Rm = shfl_up_id(Rr, sft, 0.0);
-        const double Ap = shfl_down_id(Ar, sft, 0.0), Cp = shfl_down_id(Cr, sft, 0.0);
-        const double Hp = shfl_down_id(Hr, sft, 1.0), Rp = shfl_down_id(Rr, sft, 0.0);
+        const int up = ((lane - sft) & (OCO_WAVE - 1)) * 4, dn = ((lane + sft) & (OCO_WAVE - 1)) * 4;
+        const double Am = lane_get(Ar, up), Cm = lane_get(Cr, up), Hm = lane_get(Hr, up), Rm = lane_get(Rr, up);
+        const double Ap = lane_get(Ar, dn), Cp = lane_get(Cr, dn), Hp = lane_get(Hr, dn), Rp = lane_get(Rr, dn);
         const double k1 = Ar * mx_rcp((Am + Cm) + Hm);
         const double k2 = Cr * mx_rcp((Ap + Cp) + Hp);
         Ar = k1 * Am;
@@ -856,7 +887,7 @@ __device__ static inline void solve_system_wave(const oco_sys& d, const double* 
         Rr = (Rr + k1 * Rm) + k2 * Rp;
     }
     const double G = Rr * mx_rcp((Ar + Cr) + Hr);
-    const double GL = shfl_up_id(G, 1, 0.0);
+    const double GL = lane_get(G, up1);
     /* 4. back-substitution inside the chunk */
     double x = G;
 #pragma unroll
@@ -882,9 +913,9 @@ __device__ static inline void solve_system_wave(const oco_sys& d, const double* 
     for (int z = lane; z < nz; z += OCO_WAVE) {
         const double* u = LS + z + 3;
         double df = c60 * (((45.0 * (u[1] - u[-1])) - (9.0 * (u[2] - u[-2]))) + (u[3] - u[-3]));
-        d.V[z] = ((-1.0 * mx_rcp(LA[z])) * df) * dz1;
+        out_store(d.V + z, ((-1.0 * mx_rcp(LA[z])) * df) * dz1);
     }
-    for (int e = lane; e < nz + 6; e += OCO_WAVE) d.U[e] = LS[e];
+    for (int e = lane; e < nz + 6; e += OCO_WAVE) out_store(d.U + e, LS[e]);
 }
 
 /* Mixed-precision sweep, one wavefront per system; systems in (column,
@@ -894,10 +925,22 @@ __global__ __launch_bounds__(OCO_WAVE) void sweep_mixed_wave(const oco_sweep_arg
 {
     extern __shared__ double lds[];
     const int nw = a.nw;
-    const long nsys = a.ncol * nw;
-    for (long s = blockIdx.x; s < nsys; s += gridDim.x) {
-        const long col = s / nw;
-        const int iw = (int)(s % nw);
+    /* XCD-aware order: workgroups are dispatched to the 8 XCDs round-robin,
+     * so workgroup b runs on XCD b mod 8.  With a grid that is a multiple of
+     * 8, the workgroups of XCD x take the columns x, x + 8, x + 16, ..., each
+     * column's wavelengths in turn: a column's cell records (22 fields x nz
+     * doubles, read by every one of its systems) are brought into one L2
+     * only and reused there.  Any other grid: plain (column, wavelength)
+     * order. */
+    const bool xcd = gridDim.x % OCO_XCDS == 0;
+    const int x = xcd ? (int)(blockIdx.x % OCO_XCDS) : 0;
+    const long per = xcd ? gridDim.x / OCO_XCDS : gridDim.x;
+    const long first = xcd ? blockIdx.x / OCO_XCDS : blockIdx.x;
+    const long ncx = xcd ? (a.ncol - x + OCO_XCDS - 1) / OCO_XCDS : a.ncol;   /* columns of this XCD */
+    const long nsys = ncx * nw;
+    for (long t = first; t < nsys; t += per) {
+        const long col = xcd ? (t / nw) * OCO_XCDS + x : t / nw;
+        const int iw = (int)(t % nw);
         double w[OCO_WT_STRIDE];
         oco_cptr wp = (oco_cptr)(a.wtab + (long)iw * OCO_WT_STRIDE);
 #pragma unroll
@@ -910,7 +953,7 @@ __global__ __launch_bounds__(OCO_WAVE) void sweep_mixed_wave(const oco_sweep_arg
         d.U = a.U + osys * (a.nz + 6);
         d.V = a.V + osys * a.nz;
         d.A = a.A + osys * a.nz;
-        d.key = s;
+        d.key = col * nw + iw;
         solve_system_wave<MM>(d, a.cells + col * (long)CR_NFIELD * a.nz, w, lds);
     }
 }
@@ -929,15 +972,20 @@ int oco_wave_fits(int nz, int device)
     return nz >= 4 && wave_rows(nz) <= OCO_WAVE_MAX_ROWS && wave_lds_bytes(nz) <= (size_t)lim;
 }
 
-/* Measured on MI355X (one 64x64-column block, nw = 128): the wavefront sweep
- * wins while a lane holds few rows (nz = 256: 2.5 ms vs 3.1 ms); at 9 rows
- * per lane (nz = 512) its cross-lane reduction and the opacity branches that
- * diverge inside a 64-cell group cost more than the lane sweep's scratch
- * traffic. */
-#ifndef OCO_WAVE_AUTO_ROWS
-#define OCO_WAVE_AUTO_ROWS 5
+/* Measured on MI355X on a 128 x 128-column block, nw = 128 (sweep ms, lane / wave):
+ * nz 128 5.1/5.6, 256 10.5/9.3, 384 15.7/13.1, 512 21.1/17.5, 768 33.5/28.5,
+ * 1024 42.4/46.2: the wavefront sweep is chosen for 5 to 13 rows per lane. */
+#ifndef OCO_WAVE_AUTO_MIN_ROWS
+#define OCO_WAVE_AUTO_MIN_ROWS 5
 #endif
-int oco_wave_preferred(int nz) { return wave_rows(nz) <= OCO_WAVE_AUTO_ROWS; }
+#ifndef OCO_WAVE_AUTO_ROWS
+#define OCO_WAVE_AUTO_ROWS 13
+#endif
+int oco_wave_preferred(int nz)
+{
+    const int m = wave_rows(nz);
+    return m >= OCO_WAVE_AUTO_MIN_ROWS && m <= OCO_WAVE_AUTO_ROWS;
+}
 
 static const void* wave_kernel(int nz)
 {
@@ -955,7 +1003,8 @@ int oco_wave_grid(int nz, int device)
         per_cu = 1;
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess || cus < 1)
         cus = 256;
-    return per_cu * cus;
+    const int g = per_cu * cus;
+    return g >= OCO_XCDS ? g / OCO_XCDS * OCO_XCDS : g;   /* a multiple of 8: the XCD-aware order */
 }
 
 /* ---- batched uneven columns ----
