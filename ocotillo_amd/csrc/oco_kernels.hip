@@ -30,7 +30,7 @@ __constant__ oco_mtab oco_mtab_dev = OCO_MTAB_INIT;
 #define OCO_WAVE 64
 #define OCO_XCDS 8   /* accelerator complexes (each with its own L2) of an MI355X */
 #ifndef OCO_BACK_CHUNK
-#define OCO_BACK_CHUNK 2   /* back-substitution rows whose scratch loads are batched */
+#define OCO_BACK_CHUNK 8   /* back-substitution rows whose scratch loads are batched */
 #endif
 
 __device__ static inline void load_mtab(oco_mtab* smt)
@@ -124,31 +124,39 @@ __device__ static inline void record_fail(unsigned long long* fail, long sys, in
  * instruction then covers 64/R whole runs of R consecutive doubles.
  * Every lane of the wavefront takes part in a flush, so lanes without a
  * system of their own run a duplicate one with their stores switched off. */
-/* rows staged per lane: FP64 kernels also hold the elementary-function tables
- * in LDS, so they stage 8 rows (64-byte runs); the mixed kernels stage 16
- * (128-byte runs, whole cache lines) */
+/* rows staged per lane: 16, so each lane's runs are 128 bytes (whole cache
+ * lines).  The FP64 kernels also hold the elementary-function tables in LDS;
+ * with 16 rows their LDS allows 6 wavefronts per CU against 8 at 8 rows, and
+ * they still run faster (measured on a 128 x 128 x 512, nw = 128 block:
+ * 26.8 ms at 8 rows, 24.7 ms at 16 rows with 8 rows of scratch prefetch). */
 #ifndef OCO_STAGE_ROWS
-#define OCO_STAGE_ROWS 8
+#define OCO_STAGE_ROWS 16
 #endif
 #ifndef OCO_STAGE_ROWS_MIXED
 #define OCO_STAGE_ROWS_MIXED 16
 #endif
-template <int R>
+template <int R, bool NT = true>
 struct oco_stage_t {
     static constexpr int rows = R;
     static constexpr bool emergent = false;
+    static constexpr bool nt = NT;   /* outputs by nontemporal stores */
     double ub[OCO_WAVE * (R + 1)];   /* U rows; alpha rows during the forward pass */
     double vb[OCO_WAVE * (R + 1)];   /* V rows */
     double* base[3][OCO_WAVE];       /* per lane: U, V, alpha; null = lane stores nothing */
 };
 typedef oco_stage_t<OCO_STAGE_ROWS> oco_stage;
 typedef oco_stage_t<OCO_STAGE_ROWS_MIXED> oco_stage_mixed;
+/* The batched kernels keep ordinary stores (their columns' ragged nz and nw
+ * leave many partial lines, which the L2 combines) and the FP64 one 8 rows. */
+typedef oco_stage_t<8, false> oco_stage_batch;
+typedef oco_stage_t<OCO_STAGE_ROWS_MIXED, false> oco_stage_batch_mixed;
 
 /* Output policy of the serving kernels: only the emergent flux V(1), V(nz)
  * of each system is stored (d.V[0], d.V[1]); U and alpha are not written. */
 struct oco_stage_emergent {
     static constexpr int rows = 8;
     static constexpr bool emergent = true;
+    static constexpr bool nt = false;
     double ub[1], vb[1];
     double* base[3][1];
 };
@@ -169,12 +177,17 @@ __device__ static inline void stage_put(double* buf, int idx, double v)
 
 /* Outputs are written once and not read back by the kernel: nontemporal
  * stores keep them from displacing the scratch and cell records in L2. */
-__device__ static inline void out_store(double* p, double v) { __builtin_nontemporal_store(v, p); }
+template <bool NT = true>
+__device__ static inline void out_store(double* p, double v)
+{
+    if constexpr (NT) __builtin_nontemporal_store(v, p);
+    else *p = v;
+}
 
 /* write block b (elements b*R .. b*R+R-1, clipped to [0, len)) of every lane:
  * thread t handles element r = t mod R of lanes t/R + k*(64/R), k = 0..R-1,
  * so R consecutive threads store one lane's run */
-template <int R>
+template <int R, bool NT = true>
 __device__ static inline void stage_flush(const double* buf, double* const* base, int b, int len)
 {
     constexpr int L = OCO_WAVE / R;
@@ -188,7 +201,7 @@ __device__ static inline void stage_flush(const double* buf, double* const* base
     for (int k = 0; k < R; k++) {
         double* p = pp[k * L];
         double v = bp[k * L * (R + 1)];
-        if (p != nullptr && inb) out_store(p + idx, v);
+        if (p != nullptr && inb) out_store<NT>(p + idx, v);
     }
 }
 
@@ -397,7 +410,7 @@ __device__ static inline void solve_system(const oco_sys& d, bool valid, const d
             stage_put<R>(S->ub, j, pB.alpha);
             if ((j & (R - 1)) == R - 1) {
                 wave_sync();
-                stage_flush<R>(S->ub, S->base[2], j / R, nz);
+                stage_flush<R, STAGE::nt>(S->ub, S->base[2], j / R, nz);
                 wave_sync();
             }
         }
@@ -433,7 +446,7 @@ __device__ static inline void solve_system(const oco_sys& d, bool valid, const d
             stage_put<R>(S->ub, j, pB.alpha);
             if ((j & (R - 1)) == R - 1) {
                 wave_sync();
-                stage_flush<R>(S->ub, S->base[2], j / R, nz);
+                stage_flush<R, STAGE::nt>(S->ub, S->base[2], j / R, nz);
                 wave_sync();
             }
         }
@@ -459,7 +472,7 @@ __device__ static inline void solve_system(const oco_sys& d, bool valid, const d
         if constexpr (!EM) {
             stage_put<R>(S->ub, j, pC.alpha);
             wave_sync();
-            stage_flush<R>(S->ub, S->base[2], j / R, nz);   /* last (possibly partial) block */
+            stage_flush<R, STAGE::nt>(S->ub, S->base[2], j / R, nz);   /* last (possibly partial) block */
         }
     }
     (void)om;
@@ -508,7 +521,7 @@ __device__ static inline void solve_system(const oco_sys& d, bool valid, const d
             stage_put<R>(S->ub, e, top[k]);
             if ((e & (R - 1)) == 0) {
                 wave_sync();
-                stage_flush<R>(S->ub, S->base[0], e / R, nz + 6);
+                stage_flush<R, STAGE::nt>(S->ub, S->base[0], e / R, nz + 6);
                 wave_sync();
             }
         }
@@ -523,7 +536,7 @@ __device__ static inline void solve_system(const oco_sys& d, bool valid, const d
             stage_put<R>(S->vb, nz - 1, vtop);
             if (((nz - 1) & (R - 1)) == 0) {
                 wave_sync();
-                stage_flush<R>(S->vb, S->base[1], (nz - 1) / R, nz);
+                stage_flush<R, STAGE::nt>(S->vb, S->base[1], (nz - 1) / R, nz);
                 wave_sync();
             }
         }
@@ -561,8 +574,8 @@ __device__ static inline void solve_system(const oco_sys& d, bool valid, const d
                         stage_put<R>(S->vb, e, ((-1.0 * a1q3) * df) * dz1);
                         if ((e & (R - 1)) == 0) {
                             wave_sync();
-                            stage_flush<R>(S->ub, S->base[0], e / R, nz + 6);
-                            stage_flush<R>(S->vb, S->base[1], e / R, nz);
+                            stage_flush<R, STAGE::nt>(S->ub, S->base[0], e / R, nz + 6);
+                            stage_flush<R, STAGE::nt>(S->vb, S->base[1], e / R, nz);
                             wave_sync();
                         }
                     }
@@ -596,8 +609,8 @@ __device__ static inline void solve_system(const oco_sys& d, bool valid, const d
     }
     if constexpr (!EM) {
         wave_sync();
-        stage_flush<R>(S->ub, S->base[0], 0, nz + 6);
-        stage_flush<R>(S->vb, S->base[1], 0, nz);
+        stage_flush<R, STAGE::nt>(S->ub, S->base[0], 0, nz + 6);
+        stage_flush<R, STAGE::nt>(S->vb, S->base[1], 0, nz);
     }
 }
 
@@ -791,6 +804,9 @@ __device__ static inline void solve_system_wave(const oco_sys& d, const double* 
      * arithmetic (loads past the column's records return 0) */
     const __amdgpu_buffer_rsrc_t crs =
         __builtin_amdgcn_make_buffer_rsrc((void*)cc, (short)0, CR_NFIELD * nz * (int)sizeof(double), OCO_BUF_DWORD3);
+#ifdef OCO_WAVE_POINTS_UNROLL
+#pragma unroll OCO_WAVE_POINTS_UNROLL
+#endif
     for (int z = lane; z < nz; z += OCO_WAVE) {
         double rec[CR_NFIELD];
         const int zo = z * (int)sizeof(double);
@@ -920,8 +936,16 @@ __device__ static inline void solve_system_wave(const oco_sys& d, const double* 
 
 /* Mixed-precision sweep, one wavefront per system; systems in (column,
  * wavelength) order so the wavefronts in flight share columns in L2. */
+/* Tuning knob: OCO_WAVE_WAVES_PER_EU=n asks the compiler to fit n waves of
+ * the wavefront sweep per SIMD. */
+#ifdef OCO_WAVE_WAVES_PER_EU
+#define OCO_WAVE_ATTR __attribute__((amdgpu_waves_per_eu(OCO_WAVE_WAVES_PER_EU)))
+#else
+#define OCO_WAVE_ATTR
+#endif
+
 template <int MM>
-__global__ __launch_bounds__(OCO_WAVE) void sweep_mixed_wave(const oco_sweep_args a)
+__global__ __launch_bounds__(OCO_WAVE) OCO_WAVE_ATTR void sweep_mixed_wave(const oco_sweep_args a)
 {
     extern __shared__ double lds[];
     const int nw = a.nw;
@@ -1092,7 +1116,7 @@ __global__ __launch_bounds__(OCO_WAVE) void batch_fp64(const oco_batch_args a)
 {
     __shared__ oco_mtab smt;
     __shared__ int tile_s;
-    __shared__ oco_stage stg;
+    __shared__ oco_stage_batch stg;
     load_mtab(&smt);
     batch_tiles<BC, false>(a, &smt, &tile_s, &stg);
 }
@@ -1101,7 +1125,7 @@ template <int BC>
 __global__ __launch_bounds__(OCO_WAVE) void batch_mixed(const oco_batch_args a)
 {
     __shared__ int tile_s;
-    __shared__ oco_stage_mixed stg;
+    __shared__ oco_stage_batch_mixed stg;
     batch_tiles<BC, true>(a, nullptr, &tile_s, &stg);
 }
 
