@@ -322,25 +322,26 @@ __device__ static inline void scr_get(const double* scr, int j, double& gam, dou
     }
 }
 
-/* One (column, wavelength) system, one lane: forward elimination with the
- * opacity of the cells recomputed in registers, then back-substitution fused
- * with ghosts, der and V.  `w` is the lane's wavelength-table row with
- * stride WS; `scr` its scratch column (stride 64 doubles per entry).
- * MIX selects the mixed-precision opacity and the difference-form (FP64)
- * elimination. */
-template <bool GREY, bool UNIFORM, int WS, int BC, bool MIX, class STAGE>
-__device__ static inline void solve_system(const oco_sys& d, bool valid, const double* w, double* scr,
-                                           unsigned long long* fail, const oco_mtab* M, STAGE* S)
+/* One (column, wavelength) system per lane, in two halves.  `w` is the
+ * lane's wavelength-table row with stride WS; `scr` its scratch column
+ * (stride 64 doubles per entry).  MIX selects the mixed-precision opacity and
+ * the difference-form (FP64) elimination.
+ *
+ * Forward half: points (the opacity of the cells recomputed in registers),
+ * rows and the elimination, alpha staged to its output; every row's gam, u
+ * and 1/alpha go to scratch.  Returns true when the system hit the
+ * reference's zero-pivot stop (FP64; recorded in `fail`). */
+template <bool GREY, bool UNIFORM, int WS, bool MIX, class STAGE>
+__device__ static inline bool solve_forward(const oco_sys& d, bool valid, const double* w, double* scr,
+                                            unsigned long long* fail, const oco_mtab* M, STAGE* S)
 {
     const int nz = d.nz;
-    const double dz = d.dz, dz1 = d.dz1, dz2 = d.dz2;
+    const double dz = d.dz, dz2 = d.dz2;
     constexpr int R = STAGE::rows;
     constexpr bool EM = STAGE::emergent;
     const int lane = threadIdx.x;
     if constexpr (!EM) {
         wave_sync();   /* the previous system's flushes have read the buffers */
-        S->base[0][lane] = valid ? d.U : nullptr;
-        S->base[1][lane] = valid ? d.V : nullptr;
         S->base[2][lane] = valid ? d.A : nullptr;
     }
 
@@ -477,26 +478,37 @@ __device__ static inline void solve_system(const oco_sys& d, bool valid, const d
     }
     (void)om;
     (void)Dprev;
-    if (fail_stage != 0) {
-        /* the reference stops at this system (source/auxiliary.f90:27-39):
-         * its U and V are not written (the difference form has no zero pivot) */
-        if (valid) record_fail(fail, d.key, fail_stage, fail_row);
-        if constexpr (!EM) {
-            S->base[0][lane] = nullptr;
-            S->base[1][lane] = nullptr;
-        }
-        valid = false;
+    (void)u;
+    /* the reference stops at this system (source/auxiliary.f90:27-39): its
+     * U and V are not written (the difference form has no zero pivot) */
+    if (fail_stage != 0 && valid) record_fail(fail, d.key, fail_stage, fail_row);
+    return fail_stage != 0;
+}
+
+/* Back half of one system, from the scratch rows of solve_forward:
+ * back-substitution fused with the ghosts, the derivative and V; U and V
+ * staged to their outputs.  `valid` false: nothing is stored (a lane past
+ * the end, or a system that stopped). */
+template <int BC, bool MIX, class STAGE>
+__device__ static inline void solve_backward(const oco_sys& d, bool valid, const double* scr, STAGE* S)
+{
+    const int nz = d.nz;
+    const double dz1 = d.dz1;
+    constexpr int R = STAGE::rows;
+    constexpr bool EM = STAGE::emergent;
+    const int lane = threadIdx.x;
+    if constexpr (!EM) {
+        wave_sync();   /* the forward half's flushes have read the buffers */
+        S->base[0][lane] = valid ? d.U : nullptr;
+        S->base[1][lane] = valid ? d.V : nullptr;
     }
-    if constexpr (!EM) wave_sync();
 
     /* ---- back-substitution (:43-45) fused with update_ghosts (:49-62),
      *      der (:64-78) and calc_flux (:96-109) ---- */
     const double c60 = 1.0 / 60.0;
     /* window w0..w6 = extended U at e = j .. j+6 (e = z + 3) */
-    double Un = u;                       /* U(nz-1) */
-    double gnext, a1q0, udum;            /* gam of the row above; 1/alpha of the V row pending */
-    scr_get<MIX>(scr, nz - 1, gnext, udum, a1q0);
-    (void)udum;
+    double gnext, a1q0, Un;              /* gam of the row above; 1/alpha of the V row pending */
+    scr_get<MIX>(scr, nz - 1, gnext, Un, a1q0);   /* U(nz) = u(nz) (:43) */
     double a1q1 = 0.0, a1q2 = 0.0, a1q3 = 0.0;
     double t[4];
     t[0] = Un;
@@ -614,46 +626,69 @@ __device__ static inline void solve_system(const oco_sys& d, bool valid, const d
     }
 }
 
+/* One (column, wavelength) system, one lane: the two halves in turn. */
+template <bool GREY, bool UNIFORM, int WS, int BC, bool MIX, class STAGE>
+__device__ static inline void solve_system(const oco_sys& d, bool valid, const double* w, double* scr,
+                                           unsigned long long* fail, const oco_mtab* M, STAGE* S)
+{
+    const bool stopped = solve_forward<GREY, UNIFORM, WS, MIX, STAGE>(d, valid, w, scr, fail, M, S);
+    solve_backward<BC, MIX, STAGE>(d, valid && !stopped, scr, S);
+}
+
+/* One wavefront per workgroup, grid-stride over tiles of 64 systems; the
+ * lane's wavelength-table row lives in registers. */
+/* The 64 systems of tile `tile` (64 consecutive systems in (column,
+ * wavelength) order): this lane's system, its wavelength and whether it
+ * exists (lanes past the end run a duplicate of the last system and store
+ * nothing). */
+template <bool UNIFORM>
+__device__ static inline oco_sys tile_system(const oco_sweep_args& a, long tile, int lane, bool& valid, int& iw)
+{
+    const int nw = a.nw;
+    const long nsys = a.ncol * nw;
+    const long sys0 = tile * OCO_WAVE;
+    long col;
+    if (UNIFORM) {
+        col = sys0 / nw;
+        iw = (int)(sys0 % nw) + lane;
+    } else {
+        long s = sys0 + lane;
+        if (s >= nsys) s = nsys - 1;
+        col = s / nw;
+        iw = (int)(s % nw);
+    }
+    valid = UNIFORM || sys0 + lane < nsys;
+    const long osys = (long)iw * a.ncol + col;
+    oco_sys d;
+    d.nz = a.nz;
+    d.dz = a.dz; d.dz1 = a.dz1; d.dz2 = a.dz2; d.sigma_grey = a.sigma_grey;
+    d.cells = a.cells + col * a.nz * OCO_CELL_STRIDE;
+    d.rho = a.rho + col * a.nz;
+    d.T = a.T + col * a.nz;
+    d.U = a.U + osys * (a.nz + 6);
+    d.V = a.V + osys * a.nz;
+    d.A = a.A + osys * a.nz;
+    d.key = col * nw + iw;
+    return d;
+}
+
 /* One wavefront per workgroup, grid-stride over tiles of 64 systems; the
  * lane's wavelength-table row lives in registers. */
 template <bool GREY, bool UNIFORM, bool MIX, class STAGE>
 __device__ static inline void sweep_tiles(const oco_sweep_args& a, const oco_mtab* M, STAGE* S)
 {
     const int lane = threadIdx.x;
-    const int nw = a.nw;
-    const long nsys = a.ncol * nw;
-    const long ntiles = (nsys + OCO_WAVE - 1) / OCO_WAVE;
+    const long ntiles = (a.ncol * a.nw + OCO_WAVE - 1) / OCO_WAVE;
     double* scr = a.scratch + (long)blockIdx.x * a.nz * 3 * OCO_WAVE + lane;
     for (long tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
-        long sys0 = tile * OCO_WAVE;
-        long col;
+        bool valid;
         int iw;
-        if (UNIFORM) {
-            col = sys0 / nw;
-            iw = (int)(sys0 % nw) + lane;
-        } else {
-            long s = sys0 + lane;
-            if (s >= nsys) s = nsys - 1;   /* past the end: duplicate of the last system, no stores */
-            col = s / nw;
-            iw = (int)(s % nw);
-        }
-        const bool valid = UNIFORM || sys0 + lane < nsys;
+        const oco_sys d = tile_system<UNIFORM>(a, tile, lane, valid, iw);
         double w[OCO_WT_STRIDE];
         if (!GREY) {
 #pragma unroll
             for (int k = 0; k < OCO_WT_STRIDE; k++) w[k] = a.wtab[iw * OCO_WT_STRIDE + k];
         }
-        const long osys = (long)iw * a.ncol + col;
-        oco_sys d;
-        d.nz = a.nz;
-        d.dz = a.dz; d.dz1 = a.dz1; d.dz2 = a.dz2; d.sigma_grey = a.sigma_grey;
-        d.cells = a.cells + col * a.nz * OCO_CELL_STRIDE;
-        d.rho = a.rho + col * a.nz;
-        d.T = a.T + col * a.nz;
-        d.U = a.U + osys * (a.nz + 6);
-        d.V = a.V + osys * a.nz;
-        d.A = a.A + osys * a.nz;
-        d.key = col * nw + iw;
         solve_system<GREY, UNIFORM, 1, OCO_BACK_CHUNK, MIX, STAGE>(d, valid, w, scr, a.fail, M, S);
     }
 }
