@@ -42,7 +42,7 @@ struct oco_ctx {
     int lgrey = 0;
     double sigma_grey = 0.4;
     int mixed_sweep = 0;  /* mixed mode: 0 by column length, 1 wavefront per system, 2 lane per system */
-    DevBuf wtab, rho, T, cells, scratch, U, V, A, fail, canary, raw, tg;
+    DevBuf wtab, rho, T, cells, scratch, U, V, A, fail, canary, raw, tg, tickets;
     /* block outputs to files: copy stream, per-slot events, pinned staging */
     hipStream_t copy_stream = nullptr;
     hipEvent_t ev_sweep[2] = {nullptr, nullptr}, ev_copy[2] = {nullptr, nullptr};
@@ -224,7 +224,7 @@ int oco_destroy(oco_ctx* c)
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     for (DevBuf* b : {&c->wtab, &c->rho, &c->T, &c->cells, &c->scratch, &c->U, &c->V, &c->A, &c->fail,
-                      &c->canary, &c->raw, &c->tg})
+                      &c->canary, &c->raw, &c->tg, &c->tickets})
         release(c, *b);
     for (auto& e : c->ev)
         if (e) (void)hipEventDestroy(e);
@@ -303,6 +303,7 @@ static int sweep_buffers(oco_ctx* c, const SweepPlan& p, int nz, long ncells)
         const size_t slots = ((size_t)p.nblocks + 7) / 8 * 8;
         if ((st = ensure(c, c->scratch, slots * nz * 3 * 64 * sizeof(double)))) return st;
     }
+    if ((st = ensure(c, c->tickets, OCO_TICKET_BYTES))) return st;
     if ((st = ensure(c, c->fail, sizeof(unsigned long long)))) return st;
     if ((st = ensure(c, c->canary, (4 + 5 * (size_t)nz) * sizeof(double)))) return st;
     return OCO_OK;
@@ -328,6 +329,7 @@ static oco_sweep_args sweep_args(const oco_ctx* c, int nz, long ncol, double dz,
     a.scratch = (double*)c->scratch.p;
     a.fail = (unsigned long long*)c->fail.p;
     a.canary = (double*)c->canary.p;
+    a.tickets = (unsigned long long*)c->tickets.p;
     return a;
 }
 

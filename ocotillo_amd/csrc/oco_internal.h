@@ -23,7 +23,14 @@ struct oco_sweep_args {
     double* scratch;     /* per-block slots of nz*3*64 doubles */
     unsigned long long* fail;  /* first zero pivot key (atomicMin) */
     double* canary;      /* 4 + 5*nz doubles: sum(a..d) of system 0, then its point values */
+    unsigned long long* tickets;  /* wavefront sweep: OCO_TICKET_QUEUES counters, OCO_TICKET_STRIDE apart */
 };
+
+/* Work queues of the wavefront sweep: one ticket counter per XCD, each in a
+ * 256-byte line of its own. */
+#define OCO_TICKET_QUEUES 8
+#define OCO_TICKET_STRIDE 32
+#define OCO_TICKET_BYTES (OCO_TICKET_QUEUES * OCO_TICKET_STRIDE * 8)
 
 /* Batched uneven columns (per-column shapes and wavelength grids). */
 struct oco_batch_args {

@@ -674,13 +674,36 @@ __device__ static inline oco_sys tile_system(const oco_sweep_args& a, long tile,
 
 /* One wavefront per workgroup, grid-stride over tiles of 64 systems; the
  * lane's wavelength-table row lives in registers. */
+/* Work queues of the sweeps.  Wavefronts doing the same work progress at
+ * rates up to twice apart (they share SIMDs, L2 and memory channels
+ * unevenly), so a fixed share per wavefront leaves the grid waiting on its
+ * slowest; instead each wavefront takes its next ticket from a counter when
+ * it is done with the last, and the grid drains together.  Counters are
+ * cleared by tickets_reset before every sweep launch. */
+__global__ void tickets_reset(unsigned long long* t)
+{
+    if (threadIdx.x < OCO_TICKET_QUEUES) t[threadIdx.x * OCO_TICKET_STRIDE] = 0ull;
+}
+
+/* The first of the next n items of a queue, the same in every lane (one
+ * wavefront per workgroup). */
+__device__ __forceinline__ long wave_take(unsigned long long* ctr, unsigned long long n)
+{
+    unsigned long long v = 0ull;
+    if (threadIdx.x == 0) v = atomicAdd(ctr, n);
+    const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)v);
+    const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(v >> 32));
+    return (long)(((unsigned long long)hi << 32) | lo);
+}
+
 template <bool GREY, bool UNIFORM, bool MIX, class STAGE>
 __device__ static inline void sweep_tiles(const oco_sweep_args& a, const oco_mtab* M, STAGE* S)
 {
     const int lane = threadIdx.x;
     const long ntiles = (a.ncol * a.nw + OCO_WAVE - 1) / OCO_WAVE;
     double* scr = a.scratch + (long)blockIdx.x * a.nz * 3 * OCO_WAVE + lane;
-    for (long tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    /* one queue of tiles in order: the tiles in flight stay neighbours */
+    for (long tile = wave_take(a.tickets, 1); tile < ntiles; tile = wave_take(a.tickets, 1)) {
         bool valid;
         int iw;
         const oco_sys d = tile_system<UNIFORM>(a, tile, lane, valid, iw);
@@ -979,41 +1002,58 @@ __device__ static inline void solve_system_wave(const oco_sys& d, const double* 
 #define OCO_WAVE_ATTR
 #endif
 
+/* Systems a wavefront takes per ticket of the wavefront sweep (measured on
+ * MI355X, 256 x 256 columns, nz = 512, nw = 128: 1 56.4 ms, 2 55.8, 4 54.0,
+ * 8 53.2, 16 55.5). */
+#ifndef OCO_WAVE_TICKET
+#define OCO_WAVE_TICKET 8
+#endif
+
 template <int MM>
 __global__ __launch_bounds__(OCO_WAVE) OCO_WAVE_ATTR void sweep_mixed_wave(const oco_sweep_args a)
 {
     extern __shared__ double lds[];
     const int nw = a.nw;
-    /* XCD-aware order: workgroups are dispatched to the 8 XCDs round-robin,
-     * so workgroup b runs on XCD b mod 8.  With a grid that is a multiple of
-     * 8, the workgroups of XCD x take the columns x, x + 8, x + 16, ..., each
-     * column's wavelengths in turn: a column's cell records (22 fields x nz
-     * doubles, read by every one of its systems) are brought into one L2
-     * only and reused there.  Any other grid: plain (column, wavelength)
-     * order. */
-    const bool xcd = gridDim.x % OCO_XCDS == 0;
-    const int x = xcd ? (int)(blockIdx.x % OCO_XCDS) : 0;
-    const long per = xcd ? gridDim.x / OCO_XCDS : gridDim.x;
-    const long first = xcd ? blockIdx.x / OCO_XCDS : blockIdx.x;
-    const long ncx = xcd ? (a.ncol - x + OCO_XCDS - 1) / OCO_XCDS : a.ncol;   /* columns of this XCD */
-    const long nsys = ncx * nw;
-    for (long t = first; t < nsys; t += per) {
-        const long col = xcd ? (t / nw) * OCO_XCDS + x : t / nw;
-        const int iw = (int)(t % nw);
-        double w[OCO_WT_STRIDE];
-        oco_cptr wp = (oco_cptr)(a.wtab + (long)iw * OCO_WT_STRIDE);
+    /* XCD-aware work queues: workgroups are dispatched to the 8 XCDs
+     * round-robin, so workgroup b runs on XCD b mod 8.  With a grid that is a
+     * multiple of 8, queue x holds the columns x, x + 8, x + 16, ..., each
+     * column's wavelengths in turn, and the wavefronts of XCD x draw tickets
+     * of OCO_WAVE_TICKET systems from it: a column's cell records (22 fields
+     * x nz doubles, read by every one of its systems) are brought into one L2
+     * and reused there.  A wavefront whose queue is empty goes on to the
+     * next XCD's, so the grid drains together: systems cost the same work,
+     * yet wavefronts sharing a SIMD progress at different rates.  Any other
+     * grid: one queue in (column, wavelength) order.  Every wavefront ends
+     * once it has found all queues empty. */
+    const int nq = gridDim.x % OCO_XCDS == 0 ? OCO_XCDS : 1;
+    const int home = (int)(blockIdx.x % nq);
+    for (int v = 0; v < nq; v++) {
+        const int q = (home + v) % nq;
+        const long nsys = (a.ncol - q + nq - 1) / nq * nw;   /* columns of queue q x wavelengths */
+        unsigned long long* ctr = a.tickets + q * OCO_TICKET_STRIDE;
+        for (;;) {
+            const long t0 = wave_take(ctr, OCO_WAVE_TICKET);
+            if (t0 >= nsys) break;
+            const long t1 = t0 + OCO_WAVE_TICKET < nsys ? t0 + OCO_WAVE_TICKET : nsys;
+            for (long t = t0; t < t1; t++) {
+                const long col = (t / nw) * nq + q;
+                const int iw = (int)(t % nw);
+                double w[OCO_WT_STRIDE];
+                oco_cptr wp = (oco_cptr)(a.wtab + (long)iw * OCO_WT_STRIDE);
 #pragma unroll
-        for (int k = 0; k < OCO_WT_STRIDE; k++) w[k] = wp[k];
-        const long osys = (long)iw * a.ncol + col;
-        oco_sys d;
-        d.nz = a.nz;
-        d.dz = a.dz; d.dz1 = a.dz1; d.dz2 = a.dz2; d.sigma_grey = 0.0;
-        d.cells = nullptr; d.rho = nullptr; d.T = nullptr;
-        d.U = a.U + osys * (a.nz + 6);
-        d.V = a.V + osys * a.nz;
-        d.A = a.A + osys * a.nz;
-        d.key = col * nw + iw;
-        solve_system_wave<MM>(d, a.cells + col * (long)CR_NFIELD * a.nz, w, lds);
+                for (int k = 0; k < OCO_WT_STRIDE; k++) w[k] = wp[k];
+                const long osys = (long)iw * a.ncol + col;
+                oco_sys d;
+                d.nz = a.nz;
+                d.dz = a.dz; d.dz1 = a.dz1; d.dz2 = a.dz2; d.sigma_grey = 0.0;
+                d.cells = nullptr; d.rho = nullptr; d.T = nullptr;
+                d.U = a.U + osys * (a.nz + 6);
+                d.V = a.V + osys * a.nz;
+                d.A = a.A + osys * a.nz;
+                d.key = col * nw + iw;
+                solve_system_wave<MM>(d, a.cells + col * (long)CR_NFIELD * a.nz, w, lds);
+            }
+        }
     }
 }
 
@@ -1442,6 +1482,8 @@ hipError_t oco_launch_sweep(const oco_sweep_args& a, int lgrey, int mixed, int w
         else
             hipLaunchKernelGGL((canary_fp64<false>), dim3(1), dim3(256), 0, stream, a, a.canary);
     }
+    if (!a.tickets) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(tickets_reset, dim3(1), dim3(64), 0, stream, a.tickets);
     const dim3 g(nblocks), b(OCO_WAVE);
     if (lgrey) {
         /* grey opacity has no per-wavelength transcendentals: one kernel for both modes */

@@ -186,6 +186,38 @@ def test_mixed_wave_and_lane_sweeps(nz):
     assert np.array_equal(out["wave"], out["lane"])   # same opacity code
 
 
+def test_wave_sweep_queues_cover_every_system():
+    """The wavefront sweep hands out systems from work queues (one per XCD
+    when its grid is a multiple of 8, else one), so which wavefront solves a
+    system varies from call to call; each system is still solved once and by
+    itself.  A 3 x 5 block (grid of 600 wavefronts; queues of one and two
+    columns) equals its columns solved one at a time (grid of 40: seven
+    empty queues) and with 37 wavelengths (grid of 37: one queue), bit for
+    bit, and a repeat of the block after those smaller calls equals the
+    first."""
+    import ocotillo_amd as oco
+    from ocotillo_amd.workloads import config2
+    rho, T, dz, waves = config2(3, 5, 256, 40)
+    with sweep_context("wave") as c:
+        c.set_physics(2e4, False, 0.4)
+        c.set_wavelengths(waves)
+        U, V, A, d = c.solve_block(rho, T, dz, mode=oco.MODE_MIXED)
+        assert d.status == 0 and np.isfinite(U).all() and np.isfinite(V).all()
+        for ix in range(3):
+            for iy in range(5):
+                u, v, a, _ = c.solve_block(rho[ix:ix + 1, iy:iy + 1], T[ix:ix + 1, iy:iy + 1], dz,
+                                           mode=oco.MODE_MIXED)
+                assert np.array_equal(u[:, 0, 0], U[:, ix, iy]), (ix, iy)
+                assert np.array_equal(v[:, 0, 0], V[:, ix, iy]), (ix, iy)
+                assert np.array_equal(a[:, 0, 0], A[:, ix, iy]), (ix, iy)
+        c.set_wavelengths(waves[:37])
+        u, v, a, _ = c.solve_block(rho[2:3, 4:5], T[2:3, 4:5], dz, mode=oco.MODE_MIXED)
+        assert np.array_equal(u[:, 0, 0], U[:37, 2, 4]) and np.array_equal(v[:, 0, 0], V[:37, 2, 4])
+        c.set_wavelengths(waves)
+        U2, V2, A2, _ = c.solve_block(rho, T, dz, mode=oco.MODE_MIXED)
+        assert np.array_equal(U2, U) and np.array_equal(V2, V) and np.array_equal(A2, A)
+
+
 def test_batch_mixed_equals_block(ctx):
     """Batched (persistent, graph-replayed) mixed solve == the block mixed
     solve with the same (lane-per-system) kernel, bit for bit, on every
