@@ -42,6 +42,7 @@ struct oco_ctx {
     int lgrey = 0;
     double sigma_grey = 0.4;
     int mixed_sweep = 0;  /* mixed mode: 0 by column length, 1 wavefront per system, 2 lane per system */
+    int wave_group = 0;   /* wavefronts per workgroup of the wavefront sweep: 1, 2, 4, 0 = default */
     DevBuf wtab, rho, T, cells, scratch, U, V, A, fail, canary, raw, tg, tickets;
     /* block outputs to files: copy stream, per-slot events, pinned staging */
     hipStream_t copy_stream = nullptr;
@@ -204,6 +205,9 @@ int oco_create(int device, size_t hbm_budget, oco_ctx** out)
      * column length */
     if (const char* e = getenv("OCO_MIXED_SWEEP"))
         c->mixed_sweep = strcmp(e, "wave") == 0 ? 1 : strcmp(e, "lane") == 0 ? 2 : 0;
+    /* OCO_WAVE_GROUP=1|2|4: wavefronts per workgroup of the wavefront sweep
+     * (same results; default: the library's measured choice) */
+    if (const char* e = getenv("OCO_WAVE_GROUP")) c->wave_group = atoi(e);
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
         delete c;
         return OCO_E_HIP;
@@ -284,8 +288,10 @@ static SweepPlan sweep_plan(const oco_ctx* c, int nz, long ncol, int nw_run, int
     p.wave = mixed && !c->lgrey && c->mixed_sweep != 2 && oco_wave_fits(nz, c->device) &&
              (c->mixed_sweep == 1 || oco_wave_preferred(nz));
     if (p.wave) {
-        const int grid = oco_wave_grid(nz, c->device);
-        p.nblocks = (int)(nsys < grid ? nsys : grid);
+        p.wave = oco_wave_group(nz, c->wave_group, c->device);
+        const long ngroups = ncol * ((nw_run + p.wave - 1) / p.wave);
+        const int grid = oco_wave_grid(nz, p.wave, c->device);
+        p.nblocks = (int)(ngroups < grid ? ngroups : grid);
     } else {
         const int grid = oco_sweep_grid(c->lgrey, mixed, (nw_run % 64) == 0, c->device);
         p.nblocks = (int)(ntiles < grid ? ntiles : grid);

@@ -121,13 +121,16 @@ int oco_batch_variants(void);
  * cell z of column c at (c*CR_NFIELD + k)*soa_nz + z); 0: cell-major */
 hipError_t oco_launch_cells(const double* rho, const double* T, long ncells, double fully_ionized_T,
                             double* cells, int soa_nz, hipStream_t stream);
-/* wave != 0 (mixed, non-grey): the wavefront-per-system sweep, which reads
- * field-major cell records and needs no scratch */
+/* wave != 0 (mixed, non-grey): the wavefront-per-system sweep with `wave`
+ * (1, 2 or 4) wavefronts per workgroup, which reads field-major cell records
+ * and needs no scratch */
 hipError_t oco_launch_sweep(const oco_sweep_args& a, int lgrey, int mixed, int wave, int nblocks, hipStream_t stream);
 int oco_wave_fits(int nz, int device);
 /* 1 when the wavefront sweep is the faster mixed sweep for nz (measured) */
 int oco_wave_preferred(int nz);
-int oco_wave_grid(int nz, int device);
+/* wavefronts per workgroup of the wavefront sweep (want: 1, 2, 4, or 0 for the default) */
+int oco_wave_group(int nz, int want, int device);
+int oco_wave_grid(int nz, int k, int device);
 int oco_sweep_grid(int lgrey, int mixed, int uniform, int device);
 
 /* host helpers shared by the C-ABI files */

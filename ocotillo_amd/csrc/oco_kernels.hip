@@ -840,13 +840,57 @@ __host__ __device__ static inline int wave_rows(int nz)
     return ((nz + OCO_WAVE - 1) / OCO_WAVE) | 1;
 }
 
-/* one system, the whole wavefront; `w` is its wavelength-table row
- * (wave-uniform), `cc` its column's field-major cell records (field k of
- * cell z at cc[k*nz + z]) */
-template <int MM>
-__device__ static inline void solve_system_wave(const oco_sys& d, const double* cc, const double* w, double* lds)
+/* Words of LDS a wavefront's system takes: alpha [nz], albedo [nz], S -> U
+ * with ghosts [nz + 6]. */
+__host__ __device__ static inline long wave_lds_words(int nz) { return 3L * nz + 6; }
+
+/* 1. points of nk <= K systems of one column, one per wavefront of the
+ * workgroup (source/continuous_opacity.f90:175-201): thread t takes cells
+ * z = t, t + 64 K, ..., loads the cell's record once and evaluates it for
+ * each of the nk wavelengths iw0 .. iw0 + nk - 1, into wavefront k's LDS
+ * region (alpha, albedo, S) and the alpha output of system k.  The column's
+ * field-major records (field k of cell z at cc[k*nz + z]) are read through a
+ * buffer descriptor: the field offset is a scalar, the lane offset a vector
+ * register, no per-field address arithmetic (loads past the column's records
+ * return 0). */
+template <int K>
+__device__ static inline void wave_points(const double* cc, int nz, const double* wtab, int iw0, int nk,
+                                          double* A0, long astride, double* lds)
 {
-    const int nz = d.nz, lane = threadIdx.x;
+    const long lstr = wave_lds_words(nz);
+    const __amdgpu_buffer_rsrc_t crs =
+        __builtin_amdgcn_make_buffer_rsrc((void*)cc, (short)0, CR_NFIELD * nz * (int)sizeof(double), OCO_BUF_DWORD3);
+#ifdef OCO_WAVE_POINTS_UNROLL
+#pragma unroll OCO_WAVE_POINTS_UNROLL
+#endif
+    for (int z = threadIdx.x; z < nz; z += OCO_WAVE * K) {
+        double rec[CR_NFIELD];
+        const int zo = z * (int)sizeof(double);
+#pragma unroll
+        for (int k = 0; k < CR_NFIELD; k++)
+            rec[k] = __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(crs, zo,
+                                                                                    k * nz * (int)sizeof(double), 0));
+        for (int k = 0; k < nk; k++) {
+            double w[OCO_WT_STRIDE];
+            oco_cptr wp = (oco_cptr)(wtab + (long)(iw0 + k) * OCO_WT_STRIDE);
+#pragma unroll
+            for (int f = 0; f < OCO_WT_STRIDE; f++) w[f] = wp[f];
+            const oco_pt p = oco_point_mixed<1>(rec, w);
+            double* L = lds + k * lstr;
+            L[z] = p.alpha;
+            L[nz + z] = p.albedo;
+            L[2 * nz + z + 3] = p.S;
+            out_store(A0 + k * astride + z, p.alpha);
+        }
+    }
+}
+
+/* 2.-4. one system, the whole wavefront, from its points in `lds` (written by
+ * wave_points; the caller has synchronised the workgroup since) */
+template <int MM>
+__device__ static inline void solve_system_wave(const oco_sys& d, double* lds)
+{
+    const int nz = d.nz, lane = (int)threadIdx.x & (OCO_WAVE - 1);
     const double dz = d.dz, dz1 = d.dz1, dz2 = d.dz2;
     double* LA = lds;              /* alpha */
     double* LW = lds + nz;         /* albedo */
@@ -855,30 +899,6 @@ __device__ static inline void solve_system_wave(const oco_sys& d, const double* 
     const int z0 = lane * m < nz ? lane * m : nz;
     const int z1 = z0 + m < nz ? z0 + m : nz;
 
-    wave_sync();   /* the previous system's last reads are done */
-    /* 1. points (source/continuous_opacity.f90:175-201); the column's
-     * records are read through a buffer descriptor: the field offset is a
-     * scalar, the lane offset a vector register, no per-field address
-     * arithmetic (loads past the column's records return 0) */
-    const __amdgpu_buffer_rsrc_t crs =
-        __builtin_amdgcn_make_buffer_rsrc((void*)cc, (short)0, CR_NFIELD * nz * (int)sizeof(double), OCO_BUF_DWORD3);
-#ifdef OCO_WAVE_POINTS_UNROLL
-#pragma unroll OCO_WAVE_POINTS_UNROLL
-#endif
-    for (int z = lane; z < nz; z += OCO_WAVE) {
-        double rec[CR_NFIELD];
-        const int zo = z * (int)sizeof(double);
-#pragma unroll
-        for (int k = 0; k < CR_NFIELD; k++)
-            rec[k] = __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(crs, zo,
-                                                                                    k * nz * (int)sizeof(double), 0));
-        oco_pt p = oco_point_mixed<1>(rec, w);
-        LA[z] = p.alpha;
-        LW[z] = p.albedo;
-        LS[z + 3] = p.S;
-        out_store(d.A + z, p.alpha);
-    }
-    wave_sync();
     /* 2. rows (source/auxiliary.f90:147-197) in difference form and the
      * chunk's forward elimination */
     double Dr[MM], Wr[MM], Zr[MM];
@@ -992,8 +1012,8 @@ __device__ static inline void solve_system_wave(const oco_sys& d, const double* 
     for (int e = lane; e < nz + 6; e += OCO_WAVE) out_store(d.U + e, LS[e]);
 }
 
-/* Mixed-precision sweep, one wavefront per system; systems in (column,
- * wavelength) order so the wavefronts in flight share columns in L2. */
+/* Mixed-precision sweep, one wavefront per system, K wavefronts per
+ * workgroup on K wavelengths of one column. */
 /* Tuning knob: OCO_WAVE_WAVES_PER_EU=n asks the compiler to fit n waves of
  * the wavefront sweep per SIMD. */
 #ifdef OCO_WAVE_WAVES_PER_EU
@@ -1002,73 +1022,107 @@ __device__ static inline void solve_system_wave(const oco_sys& d, const double* 
 #define OCO_WAVE_ATTR
 #endif
 
-/* Systems a wavefront takes per ticket of the wavefront sweep (measured on
- * MI355X, 256 x 256 columns, nz = 512, nw = 128: 1 56.4 ms, 2 55.8, 4 54.0,
- * 8 53.2, 16 55.5). */
+/* Systems a workgroup takes per ticket of the wavefront sweep (measured on
+ * MI355X with one wavefront per workgroup, 256 x 256 columns, nz = 512,
+ * nw = 128: 1 56.4 ms, 2 55.8, 4 54.0, 8 53.2, 16 55.5). */
 #ifndef OCO_WAVE_TICKET
 #define OCO_WAVE_TICKET 8
 #endif
 
-template <int MM>
-__global__ __launch_bounds__(OCO_WAVE) OCO_WAVE_ATTR void sweep_mixed_wave(const oco_sweep_args a)
+template <int MM, int K>
+__global__ __launch_bounds__(OCO_WAVE * K) OCO_WAVE_ATTR void sweep_mixed_wave(const oco_sweep_args a)
 {
     extern __shared__ double lds[];
-    const int nw = a.nw;
+    __shared__ long ticket;
+    const int nw = a.nw, nz = a.nz;
+    const int ngrp = (nw + K - 1) / K;   /* wavelength groups per column */
+    const int wid = (int)threadIdx.x / OCO_WAVE;
+    constexpr long TG = OCO_WAVE_TICKET / K > 0 ? OCO_WAVE_TICKET / K : 1;   /* groups per ticket */
+    const long astride = a.ncol * (long)nz;   /* alpha output: one wavelength to the next */
     /* XCD-aware work queues: workgroups are dispatched to the 8 XCDs
      * round-robin, so workgroup b runs on XCD b mod 8.  With a grid that is a
      * multiple of 8, queue x holds the columns x, x + 8, x + 16, ..., each
-     * column's wavelengths in turn, and the wavefronts of XCD x draw tickets
-     * of OCO_WAVE_TICKET systems from it: a column's cell records (22 fields
-     * x nz doubles, read by every one of its systems) are brought into one L2
-     * and reused there.  A wavefront whose queue is empty goes on to the
-     * next XCD's, so the grid drains together: systems cost the same work,
-     * yet wavefronts sharing a SIMD progress at different rates.  Any other
-     * grid: one queue in (column, wavelength) order.  Every wavefront ends
-     * once it has found all queues empty. */
+     * column's wavelength groups in turn, and the workgroups of XCD x draw
+     * tickets of TG groups from it: a column's cell records (22 fields x nz
+     * doubles) are brought into one L2 and reused there, and each record a
+     * thread loads serves the K wavelengths of its group.  A workgroup whose
+     * queue is empty goes on to the next XCD's, so the grid drains together.
+     * Any other grid: one queue in (column, wavelength) order.  Every
+     * workgroup ends once it has found all queues empty (the ticket is read
+     * by every thread after a barrier, so the exit is workgroup-uniform). */
     const int nq = gridDim.x % OCO_XCDS == 0 ? OCO_XCDS : 1;
     const int home = (int)(blockIdx.x % nq);
     for (int v = 0; v < nq; v++) {
         const int q = (home + v) % nq;
-        const long nsys = (a.ncol - q + nq - 1) / nq * nw;   /* columns of queue q x wavelengths */
+        const long ngroups = (a.ncol - q + nq - 1) / nq * ngrp;   /* columns of queue q x groups */
         unsigned long long* ctr = a.tickets + q * OCO_TICKET_STRIDE;
         for (;;) {
-            const long t0 = wave_take(ctr, OCO_WAVE_TICKET);
-            if (t0 >= nsys) break;
-            const long t1 = t0 + OCO_WAVE_TICKET < nsys ? t0 + OCO_WAVE_TICKET : nsys;
+            __syncthreads();   /* the last group's systems are done with LDS, and every thread has read `ticket` */
+            if (threadIdx.x == 0) ticket = (long)atomicAdd(ctr, (unsigned long long)TG);
+            __syncthreads();
+            const long t0 = ticket;
+            if (t0 >= ngroups) break;
+            const long t1 = t0 + TG < ngroups ? t0 + TG : ngroups;
             for (long t = t0; t < t1; t++) {
-                const long col = (t / nw) * nq + q;
-                const int iw = (int)(t % nw);
-                double w[OCO_WT_STRIDE];
-                oco_cptr wp = (oco_cptr)(a.wtab + (long)iw * OCO_WT_STRIDE);
-#pragma unroll
-                for (int k = 0; k < OCO_WT_STRIDE; k++) w[k] = wp[k];
-                const long osys = (long)iw * a.ncol + col;
-                oco_sys d;
-                d.nz = a.nz;
-                d.dz = a.dz; d.dz1 = a.dz1; d.dz2 = a.dz2; d.sigma_grey = 0.0;
-                d.cells = nullptr; d.rho = nullptr; d.T = nullptr;
-                d.U = a.U + osys * (a.nz + 6);
-                d.V = a.V + osys * a.nz;
-                d.A = a.A + osys * a.nz;
-                d.key = col * nw + iw;
-                solve_system_wave<MM>(d, a.cells + col * (long)CR_NFIELD * a.nz, w, lds);
+                const long col = (t / ngrp) * nq + q;
+                const int iw0 = (int)(t % ngrp) * K;
+                const int nk = nw - iw0 < K ? nw - iw0 : K;
+                if (t > t0) __syncthreads();   /* the previous group's systems are done with LDS */
+                wave_points<K>(a.cells + col * (long)CR_NFIELD * nz, nz, a.wtab, iw0, nk,
+                               a.A + ((long)iw0 * a.ncol + col) * nz, astride, lds);
+                __syncthreads();
+                if (wid < nk) {
+                    const int iw = iw0 + wid;
+                    const long osys = (long)iw * a.ncol + col;
+                    oco_sys d;
+                    d.nz = nz;
+                    d.dz = a.dz; d.dz1 = a.dz1; d.dz2 = a.dz2; d.sigma_grey = 0.0;
+                    d.cells = nullptr; d.rho = nullptr; d.T = nullptr;
+                    d.U = a.U + osys * (nz + 6);
+                    d.V = a.V + osys * nz;
+                    d.A = a.A + osys * nz;
+                    d.key = col * nw + iw;
+                    solve_system_wave<MM>(d, lds + wid * wave_lds_words(nz));
+                }
             }
         }
     }
 }
 
-static size_t wave_lds_bytes(int nz) { return (3 * (size_t)nz + 6) * sizeof(double); }
+static size_t wave_lds_bytes(int nz, int k) { return (size_t)k * wave_lds_words(nz) * sizeof(double); }
 
 #define OCO_WAVE_MAX_ROWS 17
 
-/* 1 when the wavefront-per-system mixed sweep takes columns of nz cells:
- * at most OCO_WAVE_MAX_ROWS rows per lane and its LDS within a workgroup's */
-int oco_wave_fits(int nz, int device)
+/* Wavefronts per workgroup of the wavefront sweep: each cell record a thread
+ * loads serves K wavelengths.  Measured on MI355X (256 x 256 columns,
+ * nz = 512, nw = 128), see oco_wave_group. */
+#ifndef OCO_WAVE_GROUP
+#define OCO_WAVE_GROUP 4
+#endif
+
+static int lds_limit(int device)
 {
     int lim = 0;
     if (hipDeviceGetAttribute(&lim, hipDeviceAttributeMaxSharedMemoryPerBlock, device) != hipSuccess || lim <= 0)
         lim = 65536;
-    return nz >= 4 && wave_rows(nz) <= OCO_WAVE_MAX_ROWS && wave_lds_bytes(nz) <= (size_t)lim;
+    return lim;
+}
+
+/* 1 when the wavefront-per-system mixed sweep takes columns of nz cells:
+ * at most OCO_WAVE_MAX_ROWS rows per lane and one wavefront's LDS within a
+ * workgroup's */
+int oco_wave_fits(int nz, int device)
+{
+    return nz >= 4 && wave_rows(nz) <= OCO_WAVE_MAX_ROWS && wave_lds_bytes(nz, 1) <= (size_t)lds_limit(device);
+}
+
+/* Wavefronts per workgroup for columns of nz cells: `want` (1, 2 or 4; 0 for
+ * OCO_WAVE_GROUP), halved until the workgroup's LDS fits. */
+int oco_wave_group(int nz, int want, int device)
+{
+    int k = want == 1 || want == 2 || want == 4 ? want : OCO_WAVE_GROUP;
+    while (k > 1 && wave_lds_bytes(nz, k) > (size_t)lds_limit(device)) k /= 2;
+    return k;
 }
 
 /* Measured on MI355X on a 128 x 128-column block, nw = 128 (sweep ms, lane / wave):
@@ -1086,19 +1140,26 @@ int oco_wave_preferred(int nz)
     return m >= OCO_WAVE_AUTO_MIN_ROWS && m <= OCO_WAVE_AUTO_ROWS;
 }
 
-static const void* wave_kernel(int nz)
+template <int K>
+static const void* wave_kernel_k(int nz)
 {
     const int m = wave_rows(nz);
-    if (m <= 5) return (const void*)sweep_mixed_wave<5>;
-    if (m <= 9) return (const void*)sweep_mixed_wave<9>;
-    return (const void*)sweep_mixed_wave<17>;
+    if (m <= 5) return (const void*)sweep_mixed_wave<5, K>;
+    if (m <= 9) return (const void*)sweep_mixed_wave<9, K>;
+    return (const void*)sweep_mixed_wave<17, K>;
 }
 
-int oco_wave_grid(int nz, int device)
+static const void* wave_kernel(int nz, int k)
+{
+    return k == 4 ? wave_kernel_k<4>(nz) : k == 2 ? wave_kernel_k<2>(nz) : wave_kernel_k<1>(nz);
+}
+
+/* Resident workgroups of the wavefront sweep with k wavefronts each. */
+int oco_wave_grid(int nz, int k, int device)
 {
     int per_cu = 0, cus = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, wave_kernel(nz), OCO_WAVE, wave_lds_bytes(nz)) !=
-            hipSuccess || per_cu < 1)
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, wave_kernel(nz, k), OCO_WAVE * k,
+                                                     wave_lds_bytes(nz, k)) != hipSuccess || per_cu < 1)
         per_cu = 1;
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess || cus < 1)
         cus = 256;
@@ -1472,6 +1533,17 @@ int oco_sweep_grid(int lgrey, int mixed, int uniform, int device)
     return per_cu * cus;
 }
 
+template <int K>
+static void launch_wave(const oco_sweep_args& a, int nblocks, hipStream_t stream)
+{
+    const int m = wave_rows(a.nz);
+    const dim3 g(nblocks), b(OCO_WAVE * K);
+    const size_t sh = wave_lds_bytes(a.nz, K);
+    if (m <= 5) hipLaunchKernelGGL((sweep_mixed_wave<5, K>), g, b, sh, stream, a);
+    else if (m <= 9) hipLaunchKernelGGL((sweep_mixed_wave<9, K>), g, b, sh, stream, a);
+    else hipLaunchKernelGGL((sweep_mixed_wave<17, K>), g, b, sh, stream, a);
+}
+
 hipError_t oco_launch_sweep(const oco_sweep_args& a, int lgrey, int mixed, int wave, int nblocks, hipStream_t stream)
 {
     (void)hipGetLastError();   /* a pending not-ready status from an event query is not this launch's */
@@ -1489,10 +1561,9 @@ hipError_t oco_launch_sweep(const oco_sweep_args& a, int lgrey, int mixed, int w
         /* grey opacity has no per-wavelength transcendentals: one kernel for both modes */
         hipLaunchKernelGGL((sweep_fp64<true, false>), g, b, 0, stream, a);
     } else if (mixed && wave) {
-        const int m = wave_rows(a.nz);
-        if (m <= 5) hipLaunchKernelGGL(sweep_mixed_wave<5>, g, b, wave_lds_bytes(a.nz), stream, a);
-        else if (m <= 9) hipLaunchKernelGGL(sweep_mixed_wave<9>, g, b, wave_lds_bytes(a.nz), stream, a);
-        else hipLaunchKernelGGL(sweep_mixed_wave<17>, g, b, wave_lds_bytes(a.nz), stream, a);
+        if (wave == 4) launch_wave<4>(a, nblocks, stream);
+        else if (wave == 2) launch_wave<2>(a, nblocks, stream);
+        else launch_wave<1>(a, nblocks, stream);
     } else if (mixed) {
         if (uniform) hipLaunchKernelGGL((sweep_mixed<true>), g, b, 0, stream, a);
         else hipLaunchKernelGGL((sweep_mixed<false>), g, b, 0, stream, a);

@@ -149,19 +149,24 @@ def test_mixed_c2_slice(ctx):
     check_against_rh(*out["mixed"], rh_oracle(rho, T, dz, waves), dz, "mixed")
 
 
-def sweep_context(kind):
-    """A context whose mixed mode uses the given sweep ("wave" or "lane")."""
+def sweep_context(kind, group=None):
+    """A context whose mixed mode uses the given sweep ("wave" or "lane"),
+    and for the wavefront sweep `group` wavefronts per workgroup."""
     import os
     import ocotillo_amd as oco
-    old = os.environ.get("OCO_MIXED_SWEEP")
-    os.environ["OCO_MIXED_SWEEP"] = kind
+    env = {"OCO_MIXED_SWEEP": kind}
+    if group is not None:
+        env["OCO_WAVE_GROUP"] = str(group)
+    old = {k: os.environ.get(k) for k in env}
+    os.environ.update(env)
     try:
         return oco.Context(0)
     finally:
-        if old is None:
-            del os.environ["OCO_MIXED_SWEEP"]
-        else:
-            os.environ["OCO_MIXED_SWEEP"] = old
+        for k, v in old.items():
+            if v is None:
+                del os.environ[k]
+            else:
+                os.environ[k] = v
 
 
 @pytest.mark.parametrize("nz", [4, 5, 63, 64, 65, 200, 256, 513, 1100])
@@ -186,36 +191,63 @@ def test_mixed_wave_and_lane_sweeps(nz):
     assert np.array_equal(out["wave"], out["lane"])   # same opacity code
 
 
-def test_wave_sweep_queues_cover_every_system():
-    """The wavefront sweep hands out systems from work queues (one per XCD
-    when its grid is a multiple of 8, else one), so which wavefront solves a
-    system varies from call to call; each system is still solved once and by
-    itself.  A 3 x 5 block (grid of 600 wavefronts; queues of one and two
-    columns) equals its columns solved one at a time (grid of 40: seven
-    empty queues) and with 37 wavelengths (grid of 37: one queue), bit for
-    bit, and a repeat of the block after those smaller calls equals the
-    first."""
+@pytest.mark.parametrize("group,nx,ny", [(1, 3, 5), (4, 4, 4)])
+def test_wave_sweep_queues_cover_every_system(group, nx, ny):
+    """The wavefront sweep hands out wavelength groups from work queues (one
+    per XCD when its grid is a multiple of 8, else one), so which workgroup
+    solves a system varies from call to call; each system is still solved
+    once and by itself.  The blocks here (3 x 5 columns, 40 wavelengths, one
+    wavefront per group: a grid of 600 and queues of one and two columns;
+    4 x 4 columns in groups of 4 wavelengths: a grid of 160 and queues of
+    two columns) equal their columns solved one at a time (one queue, or
+    seven empty ones) and with 37 wavelengths (one queue), bit for bit, and
+    a repeat of the block after those smaller calls equals the first."""
     import ocotillo_amd as oco
     from ocotillo_amd.workloads import config2
-    rho, T, dz, waves = config2(3, 5, 256, 40)
-    with sweep_context("wave") as c:
+    rho, T, dz, waves = config2(nx, ny, 256, 40)
+    with sweep_context("wave", group) as c:
         c.set_physics(2e4, False, 0.4)
         c.set_wavelengths(waves)
         U, V, A, d = c.solve_block(rho, T, dz, mode=oco.MODE_MIXED)
         assert d.status == 0 and np.isfinite(U).all() and np.isfinite(V).all()
-        for ix in range(3):
-            for iy in range(5):
+        for ix in range(nx):
+            for iy in range(ny):
                 u, v, a, _ = c.solve_block(rho[ix:ix + 1, iy:iy + 1], T[ix:ix + 1, iy:iy + 1], dz,
                                            mode=oco.MODE_MIXED)
                 assert np.array_equal(u[:, 0, 0], U[:, ix, iy]), (ix, iy)
                 assert np.array_equal(v[:, 0, 0], V[:, ix, iy]), (ix, iy)
                 assert np.array_equal(a[:, 0, 0], A[:, ix, iy]), (ix, iy)
         c.set_wavelengths(waves[:37])
-        u, v, a, _ = c.solve_block(rho[2:3, 4:5], T[2:3, 4:5], dz, mode=oco.MODE_MIXED)
-        assert np.array_equal(u[:, 0, 0], U[:37, 2, 4]) and np.array_equal(v[:, 0, 0], V[:37, 2, 4])
+        u, v, a, _ = c.solve_block(rho[2:3, 3:4], T[2:3, 3:4], dz, mode=oco.MODE_MIXED)
+        assert np.array_equal(u[:, 0, 0], U[:37, 2, 3]) and np.array_equal(v[:, 0, 0], V[:37, 2, 3])
         c.set_wavelengths(waves)
         U2, V2, A2, _ = c.solve_block(rho, T, dz, mode=oco.MODE_MIXED)
         assert np.array_equal(U2, U) and np.array_equal(V2, V) and np.array_equal(A2, A)
+
+
+@pytest.mark.parametrize("nz,nw", [(4, 5), (65, 40), (256, 37), (512, 130), (1080, 9)])
+def test_wave_groups_equal(nz, nw):
+    """The wavefront sweep's workgroups of 1, 2 and 4 wavefronts (each cell
+    record a thread loads evaluated for that many wavelengths) give the same
+    bits: the points and each system's solve are the same code whichever
+    workgroup runs them.  nw = 5, 37, 130 and 9 leave a last group of fewer
+    wavelengths than wavefronts."""
+    import ocotillo_amd as oco
+    from ocotillo_amd.workloads import config2
+    rho, T, dz, waves = config2(2, 3, nz, nw)
+    print("case nz", nz, "nw", nw, flush=True)
+    out = []
+    for g in (1, 2, 4):
+        with sweep_context("wave", g) as c:
+            c.set_wavelengths(waves)
+            c.set_physics(2e4, False, 0.4)
+            U, V, A, d = c.solve_block(rho, T, dz, mode=oco.MODE_MIXED)
+        assert d.status == 0 and np.isfinite(U).all() and np.isfinite(V).all()
+        out.append((U, V, A))
+    for g, (U, V, A) in zip((2, 4), out[1:]):
+        assert np.array_equal(U, out[0][0]), g
+        assert np.array_equal(V, out[0][1]), g
+        assert np.array_equal(A, out[0][2]), g
 
 
 def test_batch_mixed_equals_block(ctx):
