@@ -55,9 +55,9 @@ WM_IONISED = 143 + 7 * (WM_DIV - 1) + WM_OMEXP + WM_EXP10 + WM_EXPM1 + 3 * WM_LO
 # rocprofv3 FETCH_SIZE / WRITE_SIZE passes over one launch, scaled by the
 # copy-kernel calibration (x2.0 / x1.0); workloads not listed report None
 SWEEP_TRAFFIC_BYTES = {
-    (256, 256, 512, 128, "sweep_mixed_wave"): 1.89353e11,
+    (256, 256, 512, 128, "sweep_mixed_wave"): 1.11990e11,
     (256, 256, 512, 128, "sweep_mixed"): 3.40102e11,
-    (256, 256, 512, 128, "sweep_fp64"): 3.44112e11,
+    (256, 256, 512, 128, "sweep_fp64"): 3.38586e11,
 }
 
 
@@ -163,11 +163,11 @@ def main():
 
 def mixed_kernel_name(nz):
     """The mixed-mode sweep the library picks for columns of nz cells (the
-    wavefront-per-system sweep for 5 to 13 rows per lane, else the lane
-    sweep; OCO_MIXED_SWEEP=wave|lane forces one)."""
+    wavefront-per-system sweep above 128 cells, up to 17 rows per lane, else
+    the lane sweep; OCO_MIXED_SWEEP=wave|lane forces one)."""
     kind = os.environ.get("OCO_MIXED_SWEEP", "")
     rows = ((nz + 63) // 64) | 1
-    wave = kind == "wave" or (kind != "lane" and 5 <= rows <= 13)
+    wave = rows <= 17 and nz >= 4 and (kind == "wave" or (kind != "lane" and nz > 128))
     return "sweep_mixed_wave" if wave else "sweep_mixed"
 
 

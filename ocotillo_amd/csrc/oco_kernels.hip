@@ -233,6 +233,7 @@ __device__ static inline void boundary_row(const oco_pt& pc, double dz, double d
 __device__ static inline void rh_interior_row(double am, const oco_pt& pc, double ap, double dz2, double& A,
                                               double& C, double& H, double& R)
 {
+#pragma clang fp contract(fast)
     A = pc.alpha2 * mx_rcp(0.5 * (pc.alpha + am));
     C = pc.alpha2 * mx_rcp(0.5 * (ap + pc.alpha));
     H = (dz2 * (pc.alpha2 * pc.alpha)) * (1.0 - pc.albedo);
@@ -241,6 +242,7 @@ __device__ static inline void rh_interior_row(double am, const oco_pt& pc, doubl
 
 __device__ static inline void rh_boundary_row(const oco_pt& pc, double dz, double dz2, double& H, double& R)
 {
+#pragma clang fp contract(fast)
     const double ze = ((pc.alpha * dz2) * (1.0 - pc.albedo)) * 0.25;
     H = (dz + ze) * pc.alpha2;
     R = (pc.S * ze) * pc.alpha2;
@@ -890,6 +892,7 @@ __device__ static inline void wave_points(const double* cc, int nz, const double
 template <int MM>
 __device__ static inline void solve_system_wave(const oco_sys& d, double* lds)
 {
+#pragma clang fp contract(fast)   /* mixed mode: error bounds, not the reference's bits */
     const int nz = d.nz, lane = (int)threadIdx.x & (OCO_WAVE - 1);
     const double dz = d.dz, dz1 = d.dz1, dz2 = d.dz2;
     double* LA = lds;              /* alpha */
@@ -1125,19 +1128,16 @@ int oco_wave_group(int nz, int want, int device)
     return k;
 }
 
-/* Measured on MI355X on a 128 x 128-column block, nw = 128 (sweep ms, lane / wave):
- * nz 128 5.1/5.6, 256 10.5/9.3, 384 15.7/13.1, 512 21.1/17.5, 768 33.5/28.5,
- * 1024 42.4/46.2: the wavefront sweep is chosen for 5 to 13 rows per lane. */
-#ifndef OCO_WAVE_AUTO_MIN_ROWS
-#define OCO_WAVE_AUTO_MIN_ROWS 5
-#endif
-#ifndef OCO_WAVE_AUTO_ROWS
-#define OCO_WAVE_AUTO_ROWS 13
+/* Measured on MI355X on a 128 x 128-column block, nw = 128, with workgroups
+ * of 4 wavefronts (sweep ms, lane / wave): nz 64 2.17/3.75, 128 4.92/4.97,
+ * 192 7.47/5.18, 256 10.05/6.25, 1024 39.8/37.9: the wavefront sweep is
+ * chosen above 128 cells, up to its 17 rows per lane. */
+#ifndef OCO_WAVE_AUTO_MIN_NZ
+#define OCO_WAVE_AUTO_MIN_NZ 129
 #endif
 int oco_wave_preferred(int nz)
 {
-    const int m = wave_rows(nz);
-    return m >= OCO_WAVE_AUTO_MIN_ROWS && m <= OCO_WAVE_AUTO_ROWS;
+    return nz >= OCO_WAVE_AUTO_MIN_NZ && wave_rows(nz) <= OCO_WAVE_MAX_ROWS;
 }
 
 template <int K>

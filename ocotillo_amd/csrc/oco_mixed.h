@@ -95,6 +95,9 @@ __device__ static inline double mx_div(double x, double y) { return x * mx_rcp(y
 template <int WS>
 __device__ static inline oco_pt oco_point_mixed(const double* c, const double* w)
 {
+    /* products feeding sums may be fused (one rounding instead of two): the
+     * mixed mode is held to error bounds, not to the reference's bits */
+#pragma clang fp contract(fast)
     oco_pt p;
     double theta = c[CR_THETA], fn = c[CR_FN];
     double stim = mx_omexp((w[WT_CHI * WS] * theta) * OCO_LN10);
