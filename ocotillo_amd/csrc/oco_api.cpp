@@ -277,6 +277,7 @@ int oco_set_physics(oco_ctx* c, double fully_ionized_T, int lgrey, double sigma_
 /* Kernel and grid for sweeping nw_run wavelengths of ncol columns. */
 struct SweepPlan {
     int wave = 0, nblocks = 0;
+    int wg = 1;   /* lane sweeps: wavefronts per workgroup (each with its own scratch slot) */
 };
 
 static SweepPlan sweep_plan(const oco_ctx* c, int nz, long ncol, int nw_run, int mixed)
@@ -294,7 +295,9 @@ static SweepPlan sweep_plan(const oco_ctx* c, int nz, long ncol, int nw_run, int
         p.nblocks = (int)(ngroups < grid ? ngroups : grid);
     } else {
         const int grid = oco_sweep_grid(c->lgrey, mixed, (nw_run % 64) == 0, c->device);
-        p.nblocks = (int)(ntiles < grid ? ntiles : grid);
+        p.wg = oco_sweep_wg(mixed && !c->lgrey);
+        const long nb = (ntiles + p.wg - 1) / p.wg;
+        p.nblocks = (int)(nb < grid ? nb : grid);
     }
     return p;
 }
@@ -305,8 +308,8 @@ static int sweep_buffers(oco_ctx* c, const SweepPlan& p, int nz, long ncells)
     int st;
     if (!c->lgrey && (st = ensure(c, c->cells, (size_t)ncells * OCO_CELL_STRIDE * sizeof(double)))) return st;
     if (!p.wave) {
-        /* one scratch slot per wavefront; multi-wavefront workgroups round up */
-        const size_t slots = ((size_t)p.nblocks + 7) / 8 * 8;
+        /* one scratch slot per wavefront */
+        const size_t slots = (size_t)p.nblocks * p.wg;
         if ((st = ensure(c, c->scratch, slots * nz * 3 * 64 * sizeof(double)))) return st;
     }
     if ((st = ensure(c, c->tickets, OCO_TICKET_BYTES))) return st;

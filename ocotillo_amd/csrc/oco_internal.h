@@ -131,7 +131,10 @@ int oco_wave_preferred(int nz);
 /* wavefronts per workgroup of the wavefront sweep (want: 1, 2, 4, or 0 for the default) */
 int oco_wave_group(int nz, int want, int device);
 int oco_wave_grid(int nz, int k, int device);
+/* workgroups resident on the device, and wavefronts per workgroup, of the
+ * lane-per-system sweep of a mode (grey: the FP64 kernel in both modes) */
 int oco_sweep_grid(int lgrey, int mixed, int uniform, int device);
+int oco_sweep_wg(int mixed);
 
 /* host helpers shared by the C-ABI files */
 #ifdef __cplusplus

@@ -33,6 +33,9 @@ __constant__ oco_mtab oco_mtab_dev = OCO_MTAB_INIT;
 #define OCO_BACK_CHUNK 8   /* back-substitution rows whose scratch loads are batched */
 #endif
 
+/* lane of the calling thread in its wavefront */
+__device__ static inline int lane_id() { return (int)threadIdx.x & (OCO_WAVE - 1); }
+
 __device__ static inline void load_mtab(oco_mtab* smt)
 {
     const double* src = (const double*)&oco_mtab_dev;
@@ -172,7 +175,7 @@ __device__ static inline void wave_sync()
 template <int R>
 __device__ static inline void stage_put(double* buf, int idx, double v)
 {
-    buf[threadIdx.x * (R + 1) + (idx & (R - 1))] = v;
+    buf[lane_id() * (R + 1) + (idx & (R - 1))] = v;
 }
 
 /* Outputs are written once and not read back by the kernel: nontemporal
@@ -191,8 +194,8 @@ template <int R, bool NT = true>
 __device__ static inline void stage_flush(const double* buf, double* const* base, int b, int len)
 {
     constexpr int L = OCO_WAVE / R;
-    const int r = (int)threadIdx.x & (R - 1);
-    const int s0 = (int)threadIdx.x / R;
+    const int r = lane_id() & (R - 1);
+    const int s0 = lane_id() / R;
     const double* bp = buf + s0 * (R + 1) + r;
     double* const* pp = base + s0;
     const int idx = b * R + r;
@@ -289,7 +292,7 @@ template <bool MIX>
 __device__ static inline void scr_put(double* scr, int j, double gam, double u, double a1)
 {
     if constexpr (MIX) {
-        const int lane = threadIdx.x;
+        const int lane = lane_id();
         double* sp = scr + (long)j * SCR_ROW_MIXED;
         sp[0] = u;
         const double D = -gam;
@@ -310,7 +313,7 @@ template <bool MIX>
 __device__ static inline void scr_get(const double* scr, int j, double& gam, double& u, double& a1)
 {
     if constexpr (MIX) {
-        const int lane = threadIdx.x;
+        const int lane = lane_id();
         const double* sp = scr + (long)j * SCR_ROW_MIXED;
         u = sp[0];
         const float2 pk = *(const float2*)(sp + OCO_WAVE);
@@ -341,7 +344,7 @@ __device__ static inline bool solve_forward(const oco_sys& d, bool valid, const 
     const double dz = d.dz, dz2 = d.dz2;
     constexpr int R = STAGE::rows;
     constexpr bool EM = STAGE::emergent;
-    const int lane = threadIdx.x;
+    const int lane = lane_id();
     if constexpr (!EM) {
         wave_sync();   /* the previous system's flushes have read the buffers */
         S->base[2][lane] = valid ? d.A : nullptr;
@@ -498,7 +501,7 @@ __device__ static inline void solve_backward(const oco_sys& d, bool valid, const
     const double dz1 = d.dz1;
     constexpr int R = STAGE::rows;
     constexpr bool EM = STAGE::emergent;
-    const int lane = threadIdx.x;
+    const int lane = lane_id();
     if constexpr (!EM) {
         wave_sync();   /* the forward half's flushes have read the buffers */
         S->base[0][lane] = valid ? d.U : nullptr;
@@ -687,12 +690,12 @@ __global__ void tickets_reset(unsigned long long* t)
     if (threadIdx.x < OCO_TICKET_QUEUES) t[threadIdx.x * OCO_TICKET_STRIDE] = 0ull;
 }
 
-/* The first of the next n items of a queue, the same in every lane (one
- * wavefront per workgroup). */
+/* The first of the next n items of a queue, the same in every lane of the
+ * calling wavefront. */
 __device__ __forceinline__ long wave_take(unsigned long long* ctr, unsigned long long n)
 {
     unsigned long long v = 0ull;
-    if (threadIdx.x == 0) v = atomicAdd(ctr, n);
+    if (lane_id() == 0) v = atomicAdd(ctr, n);
     const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)v);
     const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(v >> 32));
     return (long)(((unsigned long long)hi << 32) | lo);
@@ -701,9 +704,11 @@ __device__ __forceinline__ long wave_take(unsigned long long* ctr, unsigned long
 template <bool GREY, bool UNIFORM, bool MIX, class STAGE>
 __device__ static inline void sweep_tiles(const oco_sweep_args& a, const oco_mtab* M, STAGE* S)
 {
-    const int lane = threadIdx.x;
+    const int lane = lane_id();
     const long ntiles = (a.ncol * a.nw + OCO_WAVE - 1) / OCO_WAVE;
-    double* scr = a.scratch + (long)blockIdx.x * a.nz * 3 * OCO_WAVE + lane;
+    /* one scratch slot per wavefront of the grid */
+    const long slot = (long)blockIdx.x * (blockDim.x / OCO_WAVE) + threadIdx.x / OCO_WAVE;
+    double* scr = a.scratch + slot * a.nz * 3 * OCO_WAVE + lane;
     /* one queue of tiles in order: the tiles in flight stay neighbours */
     for (long tile = wave_take(a.tickets, 1); tile < ntiles; tile = wave_take(a.tickets, 1)) {
         bool valid;
@@ -719,14 +724,20 @@ __device__ static inline void sweep_tiles(const oco_sweep_args& a, const oco_mta
 }
 
 /* FP64 drop-in sweep.  UNIFORM: nw is a multiple of 64, so a wavefront's
- * systems share one column. */
+ * systems share one column.  OCO_SWEEP_WG wavefronts per workgroup, each
+ * taking its own tiles, share one LDS copy of the math tables: with four,
+ * two workgroups (8 wavefronts, the register limit) fit a CU's 160 KB of LDS,
+ * where one-wavefront workgroups (24 KB each) fit 6. */
+#ifndef OCO_SWEEP_WG
+#define OCO_SWEEP_WG 4
+#endif
 template <bool GREY, bool UNIFORM>
-__global__ __launch_bounds__(OCO_WAVE) OCO_SWEEP_ATTR void sweep_fp64(const oco_sweep_args a)
+__global__ __launch_bounds__(OCO_WAVE * OCO_SWEEP_WG) OCO_SWEEP_ATTR void sweep_fp64(const oco_sweep_args a)
 {
     __shared__ oco_mtab smt;
-    __shared__ oco_stage stg;
+    __shared__ oco_stage stg[OCO_SWEEP_WG];
     load_mtab(&smt);
-    sweep_tiles<GREY, UNIFORM, false>(a, &smt, &stg);
+    sweep_tiles<GREY, UNIFORM, false>(a, &smt, &stg[threadIdx.x / OCO_WAVE]);
 }
 
 /* Mixed-precision sweep (non-grey): FP32-core opacity, FP64 elimination.
@@ -893,7 +904,7 @@ template <int MM>
 __device__ static inline void solve_system_wave(const oco_sys& d, double* lds)
 {
 #pragma clang fp contract(fast)   /* mixed mode: error bounds, not the reference's bits */
-    const int nz = d.nz, lane = (int)threadIdx.x & (OCO_WAVE - 1);
+    const int nz = d.nz, lane = lane_id();
     const double dz = d.dz, dz1 = d.dz1, dz2 = d.dz2;
     double* LA = lds;              /* alpha */
     double* LW = lds + nz;         /* albedo */
@@ -1202,7 +1213,7 @@ typedef const __attribute__((address_space(4))) double* oco_cptr_d;
 template <int BC, bool MIX, class STAGE>
 __device__ static inline void batch_tiles(const oco_batch_args& a, const oco_mtab* M, int* tile_s, STAGE* S)
 {
-    const int lane = threadIdx.x;
+    const int lane = lane_id();
     double* scr = a.scratch + (long)blockIdx.x * a.scr_stride + lane;
     oco_cptr_i tiles = (oco_cptr_i)a.tiles;
     oco_cptr_i nzs = (oco_cptr_i)a.nz;
@@ -1278,7 +1289,7 @@ __global__ __launch_bounds__(OCO_WAVE) void serve_tiles(const oco_serve_args a)
     __shared__ int tile_s;
     __shared__ oco_stage_emergent none;
     if (!MIX) load_mtab(&smt);
-    const int lane = threadIdx.x;
+    const int lane = lane_id();
     double* scr = a.scratch + (long)blockIdx.x * a.scr_stride + lane;
     oco_cptr_i tiles = (oco_cptr_i)a.tiles;
     for (;;) {
@@ -1519,15 +1530,23 @@ hipError_t oco_launch_cells(const double* rho, const double* T, long ncells, dou
 /* Workgroups of the sweep kernel that fit on the device at once (occupancy x
  * CUs): the grid-stride sweep is launched with exactly that many, so every
  * workgroup is resident from the start and takes an equal share of tiles. */
+/* wavefronts per workgroup of the lane-per-system sweeps */
+int oco_sweep_wg(int mixed)
+{
+    return mixed ? 1 : OCO_SWEEP_WG;
+}
+
+/* resident workgroups of a lane-per-system sweep */
 int oco_sweep_grid(int lgrey, int mixed, int uniform, int device)
 {
     const void* k;
     if (lgrey) k = (const void*)sweep_fp64<true, false>;
     else if (mixed) k = uniform ? (const void*)sweep_mixed<true> : (const void*)sweep_mixed<false>;
     else k = uniform ? (const void*)sweep_fp64<false, true> : (const void*)sweep_fp64<false, false>;
+    const int wg = oco_sweep_wg(mixed && !lgrey);
     int per_cu = 0, cus = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, OCO_WAVE, 0) != hipSuccess || per_cu < 1)
-        per_cu = 8;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, OCO_WAVE * wg, 0) != hipSuccess || per_cu < 1)
+        per_cu = 8 / wg > 0 ? 8 / wg : 1;
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess || cus < 1)
         cus = 256;
     return per_cu * cus;
@@ -1556,10 +1575,10 @@ hipError_t oco_launch_sweep(const oco_sweep_args& a, int lgrey, int mixed, int w
     }
     if (!a.tickets) return hipErrorInvalidValue;
     hipLaunchKernelGGL(tickets_reset, dim3(1), dim3(64), 0, stream, a.tickets);
-    const dim3 g(nblocks), b(OCO_WAVE);
+    const dim3 g(nblocks), b(OCO_WAVE), bf(OCO_WAVE * OCO_SWEEP_WG);
     if (lgrey) {
         /* grey opacity has no per-wavelength transcendentals: one kernel for both modes */
-        hipLaunchKernelGGL((sweep_fp64<true, false>), g, b, 0, stream, a);
+        hipLaunchKernelGGL((sweep_fp64<true, false>), g, bf, 0, stream, a);
     } else if (mixed && wave) {
         if (wave == 4) launch_wave<4>(a, nblocks, stream);
         else if (wave == 2) launch_wave<2>(a, nblocks, stream);
@@ -1568,9 +1587,9 @@ hipError_t oco_launch_sweep(const oco_sweep_args& a, int lgrey, int mixed, int w
         if (uniform) hipLaunchKernelGGL((sweep_mixed<true>), g, b, 0, stream, a);
         else hipLaunchKernelGGL((sweep_mixed<false>), g, b, 0, stream, a);
     } else if (uniform) {
-        hipLaunchKernelGGL((sweep_fp64<false, true>), g, b, 0, stream, a);
+        hipLaunchKernelGGL((sweep_fp64<false, true>), g, bf, 0, stream, a);
     } else {
-        hipLaunchKernelGGL((sweep_fp64<false, false>), g, b, 0, stream, a);
+        hipLaunchKernelGGL((sweep_fp64<false, false>), g, bf, 0, stream, a);
     }
     return hipGetLastError();
 }
