@@ -138,11 +138,12 @@ __device__ static inline void record_fail(unsigned long long* fail, long sys, in
 #ifndef OCO_STAGE_ROWS_MIXED
 #define OCO_STAGE_ROWS_MIXED 16
 #endif
-template <int R, bool NT = true>
+template <int R, bool NT = true, bool AR = false>
 struct oco_stage_t {
     static constexpr int rows = R;
     static constexpr bool emergent = false;
     static constexpr bool nt = NT;   /* outputs by nontemporal stores */
+    static constexpr bool reload = AR;   /* FP64: the back pass reads alpha from its output */
     double ub[OCO_WAVE * (R + 1)];   /* U rows; alpha rows during the forward pass */
     double vb[OCO_WAVE * (R + 1)];   /* V rows */
     double* base[3][OCO_WAVE];       /* per lane: U, V, alpha; null = lane stores nothing */
@@ -150,8 +151,18 @@ struct oco_stage_t {
 typedef oco_stage_t<OCO_STAGE_ROWS> oco_stage;
 typedef oco_stage_t<OCO_STAGE_ROWS_MIXED> oco_stage_mixed;
 /* The batched kernels keep ordinary stores (their columns' ragged nz and nw
- * leave many partial lines, which the L2 combines) and the FP64 one 8 rows. */
-typedef oco_stage_t<8, false> oco_stage_batch;
+ * leave many partial lines, which the L2 combines) and the FP64 one 8 rows.
+ * A batched column's alpha is one contiguous (nz, nw) array, so a lane's
+ * reads of its own alpha stay within a few pages of its neighbours': the
+ * FP64 batch kernel reads alpha back instead of keeping 1/alpha in scratch
+ * (measured on MI355X, 4096 uneven columns: 9.98 -> 9.39 ms).  In a block,
+ * the 64 lanes' alpha arrays are ncol*nz doubles apart, each lane's read a
+ * page of its own (256 x 256 x 512 block: 79 ms -> 238 ms), so the block
+ * sweep keeps 1/alpha in scratch. */
+#ifndef OCO_BATCH_ALPHA_RELOAD
+#define OCO_BATCH_ALPHA_RELOAD 1
+#endif
+typedef oco_stage_t<8, false, OCO_BATCH_ALPHA_RELOAD != 0> oco_stage_batch;
 typedef oco_stage_t<OCO_STAGE_ROWS_MIXED, false> oco_stage_batch_mixed;
 
 /* Output policy of the serving kernels: only the emergent flux V(1), V(nz)
@@ -160,6 +171,7 @@ struct oco_stage_emergent {
     static constexpr int rows = 8;
     static constexpr bool emergent = true;
     static constexpr bool nt = false;
+    static constexpr bool reload = false;
     double ub[1], vb[1];
     double* base[3][1];
 };
@@ -288,7 +300,11 @@ __device__ static inline oco_pt point_sys(const oco_sys& d, int z, const double*
  * 6e-8 on V.  `scr` is the lane's slot pointer (wave slot + lane). */
 constexpr long SCR_ROW_MIXED = 160;   /* doubles per row of a wave's mixed slot: 64 + 64 + 32 */
 
-template <bool MIX>
+/* AR (FP64 sweeps that write alpha): the scratch row is [gam, u], 16 bytes,
+ * and the back pass reads alpha from its output instead, 1/alpha being
+ * recomputed as the forward pass computed it (1.0 / alpha, correctly rounded,
+ * so the same bits). */
+template <bool MIX, bool AR = false>
 __device__ static inline void scr_put(double* scr, int j, double gam, double u, double a1)
 {
     if constexpr (MIX) {
@@ -301,6 +317,11 @@ __device__ static inline void scr_put(double* scr, int j, double gam, double u, 
         pk.y = (float)(D - (double)pk.x);
         *(float2*)(sp + OCO_WAVE) = pk;
         ((float*)(sp - lane + 2 * OCO_WAVE))[lane] = (float)a1;
+    } else if constexpr (AR) {
+        (void)a1;
+        double* sp = scr + (long)j * 2 * OCO_WAVE;
+        sp[0] = gam;
+        sp[OCO_WAVE] = u;
     } else {
         double* sp = scr + (long)j * 3 * OCO_WAVE;
         sp[0] = gam;
@@ -309,8 +330,11 @@ __device__ static inline void scr_put(double* scr, int j, double gam, double u, 
     }
 }
 
-template <bool MIX>
-__device__ static inline void scr_get(const double* scr, int j, double& gam, double& u, double& a1)
+/* a1: 1/alpha of row j (AR: alpha of row j, read from the lane's output A;
+ * the caller takes a1_of) */
+template <bool MIX, bool AR = false>
+__device__ static inline void scr_get(const double* scr, int j, double& gam, double& u, double& a1,
+                                      const double* A = nullptr)
 {
     if constexpr (MIX) {
         const int lane = lane_id();
@@ -319,13 +343,32 @@ __device__ static inline void scr_get(const double* scr, int j, double& gam, dou
         const float2 pk = *(const float2*)(sp + OCO_WAVE);
         gam = -((double)pk.x + (double)pk.y);
         a1 = (double)((const float*)(sp - lane + 2 * OCO_WAVE))[lane];
+    } else if constexpr (AR) {
+        const double* sp = scr + (long)j * 2 * OCO_WAVE;
+        gam = sp[0];
+        u = sp[OCO_WAVE];
+        a1 = A[j];
     } else {
+        (void)A;
         const double* sp = scr + (long)j * 3 * OCO_WAVE;
         gam = sp[0];
         u = sp[OCO_WAVE];
         a1 = sp[2 * OCO_WAVE];
     }
 }
+
+template <bool AR>
+__device__ static inline double a1_of(double v)
+{
+    if constexpr (AR) return 1.0 / v;   /* alpha -> 1/alpha, as oco_point's alpha1 */
+    else return v;
+}
+
+/* Whether a lane sweep with this mode and output policy reads alpha back
+ * (it is written by the forward pass) rather than keeping 1/alpha in
+ * scratch. */
+template <bool MIX, class STAGE>
+constexpr bool alpha_reload() { return !MIX && !STAGE::emergent && STAGE::reload; }
 
 /* One (column, wavelength) system per lane, in two halves.  `w` is the
  * lane's wavelength-table row with stride WS; `scr` its scratch column
@@ -383,7 +426,7 @@ __device__ static inline bool solve_forward(const oco_sys& d, bool valid, const 
         bet = bv;
         u = dv / bet;
     }
-    scr_put<MIX>(scr, 0, 0.0, u, pB.alpha1);
+    scr_put<MIX, alpha_reload<MIX, STAGE>()>(scr, 0, 0.0, u, pB.alpha1);
     if constexpr (!EM) stage_put<R>(S->ub, 0, pB.alpha);
     double aA = pB.alpha;   /* alpha of row j-1 */
     pB = pC;                /* row j = 1 */
@@ -411,7 +454,7 @@ __device__ static inline bool solve_forward(const oco_sys& d, bool valid, const 
             fail_stage = first ? 2 : fail_stage;
             u = (dv - av * u) / bet;
         }
-        scr_put<MIX>(scr, j, gam, u, pB.alpha1);
+        scr_put<MIX, alpha_reload<MIX, STAGE>()>(scr, j, gam, u, pB.alpha1);
         if constexpr (!EM) {
             stage_put<R>(S->ub, j, pB.alpha);
             if ((j & (R - 1)) == R - 1) {
@@ -447,7 +490,7 @@ __device__ static inline bool solve_forward(const oco_sys& d, bool valid, const 
             fail_stage = first ? 2 : fail_stage;
             u = (dv - av * u) / bet;
         }
-        scr_put<MIX>(scr, j, gam, u, pB.alpha1);
+        scr_put<MIX, alpha_reload<MIX, STAGE>()>(scr, j, gam, u, pB.alpha1);
         if constexpr (!EM) {
             stage_put<R>(S->ub, j, pB.alpha);
             if ((j & (R - 1)) == R - 1) {
@@ -474,7 +517,7 @@ __device__ static inline bool solve_forward(const oco_sys& d, bool valid, const 
             fail_stage = first ? 2 : fail_stage;
             u = (dv - av * u) / bet;
         }
-        scr_put<MIX>(scr, j, gam, u, pC.alpha1);
+        scr_put<MIX, alpha_reload<MIX, STAGE>()>(scr, j, gam, u, pC.alpha1);
         if constexpr (!EM) {
             stage_put<R>(S->ub, j, pC.alpha);
             wave_sync();
@@ -513,17 +556,19 @@ __device__ static inline void solve_backward(const oco_sys& d, bool valid, const
     const double c60 = 1.0 / 60.0;
     /* window w0..w6 = extended U at e = j .. j+6 (e = z + 3) */
     double gnext, a1q0, Un;              /* gam of the row above; 1/alpha of the V row pending */
-    scr_get<MIX>(scr, nz - 1, gnext, Un, a1q0);   /* U(nz) = u(nz) (:43) */
+    constexpr bool AR = alpha_reload<MIX, STAGE>();
+    scr_get<MIX, AR>(scr, nz - 1, gnext, Un, a1q0, d.A);   /* U(nz) = u(nz) (:43) */
+    a1q0 = a1_of<AR>(a1q0);
     double a1q1 = 0.0, a1q2 = 0.0, a1q3 = 0.0;
     double t[4];
     t[0] = Un;
     for (int k = 1; k < 4; k++) {
         int j = nz - 1 - k;
         double gj, uj, a1;
-        scr_get<MIX>(scr, j, gj, uj, a1);
+        scr_get<MIX, AR>(scr, j, gj, uj, a1, d.A);
         double Uj = uj - gnext * t[k - 1];
         gnext = gj;
-        a1q3 = a1q2; a1q2 = a1q1; a1q1 = a1q0; a1q0 = a1;
+        a1q3 = a1q2; a1q2 = a1q1; a1q1 = a1q0; a1q0 = a1_of<AR>(a1);
         t[k] = Uj;
     }
     /* top ghosts (f(n2+i) = 2 f(n2) - f(n2-i)) */
@@ -568,13 +613,13 @@ __device__ static inline void solve_backward(const oco_sys& d, bool valid, const
 #pragma unroll
         for (int k = 0; k < BC; k++) {
             int j = jc - k;
-            scr_get<MIX>(scr, j < 0 ? 0 : j, cg[k], cu[k], ca[k]);
+            scr_get<MIX, AR>(scr, j < 0 ? 0 : j, cg[k], cu[k], ca[k], d.A);
         }
         for (; jc >= 0; jc -= BC) {
 #pragma unroll
             for (int k = 0; k < BC; k++) {      /* next chunk, in flight */
                 int j = jc - BC - k;
-                scr_get<MIX>(scr, j < 0 ? 0 : j, ng[k], nu[k], na[k]);
+                scr_get<MIX, AR>(scr, j < 0 ? 0 : j, ng[k], nu[k], na[k], d.A);
             }
 #pragma unroll
             for (int k = 0; k < BC; k++) {
@@ -582,7 +627,7 @@ __device__ static inline void solve_backward(const oco_sys& d, bool valid, const
                 if (j >= 0) {
                     double Uj = cu[k] - gnext * w0;
                     gnext = cg[k];
-                    a1q3 = a1q2; a1q2 = a1q1; a1q1 = a1q0; a1q0 = ca[k];
+                    a1q3 = a1q2; a1q2 = a1q1; a1q1 = a1q0; a1q0 = a1_of<AR>(ca[k]);
                     const int e = j + 3;   /* U(e) and V(z = e) */
                     w6 = w5; w5 = w4; w4 = w3; w3 = w2; w2 = w1; w1 = w0; w0 = Uj;
                     if constexpr (!EM) {
@@ -637,6 +682,10 @@ __device__ static inline void solve_system(const oco_sys& d, bool valid, const d
                                            unsigned long long* fail, const oco_mtab* M, STAGE* S)
 {
     const bool stopped = solve_forward<GREY, UNIFORM, WS, MIX, STAGE>(d, valid, w, scr, fail, M, S);
+    /* the back pass reads alpha where the forward pass stored it, through
+     * other lanes of this wavefront (the staged flushes): a workgroup-scope
+     * fence makes those stores visible to this workgroup's loads */
+    if constexpr (alpha_reload<MIX, STAGE>()) __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
     solve_backward<BC, MIX, STAGE>(d, valid && !stopped, scr, S);
 }
 
