@@ -638,7 +638,10 @@ def bench_batch(args):
         t1 = time.perf_counter()
         ms_per_step = (t1 - t0) * 1e3 / args.steps
         variant, nblocks, _ = b.tuning(code)
-        kern = "batch_mixed (graph)" if mode == "mixed" else "batch_fp64 (graph)"
+        # mixed variants 3, 4, 5: the wavefront-per-system kernel in workgroups of 1, 2, 4
+        kern = ("batch_fp64 (graph)" if mode == "fp64" else
+                "batch_mixed_wave, %d wavefronts/workgroup (graph)" % (1 << (variant - 3)) if variant >= 3 else
+                "batch_mixed (graph)")
         res[mode] = {"value": points / (ms_per_step * 1e-3), "ms_per_step": ms_per_step,
                      "roofline": roofline_of(kern, points, point_weights(mode, fion), float(np.mean(ms))),
                      "variant": variant, "workgroups": nblocks}

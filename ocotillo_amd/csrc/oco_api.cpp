@@ -867,7 +867,15 @@ struct oco_batch {
     std::vector<int> nz, nw;
     std::vector<long> woff;
     DevBuf ints, longs, dz, wtab, rho, T, cells, U, V, A, scratch, counter, fail;
-    /* per precision mode: autotuned kernel variant and grid, captured graph */
+    /* mixed mode, wavefront per system (when every column fits it): item
+     * lists for workgroups of 1, 2 and 4 wavefronts, their queue tickets and
+     * the field-major records */
+    DevBuf witems, wtickets, cells_soa;
+    bool wave_ok = false;
+    long wio[3] = {0, 0, 0}, wqo[3] = {0, 0, 0};
+    /* per precision mode: autotuned kernel variant and grid, captured graph
+     * (mixed variants 0..2: lane per system; 3, 4, 5: wavefront per system in
+     * workgroups of 1, 2, 4) */
     int nblocks[2] = {0, 0}, variant[2] = {1, 1}, tuned[2] = {0, 0};
     float tune_ms[2][16] = {{0}};
     hipGraph_t graph[2] = {nullptr, nullptr};
@@ -878,7 +886,7 @@ static void batch_release(oco_batch* b)
 {
     oco_ctx* c = b->ctx;
     for (DevBuf* d : {&b->ints, &b->longs, &b->dz, &b->wtab, &b->rho, &b->T, &b->cells, &b->U, &b->V, &b->A,
-                      &b->scratch, &b->counter, &b->fail})
+                      &b->scratch, &b->counter, &b->fail, &b->witems, &b->wtickets, &b->cells_soa})
         release(c, *d);
     for (int m = 0; m < 2; m++) {
         if (b->exec[m]) (void)hipGraphExecDestroy(b->exec[m]);
@@ -912,13 +920,28 @@ static oco_batch_args batch_args(oco_batch* b)
     a.scr_stride = (long)b->max_nz * 3 * 64;
     a.counter = (unsigned int*)b->counter.p;
     a.fail = (unsigned long long*)b->fail.p;
+    a.witems = nullptr;
+    a.wq_off = nullptr;
+    a.wtickets = (unsigned long long*)b->wtickets.p;
+    a.cells_soa = (const double*)b->cells_soa.p;
+    a.max_nz = b->max_nz;
     return a;
 }
+
+static const int BATCH_WAVE_K[3] = {1, 2, 4};
+#define OCO_BATCH_LANE_VARIANTS 3
 
 /* enqueue one full batch solve with the given kernel variant / grid */
 static hipError_t batch_enqueue(oco_batch* b, int mixed, int variant, int nblocks)
 {
     oco_ctx* c = b->ctx;
+    if (mixed && variant >= OCO_BATCH_LANE_VARIANTS) {
+        const int i = variant - OCO_BATCH_LANE_VARIANTS;
+        oco_batch_args a = batch_args(b);
+        a.witems = (const int*)b->witems.p + b->wio[i];
+        a.wq_off = (const int*)b->witems.p + b->wqo[i];
+        return oco_launch_batch_wave(a, b->ncol, c->fully_ionized_T, BATCH_WAVE_K[i], nblocks, c->stream);
+    }
     hipError_t e = oco_launch_cells((const double*)b->rho.p, (const double*)b->T.p, b->ncells, c->fully_ionized_T,
                                     (double*)b->cells.p, 0, c->stream);
     if (e != hipSuccess) return e;
@@ -986,6 +1009,35 @@ int oco_batch_create(oco_ctx* c, int ncol, const int* nz, const int* nw, const d
         oco_build_wtab(g, one);
         std::copy(one.begin(), one.end(), wt.begin() + woff[i] * OCO_WT_STRIDE);
     }
+    /* mixed mode, wavefront per system: for workgroups of K wavefronts, items
+     * (column, first of K wavelengths); column p of the longest-first order
+     * goes to queue p mod 8, so each XCD's L2 holds its own columns' records */
+    std::vector<int> witems;
+    b->wave_ok = c->mixed_sweep != 2 && oco_batch_wave_grid(b->max_nz, 1, c->device) > 0;
+    if (b->wave_ok) {
+        for (int i = 0; i < 3; i++) {
+            const int K = BATCH_WAVE_K[i];
+            std::vector<int> q[OCO_TICKET_QUEUES];
+            for (int p = 0; p < ncol; p++) {
+                const int col = order[p];
+                for (int w0 = 0; w0 < nw[col]; w0 += K) {
+                    q[p % OCO_TICKET_QUEUES].push_back(col);
+                    q[p % OCO_TICKET_QUEUES].push_back(w0);
+                }
+            }
+            b->wio[i] = (long)witems.size();
+            int n = 0;
+            std::vector<int> qoff(OCO_TICKET_QUEUES + 1);
+            for (int k = 0; k < OCO_TICKET_QUEUES; k++) {
+                qoff[k] = n;
+                n += (int)(q[k].size() / 2);
+                witems.insert(witems.end(), q[k].begin(), q[k].end());
+            }
+            qoff[OCO_TICKET_QUEUES] = n;
+            b->wqo[i] = (long)witems.size();
+            witems.insert(witems.end(), qoff.begin(), qoff.end());
+        }
+    }
     int dev_cus = 256;
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, c->device) == hipSuccess && prop.multiProcessorCount > 0)
@@ -1012,8 +1064,16 @@ int oco_batch_create(oco_ctx* c, int ncol, const int* nz, const int* nw, const d
     if (!st) st = ensure(c, b->scratch, (size_t)std::min(b->ntiles, max_blocks) * b->max_nz * 3 * 64 * sizeof(double));
     if (!st) st = ensure(c, b->counter, sizeof(unsigned int));
     if (!st) st = ensure(c, b->fail, sizeof(unsigned long long));
+    if (!st && b->wave_ok) {
+        st = ensure(c, b->witems, witems.size() * sizeof(int));
+        if (!st) st = ensure(c, b->wtickets, OCO_TICKET_BYTES);
+        if (!st) st = ensure(c, b->cells_soa, (size_t)nc * CR_NFIELD * sizeof(double));
+    }
     if (!st) {
         hipError_t e = hipMemcpyAsync(b->ints.p, ints.data(), ints.size() * sizeof(int), hipMemcpyHostToDevice, c->stream);
+        if (e == hipSuccess && b->wave_ok)
+            e = hipMemcpyAsync(b->witems.p, witems.data(), witems.size() * sizeof(int), hipMemcpyHostToDevice,
+                               c->stream);
         if (e == hipSuccess)
             e = hipMemcpyAsync(b->longs.p, longs.data(), longs.size() * sizeof(long), hipMemcpyHostToDevice, c->stream);
         if (e == hipSuccess)
@@ -1084,9 +1144,16 @@ static int batch_prepare(oco_batch* b, int m)
             dev_cus = prop.multiProcessorCount;
         float best = 1e30f;
         int k = 0;
-        for (int v = 0; v < oco_batch_variants(); v++) {
-            for (int wps = 2; wps <= 4; wps++) {
-                int nb = std::min(b->ntiles, dev_cus * 4 * wps);
+        const int nvar = oco_batch_variants() + (m == OCO_MODE_MIXED && b->wave_ok ? 3 : 0);
+        /* OCO_MIXED_SWEEP=wave: only the wavefront variants */
+        const int v0 = m == OCO_MODE_MIXED && b->wave_ok && c->mixed_sweep == 1 ? OCO_BATCH_LANE_VARIANTS : 0;
+        for (int v = v0; v < nvar; v++) {
+            const bool wave = v >= OCO_BATCH_LANE_VARIANTS;
+            for (int wps = 2; wps <= (wave ? 2 : 4); wps++) {
+                /* wavefront variants: the occupancy grid (one try) */
+                int nb = wave ? oco_batch_wave_grid(b->max_nz, BATCH_WAVE_K[v - OCO_BATCH_LANE_VARIANTS], c->device)
+                              : std::min(b->ntiles, dev_cus * 4 * wps);
+                if (nb <= 0) continue;
                 HIPCHK(hipEventRecord(c->ev[0], c->stream));
                 HIPCHK(batch_enqueue(b, m, v, nb));
                 HIPCHK(hipEventRecord(c->ev[1], c->stream));

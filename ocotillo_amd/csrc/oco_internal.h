@@ -54,6 +54,15 @@ struct oco_batch_args {
     long scr_stride;       /* doubles per wavefront slot: max_nz * 3 * 64 */
     unsigned int* counter;
     unsigned long long* fail;
+    /* mixed mode, wavefront-per-system kernel: items (column, first of the
+     * group's wavelengths) in OCO_TICKET_QUEUES queues, queue q holding
+     * items [wq_off[q], wq_off[q + 1]); records field-major per column
+     * (field k of cell z of column c at cell_off[c]*CR_NFIELD + k*nz[c] + z) */
+    const int* witems;
+    const int* wq_off;
+    unsigned long long* wtickets;   /* OCO_TICKET_QUEUES counters, OCO_TICKET_STRIDE apart */
+    const double* cells_soa;
+    int max_nz;
 };
 
 /* Serving: one admitted job's device descriptor. */
@@ -116,6 +125,13 @@ int oco_serve_grid(int mixed, int device);
 hipError_t oco_launch_fail_reset(unsigned long long* fail, int n, hipStream_t stream);
 /* mixed != 0 selects the mixed-precision kernels (non-grey only). */
 hipError_t oco_launch_batch(const oco_batch_args& a, int mixed, int variant, int nblocks, hipStream_t stream);
+/* batched mixed mode, wavefront per system, k (1, 2, 4) wavefronts per
+ * workgroup: per-column field-major records, then the sweep */
+hipError_t oco_launch_batch_wave(const oco_batch_args& a, int ncol, double fully_ionized_T, int k, int nblocks,
+                                 hipStream_t stream);
+/* resident workgroups of that sweep for columns of at most max_nz cells; 0 if
+ * such columns do not fit it */
+int oco_batch_wave_grid(int max_nz, int k, int device);
 int oco_batch_variants(void);
 /* soa_nz > 0: field-major records per column of soa_nz cells (field k of
  * cell z of column c at (c*CR_NFIELD + k)*soa_nz + z); 0: cell-major */

@@ -910,16 +910,16 @@ __host__ __device__ static inline long wave_lds_words(int nz) { return 3L * nz +
  * workgroup (source/continuous_opacity.f90:175-201): thread t takes cells
  * z = t, t + 64 K, ..., loads the cell's record once and evaluates it for
  * each of the nk wavelengths iw0 .. iw0 + nk - 1, into wavefront k's LDS
- * region (alpha, albedo, S) and the alpha output of system k.  The column's
+ * region (alpha, albedo, S; regions lstr >= wave_lds_words(nz) doubles
+ * apart) and the alpha output of system k.  The column's
  * field-major records (field k of cell z at cc[k*nz + z]) are read through a
  * buffer descriptor: the field offset is a scalar, the lane offset a vector
  * register, no per-field address arithmetic (loads past the column's records
  * return 0). */
 template <int K>
 __device__ static inline void wave_points(const double* cc, int nz, const double* wtab, int iw0, int nk,
-                                          double* A0, long astride, double* lds)
+                                          double* A0, long astride, double* lds, long lstr)
 {
-    const long lstr = wave_lds_words(nz);
     const __amdgpu_buffer_rsrc_t crs =
         __builtin_amdgcn_make_buffer_rsrc((void*)cc, (short)0, CR_NFIELD * nz * (int)sizeof(double), OCO_BUF_DWORD3);
 #ifdef OCO_WAVE_POINTS_UNROLL
@@ -1132,7 +1132,7 @@ __global__ __launch_bounds__(OCO_WAVE * K) OCO_WAVE_ATTR void sweep_mixed_wave(c
                 const int nk = nw - iw0 < K ? nw - iw0 : K;
                 if (t > t0) __syncthreads();   /* the previous group's systems are done with LDS */
                 wave_points<K>(a.cells + col * (long)CR_NFIELD * nz, nz, a.wtab, iw0, nk,
-                               a.A + ((long)iw0 * a.ncol + col) * nz, astride, lds);
+                               a.A + ((long)iw0 * a.ncol + col) * nz, astride, lds, wave_lds_words(nz));
                 __syncthreads();
                 if (wid < nk) {
                     const int iw = iw0 + wid;
@@ -1410,6 +1410,138 @@ hipError_t oco_launch_batch(const oco_batch_args& a, int mixed, int variant, int
 }
 
 int oco_batch_variants(void) { return 3; }
+
+/* Cell records of a batch, field-major per column (the wavefront sweep's
+ * layout with each column's own nz): one workgroup per column. */
+__global__ __launch_bounds__(256) void cells_cols_fp64(const double* __restrict__ rho, const double* __restrict__ T,
+                                                       const int* __restrict__ nzs, const long* __restrict__ coff,
+                                                       double fully_ionized_T, double* __restrict__ cells)
+{
+    __shared__ oco_mtab smt;
+    load_mtab(&smt);
+    const int col = blockIdx.x, nz = nzs[col];
+    const long c0 = coff[col];
+    double* out = cells + c0 * CR_NFIELD;
+    for (int z = threadIdx.x; z < nz; z += blockDim.x) {
+        double rec[CR_NFIELD];
+        oco_cell_record(rho[c0 + z], T[c0 + z], fully_ionized_T, &smt, rec);
+#pragma unroll
+        for (int k = 0; k < CR_NFIELD; k++) out[(long)k * nz + z] = rec[k];
+    }
+}
+
+/* Batched mixed mode, one wavefront per system, K wavefronts per workgroup on
+ * K wavelengths of one column (the block sweep's wave_points and
+ * solve_system_wave, so a column gives the same bits as in a block of its
+ * own).  Items are drawn from per-XCD queues of whole columns, longest
+ * first; a workgroup whose queue is empty goes on to the next XCD's.  Each
+ * wavefront's LDS region is sized for the batch's longest column. */
+template <int MM, int K>
+__global__ __launch_bounds__(OCO_WAVE * K) void batch_mixed_wave(const oco_batch_args a)
+{
+    extern __shared__ double lds[];
+    __shared__ int ticket;
+    const int wid = (int)threadIdx.x / OCO_WAVE;
+    oco_cptr_i items = (oco_cptr_i)a.witems;
+    oco_cptr_i qoff = (oco_cptr_i)a.wq_off;
+    oco_cptr_i nzs = (oco_cptr_i)a.nz;
+    oco_cptr_i nws = (oco_cptr_i)a.nw;
+    oco_cptr_l coff = (oco_cptr_l)a.cell_off;
+    oco_cptr_l woff = (oco_cptr_l)a.wt_off;
+    oco_cptr_l uoff = (oco_cptr_l)a.u_off;
+    oco_cptr_l voff = (oco_cptr_l)a.v_off;
+    oco_cptr_d dzs = (oco_cptr_d)a.dz;
+    const long lstr = wave_lds_words(a.max_nz);
+    const int nq = gridDim.x % OCO_XCDS == 0 ? OCO_XCDS : 1;
+    const int home = (int)(blockIdx.x % nq);
+    for (int v = 0; v < nq; v++) {
+        const int q = (home + v) % nq;
+        const int i0 = nq == 1 ? 0 : qoff[q];
+        const int i1 = nq == 1 ? qoff[OCO_TICKET_QUEUES] : qoff[q + 1];
+        unsigned long long* ctr = a.wtickets + q * OCO_TICKET_STRIDE;
+        for (;;) {
+            __syncthreads();   /* the last item's systems are done with LDS, and every thread has read `ticket` */
+            if (threadIdx.x == 0) ticket = (int)atomicAdd(ctr, 1ull);
+            __syncthreads();
+            const int t = i0 + ticket;
+            if (t >= i1) break;
+            const int col = items[2 * t], iw0 = items[2 * t + 1];
+            const int nz = nzs[col], nw = nws[col];
+            const int nk = nw - iw0 < K ? nw - iw0 : K;
+            const long c0 = coff[col], v0 = voff[col];
+            wave_points<K>(a.cells_soa + c0 * CR_NFIELD, nz, a.wtab + woff[col] * OCO_WT_STRIDE, iw0, nk,
+                           a.A + v0 + (long)iw0 * nz, nz, lds, lstr);
+            __syncthreads();
+            if (wid < nk) {
+                const int iw = iw0 + wid;
+                oco_sys d;
+                d.nz = nz;
+                d.dz = dzs[3 * col]; d.dz1 = dzs[3 * col + 1]; d.dz2 = dzs[3 * col + 2]; d.sigma_grey = 0.0;
+                d.cells = nullptr; d.rho = nullptr; d.T = nullptr;
+                d.U = a.U + uoff[col] + (long)iw * (nz + 6);
+                d.V = a.V + v0 + (long)iw * nz;
+                d.A = a.A + v0 + (long)iw * nz;
+                d.key = woff[col] + iw;
+                solve_system_wave<MM>(d, lds + wid * lstr);
+            }
+        }
+    }
+}
+
+template <int K>
+static const void* batch_wave_kernel_k(int max_nz)
+{
+    const int m = wave_rows(max_nz);
+    if (m <= 5) return (const void*)batch_mixed_wave<5, K>;
+    if (m <= 9) return (const void*)batch_mixed_wave<9, K>;
+    return (const void*)batch_mixed_wave<17, K>;
+}
+
+static const void* batch_wave_kernel(int max_nz, int k)
+{
+    return k == 4 ? batch_wave_kernel_k<4>(max_nz) : k == 2 ? batch_wave_kernel_k<2>(max_nz)
+                                                            : batch_wave_kernel_k<1>(max_nz);
+}
+
+int oco_batch_wave_grid(int max_nz, int k, int device)
+{
+    if (max_nz < 4 || wave_rows(max_nz) > OCO_WAVE_MAX_ROWS) return 0;
+    if (wave_lds_bytes(max_nz, k) > (size_t)lds_limit(device)) return 0;
+    int per_cu = 0, cus = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, batch_wave_kernel(max_nz, k), OCO_WAVE * k,
+                                                     wave_lds_bytes(max_nz, k)) != hipSuccess || per_cu < 1)
+        per_cu = 1;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess || cus < 1)
+        cus = 256;
+    const int g = per_cu * cus;
+    return g >= OCO_XCDS ? g / OCO_XCDS * OCO_XCDS : g;
+}
+
+template <int K>
+static void launch_batch_wave(const oco_batch_args& a, int nblocks, hipStream_t stream)
+{
+    const int m = wave_rows(a.max_nz);
+    const dim3 g(nblocks), b(OCO_WAVE * K);
+    const size_t sh = wave_lds_bytes(a.max_nz, K);
+    if (m <= 5) hipLaunchKernelGGL((batch_mixed_wave<5, K>), g, b, sh, stream, a);
+    else if (m <= 9) hipLaunchKernelGGL((batch_mixed_wave<9, K>), g, b, sh, stream, a);
+    else hipLaunchKernelGGL((batch_mixed_wave<17, K>), g, b, sh, stream, a);
+}
+
+hipError_t oco_launch_batch_wave(const oco_batch_args& a, int ncol, double fully_ionized_T, int k, int nblocks,
+                                 hipStream_t stream)
+{
+    (void)hipGetLastError();   /* a pending not-ready status from an event query is not this launch's */
+    if (!a.wtickets || !a.cells_soa || ncol <= 0 || nblocks <= 0) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(batch_reset, dim3(1), dim3(64), 0, stream, a.counter, a.fail);
+    hipLaunchKernelGGL(tickets_reset, dim3(1), dim3(64), 0, stream, a.wtickets);
+    hipLaunchKernelGGL(cells_cols_fp64, dim3(ncol), dim3(256), 0, stream, a.rho, a.T, a.nz, a.cell_off,
+                       fully_ionized_T, (double*)a.cells_soa);
+    if (k == 4) launch_batch_wave<4>(a, nblocks, stream);
+    else if (k == 2) launch_batch_wave<2>(a, nblocks, stream);
+    else launch_batch_wave<1>(a, nblocks, stream);
+    return hipGetLastError();
+}
 
 /* Athena input construction: one z slab (one id<iproc> file) of raw code-unit
  * fields into the block's rho, T -- the arithmetic of

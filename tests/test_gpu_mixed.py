@@ -250,21 +250,20 @@ def test_wave_groups_equal(nz, nw):
         assert np.array_equal(A, out[0][2]), g
 
 
-def test_batch_mixed_equals_block(ctx):
+@pytest.mark.parametrize("kind", ["lane", "wave"])
+def test_batch_mixed_equals_block(kind):
     """Batched (persistent, graph-replayed) mixed solve == the block mixed
-    solve with the same (lane-per-system) kernel, bit for bit, on every
-    replay."""
+    solve with the same kernel kind (OCO_MIXED_SWEEP restricts the batch's
+    autotune to it), bit for bit, on every replay."""
     import ocotillo_amd as oco
     from ocotillo_amd.workloads import config2
     rho, T, dz, waves = config2(2, 3, 96, 80)
     ncol = 6
     nz, nw = rho.shape[2], waves.size
+    ctx = sweep_context(kind)
     ctx.set_wavelengths(waves)
     ctx.set_physics(2e4, False, 0.4)
-    with sweep_context("lane") as lc:
-        lc.set_wavelengths(waves)
-        lc.set_physics(2e4, False, 0.4)
-        U0, V0, A0, _ = lc.solve_block(rho, T, dz, mode=oco.MODE_MIXED)
+    U0, V0, A0, _ = ctx.solve_block(rho, T, dz, mode=oco.MODE_MIXED)
     b = oco.Batch(ctx, [nz] * ncol, [nw] * ncol, np.tile(waves, ncol), [dz] * ncol)
     try:
         for rep in range(3):
@@ -278,6 +277,48 @@ def test_batch_mixed_equals_block(ctx):
                 assert np.array_equal(As[c], A0[:, ix, iy]), (rep, c, "A")
             # FP64 run in between: the two modes keep separate graphs
             b.run_host(rho.reshape(-1), T.reshape(-1), mode=oco.MODE_FP64)
-        assert b.tuning(oco.MODE_MIXED)[2] == 1
+        v, nb, tuned = b.tuning(oco.MODE_MIXED)
+        assert tuned == 1 and (v >= 3) == (kind == "wave")
     finally:
         b.close()
+        ctx.close()
+
+
+def test_batch_mixed_wave_uneven_columns():
+    """Batched mixed mode on the wavefront-per-system kernel, uneven columns
+    (nz 4..700, nw 1..140; the kernel is instantiated for the longest
+    column): every column equals that column solved as a block of its own by
+    the block wavefront sweep, bit for bit, on every graph replay, with the
+    context's own eager work between replays."""
+    import ocotillo_amd as oco
+    from ocotillo_amd.workloads import config3
+    nz, nw, waves, dz, rho, T = config3(ncol=14, seed=78, nz_range=(4, 700), nw_range=(1, 140))
+    print("batch nz", nz.tolist(), "nw", nw.tolist(), flush=True)
+    with sweep_context("wave") as c:
+        c.set_physics(2e4, False, 0.4)
+        b = oco.Batch(c, nz, nw, waves, dz)
+        try:
+            ref = []
+            c0 = w0 = 0
+            for n, m, d in zip(nz.tolist(), nw.tolist(), dz.tolist()):
+                c.set_wavelengths(waves[w0:w0 + m])
+                u, v, a, dg = c.solve_block(rho[c0:c0 + n].reshape(1, 1, n), T[c0:c0 + n].reshape(1, 1, n), d,
+                                            mode=oco.MODE_MIXED)
+                assert dg.status == 0
+                ref.append((u[:, 0, 0], v[:, 0, 0], a[:, 0, 0]))
+                c0 += n
+                w0 += m
+            for rep in range(3):
+                U, V, A, d = b.run_host(rho, T, mode=oco.MODE_MIXED)
+                assert d.status == 0
+                Us, Vs, As = b.split(U, V, A)
+                for i, (ru, rv, ra) in enumerate(ref):
+                    assert np.array_equal(Us[i], ru), (rep, i, "U")
+                    assert np.array_equal(Vs[i], rv), (rep, i, "V")
+                    assert np.array_equal(As[i], ra), (rep, i, "A")
+                c.solve_block(rho[:nz[0]].reshape(1, 1, -1), T[:nz[0]].reshape(1, 1, -1), float(dz[0]),
+                              mode=oco.MODE_MIXED)
+            v, nb, tuned = b.tuning(oco.MODE_MIXED)
+            assert tuned == 1 and v >= 3 and nb > 0
+        finally:
+            b.close()
