@@ -138,31 +138,44 @@ __device__ static inline void record_fail(unsigned long long* fail, long sys, in
 #ifndef OCO_STAGE_ROWS_MIXED
 #define OCO_STAGE_ROWS_MIXED 16
 #endif
-template <int R, bool NT = true, bool AR = false>
+/* How an FP64 back pass gets 1/alpha: OCO_A1_SCRATCH from the scratch row;
+ * OCO_A1_DIRECT by reading alpha from the lane's own output; OCO_A1_STAGED
+ * by loading alpha blocks of R rows into the V stage, coalesced (the reverse
+ * of the output flush), a block ahead.  The last two recompute 1/alpha as
+ * the forward pass computed it (1.0 / alpha, correctly rounded: the same
+ * bits) and keep [gam, u] only in scratch. */
+#define OCO_A1_SCRATCH 0
+#define OCO_A1_DIRECT 1
+#define OCO_A1_STAGED 2
+template <int R, bool NT = true, int AR = OCO_A1_SCRATCH>
 struct oco_stage_t {
     static constexpr int rows = R;
     static constexpr bool emergent = false;
     static constexpr bool nt = NT;   /* outputs by nontemporal stores */
-    static constexpr bool reload = AR;   /* FP64: the back pass reads alpha from its output */
+    static constexpr int reload = AR;
     double ub[OCO_WAVE * (R + 1)];   /* U rows; alpha rows during the forward pass */
     double vb[OCO_WAVE * (R + 1)];   /* V rows */
     double* base[3][OCO_WAVE];       /* per lane: U, V, alpha; null = lane stores nothing */
 };
-typedef oco_stage_t<OCO_STAGE_ROWS> oco_stage;
+/* FP64 block sweep: a lane's alpha is a run of its own output, the 64 lanes'
+ * runs ncol*nz doubles apart, so alpha comes back through the stage in
+ * coalesced 128-byte runs per lane (read directly, each lane's read lands
+ * on a page of its own: 256 x 256 x 512 block 79 ms -> 238 ms). */
+#ifndef OCO_BLOCK_A1
+#define OCO_BLOCK_A1 OCO_A1_STAGED
+#endif
+typedef oco_stage_t<OCO_STAGE_ROWS, true, OCO_BLOCK_A1> oco_stage;
 typedef oco_stage_t<OCO_STAGE_ROWS_MIXED> oco_stage_mixed;
 /* The batched kernels keep ordinary stores (their columns' ragged nz and nw
  * leave many partial lines, which the L2 combines) and the FP64 one 8 rows.
  * A batched column's alpha is one contiguous (nz, nw) array, so a lane's
  * reads of its own alpha stay within a few pages of its neighbours': the
- * FP64 batch kernel reads alpha back instead of keeping 1/alpha in scratch
- * (measured on MI355X, 4096 uneven columns: 9.98 -> 9.39 ms).  In a block,
- * the 64 lanes' alpha arrays are ncol*nz doubles apart, each lane's read a
- * page of its own (256 x 256 x 512 block: 79 ms -> 238 ms), so the block
- * sweep keeps 1/alpha in scratch. */
-#ifndef OCO_BATCH_ALPHA_RELOAD
-#define OCO_BATCH_ALPHA_RELOAD 1
+ * FP64 batch kernel reads it directly (measured on MI355X, 4096 uneven
+ * columns: 9.98 ms with 1/alpha in scratch, 9.39 ms). */
+#ifndef OCO_BATCH_A1
+#define OCO_BATCH_A1 OCO_A1_DIRECT
 #endif
-typedef oco_stage_t<8, false, OCO_BATCH_ALPHA_RELOAD != 0> oco_stage_batch;
+typedef oco_stage_t<8, false, OCO_BATCH_A1> oco_stage_batch;
 typedef oco_stage_t<OCO_STAGE_ROWS_MIXED, false> oco_stage_batch_mixed;
 
 /* Output policy of the serving kernels: only the emergent flux V(1), V(nz)
@@ -171,7 +184,7 @@ struct oco_stage_emergent {
     static constexpr int rows = 8;
     static constexpr bool emergent = true;
     static constexpr bool nt = false;
-    static constexpr bool reload = false;
+    static constexpr int reload = OCO_A1_SCRATCH;
     double ub[1], vb[1];
     double* base[3][1];
 };
@@ -300,11 +313,8 @@ __device__ static inline oco_pt point_sys(const oco_sys& d, int z, const double*
  * 6e-8 on V.  `scr` is the lane's slot pointer (wave slot + lane). */
 constexpr long SCR_ROW_MIXED = 160;   /* doubles per row of a wave's mixed slot: 64 + 64 + 32 */
 
-/* AR (FP64 sweeps that write alpha): the scratch row is [gam, u], 16 bytes,
- * and the back pass reads alpha from its output instead, 1/alpha being
- * recomputed as the forward pass computed it (1.0 / alpha, correctly rounded,
- * so the same bits). */
-template <bool MIX, bool AR = false>
+/* AR (OCO_A1_*, FP64): with a reload, the scratch row is [gam, u], 16 bytes. */
+template <bool MIX, int AR = OCO_A1_SCRATCH>
 __device__ static inline void scr_put(double* scr, int j, double gam, double u, double a1)
 {
     if constexpr (MIX) {
@@ -317,7 +327,7 @@ __device__ static inline void scr_put(double* scr, int j, double gam, double u, 
         pk.y = (float)(D - (double)pk.x);
         *(float2*)(sp + OCO_WAVE) = pk;
         ((float*)(sp - lane + 2 * OCO_WAVE))[lane] = (float)a1;
-    } else if constexpr (AR) {
+    } else if constexpr (AR != OCO_A1_SCRATCH) {
         (void)a1;
         double* sp = scr + (long)j * 2 * OCO_WAVE;
         sp[0] = gam;
@@ -330,9 +340,9 @@ __device__ static inline void scr_put(double* scr, int j, double gam, double u, 
     }
 }
 
-/* a1: 1/alpha of row j (AR: alpha of row j, read from the lane's output A;
- * the caller takes a1_of) */
-template <bool MIX, bool AR = false>
+/* a1: 1/alpha of row j (OCO_A1_DIRECT: alpha of row j, read from the lane's
+ * output A, the caller takes a1_of; OCO_A1_STAGED: not set) */
+template <bool MIX, int AR = OCO_A1_SCRATCH>
 __device__ static inline void scr_get(const double* scr, int j, double& gam, double& u, double& a1,
                                       const double* A = nullptr)
 {
@@ -343,11 +353,17 @@ __device__ static inline void scr_get(const double* scr, int j, double& gam, dou
         const float2 pk = *(const float2*)(sp + OCO_WAVE);
         gam = -((double)pk.x + (double)pk.y);
         a1 = (double)((const float*)(sp - lane + 2 * OCO_WAVE))[lane];
-    } else if constexpr (AR) {
+    } else if constexpr (AR == OCO_A1_DIRECT) {
         const double* sp = scr + (long)j * 2 * OCO_WAVE;
         gam = sp[0];
         u = sp[OCO_WAVE];
         a1 = A[j];
+    } else if constexpr (AR == OCO_A1_STAGED) {
+        (void)A;
+        const double* sp = scr + (long)j * 2 * OCO_WAVE;
+        gam = sp[0];
+        u = sp[OCO_WAVE];
+        a1 = 0.0;
     } else {
         (void)A;
         const double* sp = scr + (long)j * 3 * OCO_WAVE;
@@ -357,18 +373,42 @@ __device__ static inline void scr_get(const double* scr, int j, double& gam, dou
     }
 }
 
-template <bool AR>
+template <int AR>
 __device__ static inline double a1_of(double v)
 {
-    if constexpr (AR) return 1.0 / v;   /* alpha -> 1/alpha, as oco_point's alpha1 */
+    if constexpr (AR == OCO_A1_DIRECT) return 1.0 / v;   /* alpha -> 1/alpha, as oco_point's alpha1 */
     else return v;
 }
 
-/* Whether a lane sweep with this mode and output policy reads alpha back
- * (it is written by the forward pass) rather than keeping 1/alpha in
- * scratch. */
+/* How a lane sweep with this mode and output policy gets 1/alpha in its back
+ * pass (OCO_A1_*; alpha is written by the forward pass). */
 template <bool MIX, class STAGE>
-constexpr bool alpha_reload() { return !MIX && !STAGE::emergent && STAGE::reload; }
+constexpr int alpha_reload() { return MIX || STAGE::emergent ? OCO_A1_SCRATCH : STAGE::reload; }
+
+/* OCO_A1_STAGED: alpha rows [b*R, b*R + R) of every lane into registers
+ * (thread t holds element t mod R of lanes t/R + k*64/R, the output flush's
+ * pattern: R consecutive threads read one lane's 8R-byte run), then into
+ * the V stage's slots; a lane without an alpha output gets 1. */
+template <int R>
+__device__ static inline void alpha_fetch(double* regs, double* const* base, int b, int len)
+{
+    constexpr int L = OCO_WAVE / R;
+    const int r = lane_id() & (R - 1), s0 = lane_id() / R, idx = b * R + r;
+#pragma unroll
+    for (int k = 0; k < R; k++) {
+        const double* p = base[s0 + k * L];
+        regs[k] = p != nullptr && idx < len ? p[idx] : 1.0;
+    }
+}
+
+template <int R>
+__device__ static inline void alpha_park(double* buf, const double* regs)
+{
+    constexpr int L = OCO_WAVE / R;
+    const int r = lane_id() & (R - 1), s0 = lane_id() / R;
+#pragma unroll
+    for (int k = 0; k < R; k++) buf[(s0 + k * L) * (R + 1) + r] = regs[k];
+}
 
 /* One (column, wavelength) system per lane, in two halves.  `w` is the
  * lane's wavelength-table row with stride WS; `scr` its scratch column
@@ -556,7 +596,33 @@ __device__ static inline void solve_backward(const oco_sys& d, bool valid, const
     const double c60 = 1.0 / 60.0;
     /* window w0..w6 = extended U at e = j .. j+6 (e = z + 3) */
     double gnext, a1q0, Un;              /* gam of the row above; 1/alpha of the V row pending */
-    constexpr bool AR = alpha_reload<MIX, STAGE>();
+    constexpr int AR = alpha_reload<MIX, STAGE>();
+    constexpr bool ST = AR == OCO_A1_STAGED && !EM;
+    /* OCO_A1_STAGED: the V stage's slot of row e holds alpha(e) until V(e)
+     * replaces it; apf holds the next block's alpha, in flight */
+    double apf[ST ? R : 1];
+    if constexpr (ST) {
+        const int bt = (nz - 1) / R;
+        alpha_fetch<R>(apf, S->base[2], bt, nz);
+        alpha_park<R>(S->vb, apf);
+        if (bt >= 1) alpha_fetch<R>(apf, S->base[2], bt - 1, nz);
+        wave_sync();
+    }
+    /* 1/alpha of V row e (a1p: the register pipeline's value) */
+    auto a1v = [&](int e, double a1p) -> double {
+        if constexpr (ST) return 1.0 / S->vb[lane * (R + 1) + (e & (R - 1))];
+        else return a1p;
+    };
+    /* V block b has been flushed: park block b - 1's alpha, fetch b - 2's */
+    auto vflushed = [&](int b) {
+        if constexpr (ST) {
+            if (b >= 1) {
+                alpha_park<R>(S->vb, apf);
+                wave_sync();
+                if (b >= 2) alpha_fetch<R>(apf, S->base[2], b - 2, nz);
+            }
+        }
+    };
     scr_get<MIX, AR>(scr, nz - 1, gnext, Un, a1q0, d.A);   /* U(nz) = u(nz) (:43) */
     a1q0 = a1_of<AR>(a1q0);
     double a1q1 = 0.0, a1q2 = 0.0, a1q3 = 0.0;
@@ -591,7 +657,7 @@ __device__ static inline void solve_backward(const oco_sys& d, bool valid, const
     double w0 = t[3], w1 = t[2], w2 = t[1], w3 = t[0], w4 = g1, w5 = g2, w6 = g3;
     {
         double df = c60 * (((45.0 * (w4 - w2)) - (9.0 * (w5 - w1))) + (w6 - w0));
-        const double vtop = ((-1.0 * a1q3) * df) * dz1;
+        const double vtop = ((-1.0 * a1v(nz - 1, a1q3)) * df) * dz1;
         if constexpr (EM) {
             if (valid) d.V[1] = vtop;
         } else {
@@ -600,6 +666,7 @@ __device__ static inline void solve_backward(const oco_sys& d, bool valid, const
                 wave_sync();
                 stage_flush<R, STAGE::nt>(S->vb, S->base[1], (nz - 1) / R, nz);
                 wave_sync();
+                vflushed((nz - 1) / R);
             }
         }
     }
@@ -633,12 +700,13 @@ __device__ static inline void solve_backward(const oco_sys& d, bool valid, const
                     if constexpr (!EM) {
                         stage_put<R>(S->ub, e, Uj);
                         double df = c60 * (((45.0 * (w4 - w2)) - (9.0 * (w5 - w1))) + (w6 - w0));
-                        stage_put<R>(S->vb, e, ((-1.0 * a1q3) * df) * dz1);
+                        stage_put<R>(S->vb, e, ((-1.0 * a1v(e, a1q3)) * df) * dz1);
                         if ((e & (R - 1)) == 0) {
                             wave_sync();
                             stage_flush<R, STAGE::nt>(S->ub, S->base[0], e / R, nz + 6);
                             stage_flush<R, STAGE::nt>(S->vb, S->base[1], e / R, nz);
                             wave_sync();
+                            vflushed(e / R);
                         }
                     }
                 }
@@ -661,7 +729,7 @@ __device__ static inline void solve_backward(const oco_sys& d, bool valid, const
         a1q3 = a1q2; a1q2 = a1q1; a1q1 = a1q0;
         if (!EM || k == 2) {
             double df = c60 * (((45.0 * (w4 - w2)) - (9.0 * (w5 - w1))) + (w6 - w0));
-            const double v = ((-1.0 * a1q3) * df) * dz1;
+            const double v = ((-1.0 * a1v(2 - k, a1q3)) * df) * dz1;
             if constexpr (EM) {
                 if (valid) d.V[0] = v;
             } else {
