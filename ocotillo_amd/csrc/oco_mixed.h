@@ -25,12 +25,14 @@
 #define OCO_LN2      0x1.62e42fefa39efp-1
 #define OCO_LN10     0x1.26bb1bbb55516p+1
 
-/* 2^t for any finite t: t = n + f, |f| <= 1/2, 2^f in FP32, 2^n by ldexp */
+/* 2^t for |t| < 2^31: t = n + f, |f| <= 1/2, 2^f in FP32, 2^n by ldexp
+ * (which gives 0 or inf beyond the double range).  Callers bound their
+ * arguments: every exponent of the opacity is bounded by the cell's
+ * temperature-only factors, except 1 - exp(-y), which clamps y. */
 __device__ static inline double mx_exp2(double t)
 {
-    double tc = __builtin_fmin(__builtin_fmax(t, -2200.0), 2200.0);
-    double n = __builtin_rint(tc);
-    float f = (float)(tc - n);
+    double n = __builtin_rint(t);
+    float f = (float)(t - n);
     double m = (double)__builtin_amdgcn_exp2f(f);
     return __builtin_amdgcn_ldexp(m, (int)n);
 }
@@ -66,11 +68,11 @@ __device__ static inline double mx_series_omexp(double y)
     return y * q;
 }
 
-/* 1 - exp(-y), y >= 0 */
+/* 1 - exp(-y), y >= 0 (1 - exp(-1000) is 1 in double: larger y are clamped) */
 __device__ static inline double mx_omexp(double y)
 {
     double s = mx_series_omexp(y);
-    double l = 1.0 - mx_exp(-y);
+    double l = 1.0 - mx_exp(-__builtin_fmin(y, 1000.0));
     return y < 0.5 ? s : l;
 }
 
