@@ -67,7 +67,8 @@ struct oco_batch_args {
 
 /* Serving: one admitted job's device descriptor. */
 struct oco_serve_job {
-    const double* cells;   /* cell records, cell-major (ncol*nz, OCO_CELL_STRIDE) */
+    const double* cells;   /* cell records: FP64 mode cell-major (ncol*nz, OCO_CELL_STRIDE); mixed mode
+                            * chunk-major per column (oco_launch_cells_chunked) */
     const double* wtab;    /* (nw, OCO_WT_STRIDE) */
     double* em;            /* emergent V: (2, ncol, nw) -- V(1), V(nz) per system */
     long ncol;
@@ -83,6 +84,7 @@ struct oco_serve_args {
     long scr_stride;              /* doubles per wavefront slot: max_nz * 3 * 64 */
     unsigned int* counter;
     unsigned long long* fail;     /* per slot: first zero pivot key (FP64 mode) */
+    int win_rows;                 /* mixed mode: boundary-window rows per system (oco_serve_win_rows(max_nz)) */
 };
 
 /* One-record Fortran unformatted sequential file in gfortran's layout
@@ -121,7 +123,15 @@ hipError_t oco_launch_athena(const oco_athena_args& a, hipStream_t stream);
 
 /* Launchers (enqueue on `stream`, no synchronisation). */
 hipError_t oco_launch_serve(const oco_serve_args& a, int mixed, int nblocks, hipStream_t stream);
-int oco_serve_grid(int mixed, int device);
+/* mixed-mode serving records: column c's cell z = l*m + j (m = rows per lane
+ * of the wavefront solve, l the lane, j the row in its chunk) has field k at
+ * c*(m*CR_NFIELD*64) + (j*CR_NFIELD + k)*64 + l, so the 64 lanes of a
+ * wavefront load one 512-byte line per field, each lane from its own chunk */
+hipError_t oco_launch_cells_chunked(const double* rho, const double* T, long ncells, int nz, double fully_ionized_T,
+                                    double* cells, hipStream_t stream);
+size_t oco_cells_chunked_doubles(long ncol, int nz);
+int oco_serve_win_rows(int max_nz);
+int oco_serve_grid(int mixed, int max_nz, int device);
 hipError_t oco_launch_fail_reset(unsigned long long* fail, int n, hipStream_t stream);
 /* mixed != 0 selects the mixed-precision kernels (non-grey only). */
 hipError_t oco_launch_batch(const oco_batch_args& a, int mixed, int variant, int nblocks, hipStream_t stream);

@@ -1015,90 +1015,90 @@ __device__ static inline void wave_points(const double* cc, int nz, const double
     }
 }
 
-/* 2.-4. one system, the whole wavefront, from its points in `lds` (written by
- * wave_points; the caller has synchronised the workgroup since) */
-template <int MM>
-__device__ static inline void solve_system_wave(const oco_sys& d, double* lds)
+/* A lane's chunk of rows in the wavefront-per-system solve (difference form):
+ * x[i] = D x[i+1] + W xL + Z for the chunk's rows, xL the previous chunk's last
+ * unknown, the deficit E = 1 - D - W carried positively. */
+struct rh_chunk {
+    double Wp, Zp, Ep;       /* the previous row's W, Z, E (the parameter row x = xL: 1, 0, 0) */
+    double pr, sD, sE;       /* the chunk's first row: pr G + (..) xL + sD, deficit sE */
+    double DL, WL, ZL, EL;   /* its last row */
+    double rlo;              /* 1 / mean alpha of the interface below the current row */
+};
+
+__device__ static inline void rh_chunk_init(rh_chunk& c)
+{
+    c.Wp = 1.0; c.Zp = 0.0; c.Ep = 0.0;
+    c.pr = 1.0; c.sD = 0.0; c.sE = 0.0;
+    c.DL = 0.0; c.WL = 0.0; c.ZL = 0.0; c.EL = 0.0;
+    c.rlo = 0.0;
+}
+
+/* Row z of the system (source/auxiliary.f90:147-197 in difference form) and
+ * its elimination step: al, om, S the cell's point, anext the alpha of cell
+ * z + 1 (read for z < nz - 1), last whether z is the chunk's last row.  The
+ * interface reciprocal is shared by this row's A and the previous row's C
+ * (the same sum, so the same bits as rh_interior_row); a chunk starting above
+ * row 0 sets c.rlo from its first cell and the cell below before its first
+ * row.  D, W, Z: the row's coefficients of x[z+1], xL and 1. */
+__device__ static inline void rh_chunk_row(rh_chunk& c, int z, int nz, bool last, double al, double om, double S,
+                                           double anext, double dz, double dz2, double& D, double& W, double& Z)
 {
 #pragma clang fp contract(fast)   /* mixed mode: error bounds, not the reference's bits */
-    const int nz = d.nz, lane = lane_id();
-    const double dz = d.dz, dz1 = d.dz1, dz2 = d.dz2;
-    double* LA = lds;              /* alpha */
-    double* LW = lds + nz;         /* albedo */
-    double* LS = lds + 2 * nz;     /* extended, e = z + 3: S, then U with ghosts */
-    const int m = wave_rows(nz);
-    const int z0 = lane * m < nz ? lane * m : nz;
-    const int z1 = z0 + m < nz ? z0 + m : nz;
+    const double al2 = al * al;
+    double Ac, Cc, H, R;
+    if (z == 0 || z == nz - 1) {
+        oco_pt pc;
+        pc.alpha = al; pc.albedo = om; pc.S = S; pc.alpha2 = al2; pc.alpha1 = 0.0;
+        rh_boundary_row(pc, dz, dz2, H, R);
+        Ac = z == 0 ? 0.0 : al;
+        Cc = z == 0 ? al : 0.0;
+        if (z == 0) c.rlo = mx_rcp(0.5 * (anext + al));
+    } else {
+        const double rhi = mx_rcp(0.5 * (anext + al));
+        Ac = al2 * c.rlo;
+        Cc = al2 * rhi;
+        H = (dz2 * (al2 * al)) * (1.0 - om);
+        R = S * H;
+        c.rlo = rhi;
+    }
+    const double r = mx_rcp((Ac * (c.Wp + c.Ep) + Cc) + H);   /* 1 - D[i-1] = W + E */
+    D = Cc * r;
+    W = (Ac * c.Wp) * r;
+    Z = (R + Ac * c.Zp) * r;
+    const double E = (Ac * c.Ep + H) * r;
+    if (!last) {
+        c.sD += c.pr * Z;
+        c.sE += c.pr * E;
+        c.pr *= D;
+    } else {
+        c.DL = D; c.WL = W; c.ZL = Z; c.EL = E;
+    }
+    c.Wp = W; c.Zp = Z; c.Ep = E;
+}
 
-    /* 2. rows (source/auxiliary.f90:147-197) in difference form and the
-     * chunk's forward elimination */
-    double Dr[MM], Wr[MM], Zr[MM];
-    double Wp = 1.0, Zp = 0.0, Ep = 0.0;            /* the parameter row x = xL */
-    double pr = 1.0, sP = 0.0, sD = 0.0, sE = 0.0;  /* first row: pr G + sP xL + sD, deficit sE */
-    double DL = 0.0, WL = 0.0, ZL = 0.0, EL = 0.0;  /* last row */
-    /* 1 / mean alpha of the interface below the current row, shared by this
-     * row's A and the previous row's C (the same sum, so the same bits as
-     * rh_interior_row) */
-    double rlo = z0 > 0 && z0 < z1 ? mx_rcp(0.5 * (LA[z0] + LA[z0 - 1])) : 0.0;
-#pragma unroll
-    for (int j = 0; j < MM; j++) {
-        const int z = z0 + j;
-        Dr[j] = 0.0; Wr[j] = 0.0; Zr[j] = 0.0;
-        if (z < z1) {
-            const double al = LA[z], om = LW[z], S = LS[z + 3];
-            const double al2 = al * al;
-            double Ac, Cc, H, R;
-            oco_pt pc;
-            pc.alpha = al; pc.albedo = om; pc.S = S; pc.alpha2 = al2; pc.alpha1 = 0.0;
-            if (z == 0 || z == nz - 1) {
-                rh_boundary_row(pc, dz, dz2, H, R);
-                Ac = z == 0 ? 0.0 : al;
-                Cc = z == 0 ? al : 0.0;
-                if (z == 0) rlo = mx_rcp(0.5 * (LA[1] + al));
-            } else {
-                const double rhi = mx_rcp(0.5 * (LA[z + 1] + al));
-                Ac = al2 * rlo;
-                Cc = al2 * rhi;
-                H = (dz2 * (al2 * al)) * (1.0 - om);
-                R = S * H;
-                rlo = rhi;
-            }
-            const double r = mx_rcp((Ac * (Wp + Ep) + Cc) + H);   /* 1 - D[i-1] = W + E */
-            const double D = Cc * r;
-            const double W = (Ac * Wp) * r;
-            const double Z = (R + Ac * Zp) * r;
-            const double E = (Ac * Ep + H) * r;
-            Dr[j] = D; Wr[j] = W; Zr[j] = Z;
-            if (z < z1 - 1) {
-                sP += pr * W;
-                sD += pr * Z;
-                sE += pr * E;
-                pr *= D;
-            } else {
-                DL = D; WL = W; ZL = Z; EL = E;
-            }
-            Wp = W; Zp = Z; Ep = E;
-        }
-    }
-    /* 3. reduced system in the chunks' last unknowns:
-     * -WL G[k-1] + (WL + DL Gn + EL + DL En) G[k] - DL Gn G[k+1] = ZL + DL Dn
-     * (Gn, Dn, En: the next chunk's first row) */
-    /* Neighbours are read by rotation, without masking the lanes whose
-     * neighbour does not exist: those lanes' couplings are exactly zero (the
-     * first chunk's W, the last row's D, and after each reduction step A = 0
-     * for lanes < s, C = 0 for lanes > 63 - s), and every row's A + C + H is
-     * positive and finite, so the wrapped-around values are multiplied by
-     * zero.  Empty chunks are identity rows. */
+/* The reduced system in the chunks' last unknowns, all lanes of the wavefront:
+ * -WL G[k-1] + (WL + DL Gn + EL + DL En) G[k] - DL Gn G[k+1] = ZL + DL Dn
+ * (Gn, Dn, En: the next chunk's first row), by parallel cyclic reduction in
+ * difference form (H' = H + k1 H[-s] + k2 H[+s]: no subtraction).  Neighbours
+ * are read by rotation, without masking the lanes whose neighbour does not
+ * exist: those lanes' couplings are exactly zero (the first chunk's W, the
+ * last row's D, and after each reduction step A = 0 for lanes < s, C = 0 for
+ * lanes > 63 - s), and every row's A + C + H is positive and finite, so the
+ * wrapped-around values are multiplied by zero.  Empty chunks are identity
+ * rows.  G: this chunk's last unknown, GL the previous chunk's. */
+__device__ static inline void rh_reduced(const rh_chunk& c, bool rows, double& G, double& GL)
+{
+#pragma clang fp contract(fast)
+    const int lane = lane_id();
     const int dn1 = ((lane + 1) & (OCO_WAVE - 1)) * 4, up1 = ((lane - 1) & (OCO_WAVE - 1)) * 4;
-    const double Gn = lane_get(pr, dn1), Dn = lane_get(sD, dn1), En = lane_get(sE, dn1);
+    const double Gn = lane_get(c.pr, dn1), Dn = lane_get(c.sD, dn1), En = lane_get(c.sE, dn1);
     double Ar = 0.0, Cr = 0.0, Hr = 1.0, Rr = 0.0;
-    if (z1 > z0) {
-        Ar = WL;
-        Cr = DL * Gn;
-        Hr = EL + DL * En;
-        Rr = ZL + DL * Dn;
+    if (rows) {
+        Ar = c.WL;
+        Cr = c.DL * Gn;
+        Hr = c.EL + c.DL * En;
+        Rr = c.ZL + c.DL * Dn;
     }
-    (void)sP;
 #pragma unroll
     for (int sft = 1; sft < OCO_WAVE; sft <<= 1) {
         const int up = ((lane - sft) & (OCO_WAVE - 1)) * 4, dn = ((lane + sft) & (OCO_WAVE - 1)) * 4;
@@ -1111,15 +1111,76 @@ __device__ static inline void solve_system_wave(const oco_sys& d, double* lds)
         Hr = (Hr + k1 * Hm) + k2 * Hp;
         Rr = (Rr + k1 * Rm) + k2 * Rp;
     }
-    const double G = Rr * mx_rcp((Ar + Cr) + Hr);
-    const double GL = lane_get(G, up1);
+    G = Rr * mx_rcp((Ar + Cr) + Hr);
+    GL = lane_get(G, up1);
+}
+
+/* x[z] from x[z+1] and the previous chunk's last unknown */
+__device__ static inline double rh_back(double D, double W, double Z, double x, double GL)
+{
+#pragma clang fp contract(fast)
+    return (D * x + W * GL) + Z;
+}
+
+/* 6th-order derivative at e from U(e-3 .. e+3) (source/auxiliary.f90:64-78) */
+__device__ static inline double der7(double um3, double um2, double um1, double up1, double up2, double up3)
+{
+#pragma clang fp contract(fast)
+    const double c60 = 1.0 / 60.0;
+    return c60 * (((45.0 * (up1 - um1)) - (9.0 * (up2 - um2))) + (up3 - um3));
+}
+
+/* calc_flux (source/auxiliary.f90:96-109) of a cell from its alpha and der */
+__device__ static inline double flux_v(double al, double df, double dz1)
+{
+    return ((-1.0 * mx_rcp(al)) * df) * dz1;
+}
+
+/* ghost f(n -+ i) = 2 f(n) - f(n +- i) (source/auxiliary.f90:49-62) */
+__device__ static inline double ghost(double fn, double fi)
+{
+#pragma clang fp contract(fast)
+    return 2.0 * fn - fi;
+}
+
+/* 2.-4. one system, the whole wavefront, from its points in `lds` (written by
+ * wave_points; the caller has synchronised the workgroup since) */
+template <int MM>
+__device__ static inline void solve_system_wave(const oco_sys& d, double* lds)
+{
+    const int nz = d.nz, lane = lane_id();
+    const double dz = d.dz, dz1 = d.dz1, dz2 = d.dz2;
+    double* LA = lds;              /* alpha */
+    double* LW = lds + nz;         /* albedo */
+    double* LS = lds + 2 * nz;     /* extended, e = z + 3: S, then U with ghosts */
+    const int m = wave_rows(nz);
+    const int z0 = lane * m < nz ? lane * m : nz;
+    const int z1 = z0 + m < nz ? z0 + m : nz;
+
+    /* 2. rows (source/auxiliary.f90:147-197) in difference form and the
+     * chunk's forward elimination */
+    double Dr[MM], Wr[MM], Zr[MM];
+    rh_chunk c;
+    rh_chunk_init(c);
+    if (z0 > 0 && z0 < z1) c.rlo = mx_rcp(0.5 * (LA[z0] + LA[z0 - 1]));
+#pragma unroll
+    for (int j = 0; j < MM; j++) {
+        const int z = z0 + j;
+        Dr[j] = 0.0; Wr[j] = 0.0; Zr[j] = 0.0;
+        if (z < z1)
+            rh_chunk_row(c, z, nz, z == z1 - 1, LA[z], LW[z], LS[z + 3], z < nz - 1 ? LA[z + 1] : 0.0, dz, dz2,
+                         Dr[j], Wr[j], Zr[j]);
+    }
+    /* 3. the reduced system */
+    double G, GL;
+    rh_reduced(c, z1 > z0, G, GL);
     /* 4. back-substitution inside the chunk */
     double x = G;
 #pragma unroll
     for (int j = MM - 1; j >= 0; j--) {
         const int z = z0 + j;
         if (z < z1) {
-            if (z < z1 - 1) x = (Dr[j] * x + Wr[j] * GL) + Zr[j];
+            if (z < z1 - 1) x = rh_back(Dr[j], Wr[j], Zr[j], x, GL);
             LS[z + 3] = x;
         }
     }
@@ -1127,18 +1188,16 @@ __device__ static inline void solve_system_wave(const oco_sys& d, double* lds)
     /* ghosts (source/auxiliary.f90:49-62) */
     if (lane < 3) {
         const int i = lane + 1;   /* f(n2+i) = 2 f(n2) - f(n2-i) */
-        LS[nz + 2 + i] = 2.0 * LS[nz + 2] - LS[nz + 2 - i];
+        LS[nz + 2 + i] = ghost(LS[nz + 2], LS[nz + 2 - i]);
     } else if (lane < 6) {
         const int i = lane - 2;   /* f(n1-i) = 2 f(n1) - f(n1+i) */
-        LS[3 - i] = 2.0 * LS[3] - LS[3 + i];
+        LS[3 - i] = ghost(LS[3], LS[3 + i]);
     }
     wave_sync();
     /* der and calc_flux (:64-78, :96-109) */
-    const double c60 = 1.0 / 60.0;
     for (int z = lane; z < nz; z += OCO_WAVE) {
         const double* u = LS + z + 3;
-        double df = c60 * (((45.0 * (u[1] - u[-1])) - (9.0 * (u[2] - u[-2]))) + (u[3] - u[-3]));
-        out_store(d.V + z, ((-1.0 * mx_rcp(LA[z])) * df) * dz1);
+        out_store(d.V + z, flux_v(LA[z], der7(u[-3], u[-2], u[-1], u[1], u[2], u[3]), dz1));
     }
     for (int e = lane; e < nz + 6; e += OCO_WAVE) out_store(d.U + e, LS[e]);
 }
@@ -1438,20 +1497,240 @@ __global__ __launch_bounds__(OCO_WAVE) void serve_tiles(const oco_serve_args a)
     }
 }
 
+/* ---- serving, mixed mode: one wavefront per system, emergent flux only ----
+ * The same partitioned difference-form solve as solve_system_wave (its rows,
+ * reduced system, back-substitution and derivative are the shared functions
+ * above, so a job's V(1), V(nz) are those of the block wavefront sweep, bit
+ * for bit), restructured for outputs at the two boundaries only:
+ *   - each lane evaluates the points of its own chunk of rows, loading its
+ *     records from the chunk-major layout (one 512-byte line per field
+ *     across the wavefront) and evaluating each for K wavelengths, and
+ *     eliminates each row as soon as the next cell's alpha is known; a
+ *     system's state is a few registers whatever nz is (no LDS copy of the
+ *     column, no per-row registers), so any nz up to the server's max_nz is
+ *     taken;
+ *   - a chunk's first row needs the alpha of the cell below it (the previous
+ *     lane's last cell): each lane evaluates that cell's alpha itself; its
+ *     last row needs the next lane's first alpha, read across lanes;
+ *   - only the chunks holding rows 0..3 and nz-4..nz-1 keep their rows'
+ *     D, W, Z and alpha (LDS, the boundary window) and back-substitute;
+ *     two lanes then form V(1) and V(nz).
+ * A tile is (job, column, 64 wavelengths), taken by a workgroup of
+ * OCO_SERVE_WG wavefronts, each solving every OCO_SERVE_WG-th group of K of
+ * its wavelengths: the wavefronts stream the same column's records at the
+ * same time, so they come from L2 once per workgroup rather than from HBM
+ * once per group (one wavefront per tile: capacity 7.0e10 points/s on the
+ * serving workload). */
+#ifndef OCO_SERVE_WG
+#define OCO_SERVE_WG 4
+#endif
+#ifndef OCO_SERVE_K
+#define OCO_SERVE_K 2
+#endif
+/* Rows per lane of the serving solve: the wavefront sweep's, at least
+ * OCO_SERVE_MIN_ROWS (short columns then leave lanes empty: longer sequential
+ * chunks and a smaller reduced system) */
+#ifndef OCO_SERVE_MIN_ROWS
+#define OCO_SERVE_MIN_ROWS 5
+#endif
+__host__ __device__ static inline int serve_rows(int nz)
+{
+    const int m = wave_rows(nz);
+    return m < OCO_SERVE_MIN_ROWS ? OCO_SERVE_MIN_ROWS : m;
+}
+
+int oco_serve_win_rows(int max_nz) { return 2 * (serve_rows(max_nz) + 3); }
+
+size_t oco_cells_chunked_doubles(long ncol, int nz) { return (size_t)ncol * serve_rows(nz) * CR_NFIELD * OCO_WAVE; }
+
+__global__ __launch_bounds__(256) void cells_chunked_fp64(const double* __restrict__ rho, const double* __restrict__ T,
+                                                          long ncells, int nz, double fully_ionized_T,
+                                                          double* __restrict__ cells)
+{
+    __shared__ oco_mtab smt;
+    load_mtab(&smt);
+    const int m = serve_rows(nz);
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < ncells; i += (long)gridDim.x * blockDim.x) {
+        double rec[CR_NFIELD];
+        oco_cell_record(rho[i], T[i], fully_ionized_T, &smt, rec);
+        const long col = i / nz;
+        const int z = (int)(i - col * nz), l = z / m, j = z - l * m;
+        double* out = cells + col * (long)m * CR_NFIELD * OCO_WAVE + (long)j * CR_NFIELD * OCO_WAVE + l;
+#pragma unroll
+        for (int k = 0; k < CR_NFIELD; k++) out[k * OCO_WAVE] = rec[k];
+    }
+}
+
+hipError_t oco_launch_cells_chunked(const double* rho, const double* T, long ncells, int nz, double fully_ionized_T,
+                                    double* cells, hipStream_t stream)
+{
+    (void)hipGetLastError();
+    if (ncells <= 0) return hipSuccess;
+    long blocks = (ncells + 255) / 256;
+    if (blocks > 65536) blocks = 65536;
+    hipLaunchKernelGGL(cells_chunked_fp64, dim3((unsigned)blocks), dim3(256), 0, stream, rho, T, ncells, nz,
+                       fully_ionized_T, cells);
+    return hipGetLastError();
+}
+
+template <int K>
+__global__ __launch_bounds__(OCO_WAVE * OCO_SERVE_WG) void serve_wave(const oco_serve_args a)
+{
+    extern __shared__ double win[];   /* per wavefront: K systems x win_rows rows x (D or x, W, Z, alpha) */
+    __shared__ int tile_s;
+    /* the wavefront's index, wave-uniform (its wavelengths' table rows are then scalar loads) */
+    const int lane = lane_id(), wid = __builtin_amdgcn_readfirstlane((int)threadIdx.x / OCO_WAVE);
+    const int WR = a.win_rows;
+    double* lds = win + (long)wid * K * WR * 4;
+    oco_cptr_i tiles = (oco_cptr_i)a.tiles;
+    for (;;) {
+        __syncthreads();   /* every wavefront has read the last tile_s */
+        if (threadIdx.x == 0) tile_s = (int)atomicAdd(a.counter, 1u);
+        __syncthreads();
+        const int t = __builtin_amdgcn_readfirstlane(tile_s);
+        if (t >= a.ntiles) break;
+        const int slot = tiles[3 * t];
+        const long col = tiles[3 * t + 1];
+        const int wt0 = tiles[3 * t + 2];
+        const oco_serve_job jb = a.jobs[slot];
+        const int nz = jb.nz, m = serve_rows(nz);
+        const double dz = jb.dz, dz1 = jb.dz1, dz2 = jb.dz2;
+        const int z0 = lane * m < nz ? lane * m : nz;
+        const int z1 = z0 + m < nz ? z0 + m : nz;
+        const double* cc = jb.cells + col * (long)m * CR_NFIELD * OCO_WAVE;
+        /* boundary window: rows [0, nB) and [t0, nz), whole chunks */
+        const int nB = nz < (3 / m + 1) * m ? nz : (3 / m + 1) * m;
+        const int t0 = ((nz - 4) / m) * m > nB ? ((nz - 4) / m) * m : nB;
+        const bool win = z0 < z1 && (z0 < nB || z1 > t0);
+        auto wslot = [&](int k, int z) -> double* { return lds + ((long)k * WR + (z < nB ? z : nB + (z - t0))) * 4; };
+        const int wend = wt0 + OCO_WAVE < jb.nw ? wt0 + OCO_WAVE : jb.nw;
+        for (int iw0 = wt0 + wid * K; iw0 < wend; iw0 += K * OCO_SERVE_WG) {
+            const int nk = wend - iw0 < K ? wend - iw0 : K;
+            rh_chunk st[K];
+            double ap[K], op[K], Sp[K], a0[K], anb[K];
+#pragma unroll
+            for (int k = 0; k < K; k++) {
+                rh_chunk_init(st[k]);
+                ap[k] = op[k] = Sp[k] = a0[k] = anb[k] = 0.0;
+            }
+            double rec[CR_NFIELD];
+            if (z0 > 0 && z0 < z1) {   /* the cell below the chunk: the previous lane's last */
+#pragma unroll
+                for (int f = 0; f < CR_NFIELD; f++) rec[f] = cc[(long)((m - 1) * CR_NFIELD + f) * OCO_WAVE + lane - 1];
+#pragma unroll
+                for (int k = 0; k < K; k++) {
+                    if (k < nk) {
+                        double w[OCO_WT_STRIDE];
+                        oco_cptr wp = (oco_cptr)(jb.wtab + (long)(iw0 + k) * OCO_WT_STRIDE);
+#pragma unroll
+                        for (int f = 0; f < OCO_WT_STRIDE; f++) w[f] = wp[f];
+                        anb[k] = oco_point_mixed<1>(rec, w).alpha;
+                    }
+                }
+            }
+            for (int j = 0; j < m; j++) {
+                const int z = z0 + j;
+                if (z < z1) {
+#pragma unroll
+                    for (int f = 0; f < CR_NFIELD; f++) rec[f] = cc[(long)(j * CR_NFIELD + f) * OCO_WAVE + lane];
+#pragma unroll
+                    for (int k = 0; k < K; k++) {
+                        if (k < nk) {
+                            double w[OCO_WT_STRIDE];
+                            oco_cptr wp = (oco_cptr)(jb.wtab + (long)(iw0 + k) * OCO_WT_STRIDE);
+#pragma unroll
+                            for (int f = 0; f < OCO_WT_STRIDE; f++) w[f] = wp[f];
+                            const oco_pt p = oco_point_mixed<1>(rec, w);
+                            if (j == 0) {
+                                a0[k] = p.alpha;
+                                if (z0 > 0) st[k].rlo = mx_rcp(0.5 * (p.alpha + anb[k]));
+                            } else {   /* row z - 1, now that alpha(z) is known */
+                                double D, W, Z;
+                                rh_chunk_row(st[k], z - 1, nz, false, ap[k], op[k], Sp[k], p.alpha, dz, dz2, D, W, Z);
+                                if (win) {
+                                    double* q = wslot(k, z - 1);
+                                    q[0] = D; q[1] = W; q[2] = Z; q[3] = ap[k];
+                                }
+                            }
+                            ap[k] = p.alpha; op[k] = p.albedo; Sp[k] = p.S;
+                        }
+                    }
+                }
+            }
+            const int dn1 = ((lane + 1) & (OCO_WAVE - 1)) * 4;
+#pragma unroll
+            for (int k = 0; k < K; k++) {
+                if (k < nk) {
+                    const double an = lane_get(a0[k], dn1);   /* alpha of cell z1, the next chunk's first */
+                    if (z0 < z1) {   /* the chunk's last row */
+                        double D, W, Z;
+                        rh_chunk_row(st[k], z1 - 1, nz, true, ap[k], op[k], Sp[k], z1 < nz ? an : 0.0, dz, dz2, D, W, Z);
+                        if (win) {
+                            double* q = wslot(k, z1 - 1);
+                            q[0] = D; q[1] = W; q[2] = Z; q[3] = ap[k];
+                        }
+                    }
+                    double G, GL;
+                    rh_reduced(st[k], z0 < z1, G, GL);
+                    if (win) {   /* back-substitution in the window's chunks; x replaces D */
+                        double x = G;
+                        wslot(k, z1 - 1)[0] = x;
+                        for (int z = z1 - 2; z >= z0; z--) {
+                            double* q = wslot(k, z);
+                            x = rh_back(q[0], q[1], q[2], x, GL);
+                            q[0] = x;
+                        }
+                    }
+                }
+            }
+            wave_sync();
+            if (lane < 2) {
+#pragma unroll
+                for (int k = 0; k < K; k++) {
+                    if (k < nk) {
+                        double v;
+                        if (lane == 0) {   /* V(1): U(3 + i) = x[i], ghosts below */
+                            const double x0 = wslot(k, 0)[0], x1 = wslot(k, 1)[0], x2 = wslot(k, 2)[0],
+                                         x3 = wslot(k, 3)[0];
+                            v = flux_v(wslot(k, 0)[3],
+                                       der7(ghost(x0, x3), ghost(x0, x2), ghost(x0, x1), x1, x2, x3), dz1);
+                        } else {           /* V(nz): ghosts above */
+                            const double xn1 = wslot(k, nz - 1)[0], xn2 = wslot(k, nz - 2)[0],
+                                         xn3 = wslot(k, nz - 3)[0], xn4 = wslot(k, nz - 4)[0];
+                            v = flux_v(wslot(k, nz - 1)[3],
+                                       der7(xn4, xn3, xn2, ghost(xn1, xn2), ghost(xn1, xn3), ghost(xn1, xn4)), dz1);
+                        }
+                        jb.em[((long)(iw0 + k) * jb.ncol + col) * 2 + lane] = v;
+                    }
+                }
+            }
+            wave_sync();   /* the window is read before the next group writes it */
+        }
+    }
+}
+
+static size_t serve_wave_lds(int win_rows) { return (size_t)OCO_SERVE_WG * OCO_SERVE_K * win_rows * 4 * sizeof(double); }
+
 hipError_t oco_launch_serve(const oco_serve_args& a, int mixed, int nblocks, hipStream_t stream)
 {
     (void)hipGetLastError();   /* a pending not-ready status from an event query is not this launch's */
     hipLaunchKernelGGL(batch_reset, dim3(1), dim3(64), 0, stream, a.counter, (unsigned long long*)nullptr);
-    if (mixed) hipLaunchKernelGGL((serve_tiles<2, true>), dim3(nblocks), dim3(OCO_WAVE), 0, stream, a);
-    else hipLaunchKernelGGL((serve_tiles<2, false>), dim3(nblocks), dim3(OCO_WAVE), 0, stream, a);
+    if (mixed)
+        hipLaunchKernelGGL((serve_wave<OCO_SERVE_K>), dim3(nblocks), dim3(OCO_WAVE * OCO_SERVE_WG),
+                           serve_wave_lds(a.win_rows), stream, a);
+    else
+        hipLaunchKernelGGL((serve_tiles<2, false>), dim3(nblocks), dim3(OCO_WAVE), 0, stream, a);
     return hipGetLastError();
 }
 
-int oco_serve_grid(int mixed, int device)
+int oco_serve_grid(int mixed, int max_nz, int device)
 {
     int per_cu = 0, cus = 0;
-    const void* k = mixed ? (const void*)serve_tiles<2, true> : (const void*)serve_tiles<2, false>;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, OCO_WAVE, 0) != hipSuccess || per_cu < 1) per_cu = 4;
+    const void* k = mixed ? (const void*)serve_wave<OCO_SERVE_K> : (const void*)serve_tiles<2, false>;
+    const size_t sh = mixed ? serve_wave_lds(oco_serve_win_rows(max_nz)) : 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, mixed ? OCO_WAVE * OCO_SERVE_WG : OCO_WAVE, sh) !=
+            hipSuccess || per_cu < 1)
+        per_cu = mixed ? 1 : 4;
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess || cus < 1) cus = 256;
     return per_cu * cus;
 }

@@ -196,7 +196,8 @@ bool oco_server::admit_one(Job* j)
     const oco_job& d = j->desc;
     const long ncells = j->ncol * d.nz;
     const size_t b_in = (size_t)ncells * sizeof(double);
-    const size_t b_cells = (size_t)ncells * OCO_CELL_STRIDE * sizeof(double);
+    const size_t b_cells = (mixed ? oco_cells_chunked_doubles(j->ncol, d.nz) : (size_t)ncells * OCO_CELL_STRIDE) *
+                           sizeof(double);
     const size_t b_em = (size_t)j->nsys * 2 * sizeof(double);
     j->bytes = Arena::round(b_in) * 2 + Arena::round(b_cells) + Arena::round(b_em);
     const double* wt = table_for(d.waves, d.nw);
@@ -282,6 +283,7 @@ bool oco_server::launch_iteration()
     a.scr_stride = (long)max_nz * 3 * 64;
     a.counter = counter;
     a.fail = fail;
+    a.win_rows = oco_serve_win_rows(max_nz);
     /* the tiles' records were built on the copy stream */
     hipError_t e = hipSuccess;
     for (auto& pr : it.tiles)
@@ -394,7 +396,9 @@ void oco_server::copy_loop()
             double* d_cells = (double*)(p + 2 * Arena::round(b_in));
             e = hipMemcpyAsync(d_rho, d.rho3d, b_in, hipMemcpyHostToDevice, xs);
             if (e == hipSuccess) e = hipMemcpyAsync(d_T, d.T3d, b_in, hipMemcpyHostToDevice, xs);
-            if (e == hipSuccess) e = oco_launch_cells(d_rho, d_T, ncells, fully_ionized_T, d_cells, 0, xs);
+            if (e == hipSuccess)
+                e = mixed ? oco_launch_cells_chunked(d_rho, d_T, ncells, d.nz, fully_ionized_T, d_cells, xs)
+                          : oco_launch_cells(d_rho, d_T, ncells, fully_ionized_T, d_cells, 0, xs);
             if (e == hipSuccess) e = oco_launch_fail_reset(fail + slot, 1, xs);
             if (e == hipSuccess) e = hipEventRecord(j->ready, xs);
         } else {
@@ -463,10 +467,12 @@ int oco_server_create(oco_ctx* ctx, size_t hbm_budget, int max_nz, int mode, oco
     chk(hipSetDevice(s->device));
     chk(hipStreamCreateWithFlags(&s->cs, hipStreamNonBlocking));
     chk(hipStreamCreateWithFlags(&s->xs, hipStreamNonBlocking));
-    s->nblocks = oco_serve_grid(s->mixed, s->device);
+    s->nblocks = oco_serve_grid(s->mixed, max_nz, s->device);
     s->tile_cap = (size_t)(ITER_POINTS / (64.0 * 4)) + 4 * MAX_SLOTS;
     if (st == OCO_OK) chk(hipMalloc(&s->mem, hbm_budget));
-    if (st == OCO_OK) chk(hipMalloc((void**)&s->scratch, (size_t)s->nblocks * max_nz * 3 * 64 * sizeof(double)));
+    /* elimination scratch: FP64 mode only (the mixed kernel keeps its state in registers) */
+    if (st == OCO_OK && !s->mixed)
+        chk(hipMalloc((void**)&s->scratch, (size_t)s->nblocks * max_nz * 3 * 64 * sizeof(double)));
     if (st == OCO_OK) chk(hipMalloc((void**)&s->counter, sizeof(unsigned)));
     if (st == OCO_OK) chk(hipMalloc((void**)&s->fail, MAX_SLOTS * sizeof(unsigned long long)));
     if (st == OCO_OK) chk(hipMalloc((void**)&s->d_jobs, MAX_SLOTS * sizeof(oco_serve_job)));

@@ -5,14 +5,16 @@ Bars:
   * a job's result is bit-identical alone and in a mix of concurrently
     submitted jobs (every system is solved by one lane, whatever else shares
     the launch);
-  * the emergent flux V(1), V(nz) equals the block path's V[..., 0] and
-    V[..., -1] bit for bit, in both precision modes (same per-system code);
-    in FP64 mode that is the oracle's result bit for bit;
+  * FP64: the emergent flux V(1), V(nz) equals the block path's V[..., 0]
+    and V[..., -1] and the oracle's, bit for bit;
+  * mixed: the emergent flux meets the mixed mode's stated bounds against
+    the difference-form oracle, up to nz = 2048, and agrees with the block
+    wavefront sweep (whose rows, reduced system, back-substitution and
+    derivative the serving kernel shares) within those bounds;
   * admission keeps the job arena within its budget; a job larger than the
     budget is refused at submit; the reference's zero-pivot stop is reported
     per job with the reference's (ix, iy, iw).
 """
-import os
 
 import numpy as np
 import pytest
@@ -30,22 +32,6 @@ def ctx():
     with oco.Context(0) as c:
         c.set_physics(2e4, False, 0.4)
         yield c
-
-
-def lane_context():
-    """Block solves with the lane-per-system mixed sweep, the serving kernel's."""
-    import ocotillo_amd as oco
-    old = os.environ.get("OCO_MIXED_SWEEP")
-    os.environ["OCO_MIXED_SWEEP"] = "lane"
-    try:
-        c = oco.Context(0)
-    finally:
-        if old is None:
-            del os.environ["OCO_MIXED_SWEEP"]
-        else:
-            os.environ["OCO_MIXED_SWEEP"] = old
-    c.set_physics(2e4, False, 0.4)
-    return c
 
 
 def make_job(seed, nx, ny, nz, nw):
@@ -81,35 +67,94 @@ def test_alone_equals_in_the_mix(ctx, mode_name):
     print("  iterations %d, p50 %.1f ms, p99 %.1f ms" % (st.iterations, st.p50_ms, st.p99_ms), flush=True)
 
 
-@pytest.mark.parametrize("mode_name", ["mixed", "fp64"])
-def test_emergent_equals_block_path(mode_name):
+def test_emergent_equals_block_path_fp64():
+    """FP64 serving: V(1), V(nz) equal the block sweep's and the oracle's, bit
+    for bit."""
     import ocotillo_amd as oco
-    mode = oco.MODE_MIXED if mode_name == "mixed" else oco.MODE_FP64
+    from oracle.oracle import Oracle
     jobs = [make_job(*j) for j in JOBS[:3]]
-    with lane_context() as c:
-        with oco.Server(c, 1 * GiB, max_nz=1024, mode=mode) as srv:
-            tickets = [srv.submit(*j) for j in jobs]
-            outs = [srv.wait(t) for t in tickets]
+    with oco.Context(0) as c:
+        c.set_physics(2e4, False, 0.4)
+        with oco.Server(c, 1 * GiB, max_nz=1024, mode=oco.MODE_FP64) as srv:
+            outs = [srv.wait(srv.submit(*j)) for j in jobs]
         for (rho, T, dz, waves), (em, r) in zip(jobs, outs):
             print("case", rho.shape, waves.size, flush=True)
             assert r.status == 0
             c.set_wavelengths(waves)
-            U, V, A, d = c.solve_block(rho, T, dz, mode=mode)
-            assert np.array_equal(em[..., 0], V[..., 0])
-            assert np.array_equal(em[..., 1], V[..., -1])
-            if mode == oco.MODE_FP64:
-                from oracle.oracle import Oracle
-                _, Vo, _, _ = Oracle(waves, portable=True).solve_block(rho, T, dz)
-                assert np.array_equal(em[..., 0], Vo[..., 0]) and np.array_equal(em[..., 1], Vo[..., -1])
+            U, V, A, d = c.solve_block(rho, T, dz, mode=oco.MODE_FP64)
+            assert np.array_equal(em[..., 0], V[..., 0]) and np.array_equal(em[..., 1], V[..., -1])
+            _, Vo, _, _ = Oracle(waves, portable=True).solve_block(rho, T, dz)
+            assert np.array_equal(em[..., 0], Vo[..., 0]) and np.array_equal(em[..., 1], Vo[..., -1])
+
+
+def check_emergent(em, rho, T, dz, waves, label):
+    """Mixed emergent V against the difference-form oracle: within 1e-6 where
+    FP64 resolves V, else within 1e-6 |V| + 64 q."""
+    from test_gpu_mixed import FLUX_TOL, RES_FACTOR, resolution, resolved, rh_oracle
+    Ur, Vr, Ar = rh_oracle(rho, T, dz, waves)
+    q = resolution(Ur, Ar, dz)
+    res = resolved(Ur, Vr, Ar, dz)
+    for i, k in ((0, 0), (1, -1)):
+        dv = np.abs(em[..., i] - Vr[..., k])
+        assert (dv <= FLUX_TOL * np.abs(Vr[..., k]) + RES_FACTOR * q[i]).all(), label
+        if res[i].any():
+            assert (dv[res[i]] <= FLUX_TOL * np.abs(Vr[..., k][res[i]])).all(), label
+    return q
+
+
+def test_emergent_equals_block_path_mixed():
+    """Mixed serving against the block path: both meet the stated bounds
+    against the difference-form oracle, and they agree to within those
+    bounds.  They are not compared bit for bit: the serving kernel partitions
+    a column into chunks of at least 5 rows (the block wavefront sweep into
+    its own row counts, and the block path takes the lane sweep below 129
+    cells), and the boundary derivative amplifies a last-bit difference of U
+    by up to q/|V| ~ 1e-3 on optically thin columns."""
+    import ocotillo_amd as oco
+    from test_gpu_mixed import FLUX_TOL, RES_FACTOR
+    jobs = [make_job(*j) for j in JOBS[:3]] + [make_job(19, 2, 1, 1000, 9), make_job(20, 1, 2, 7, 5)]
+    with oco.Context(0) as c:
+        c.set_physics(2e4, False, 0.4)
+        with oco.Server(c, 1 * GiB, max_nz=1024, mode=oco.MODE_MIXED) as srv:
+            outs = [srv.wait(srv.submit(*j)) for j in jobs]
+        for (rho, T, dz, waves), (em, r) in zip(jobs, outs):
+            assert r.status == 0
+            c.set_wavelengths(waves)
+            U, V, A, d = c.solve_block(rho, T, dz, mode=oco.MODE_MIXED)
+            blk = np.stack([V[..., 0], V[..., -1]], -1)
+            q = check_emergent(em, rho, T, dz, waves, "served")
+            check_emergent(blk, rho, T, dz, waves, "block")
+            same = (em == blk).mean()
+            rel = np.abs(em - blk) / np.abs(blk)
+            print("case", rho.shape, waves.size, "bit-identical %.4f, max rel diff %.2e" % (same, rel.max()),
+                  flush=True)
+            for i in (0, 1):
+                assert (np.abs(em[..., i] - blk[..., i]) <= 2 * (FLUX_TOL * np.abs(blk[..., i]) + RES_FACTOR * q[i])).all()
+
+
+def test_mixed_long_columns_within_bounds(ctx):
+    """Mixed serving of columns beyond the block wavefront sweep (nz 1500 and
+    2048, 33 rows per lane): emergent V within 1e-6 of the difference-form
+    oracle where FP64 resolves V, else within 1e-6 |V| + 64 q."""
+    import ocotillo_amd as oco
+    jobs = [make_job(21, 2, 1, 1500, 12), make_job(22, 1, 1, 2048, 7)]
+    with oco.Server(ctx, 1 * GiB, max_nz=2048, mode=oco.MODE_MIXED) as srv:
+        outs = [srv.wait(srv.submit(*j)) for j in jobs]
+    for (rho, T, dz, waves), (em, r) in zip(jobs, outs):
+        print("case", rho.shape, waves.size, flush=True)
+        assert r.status == 0 and np.isfinite(em).all()
+        check_emergent(em, rho, T, dz, waves, "served")
 
 
 def test_admission_within_budget(ctx):
     """A budget that holds a few jobs at a time: all complete, the arena's
     high-water mark stays within the budget, later jobs queue."""
     import ocotillo_amd as oco
-    # a job: 16 columns x 256 cells -> inputs 2 x 32 KiB, cell records 768 KiB,
-    # emergent flux 16 KiB; the arena holds two of them beside the table
-    job_bytes = 2 * 32768 + 16 * 256 * 24 * 8 + 16 * 64 * 2 * 8
+    # a mixed job: 16 columns x 256 cells -> inputs 2 x 32 KiB, cell records
+    # in the serving layout (5 rows per lane: 5 x 64 x 22 doubles a column,
+    # 880 KiB), emergent flux 16 KiB; the arena holds two of them beside the
+    # table
+    job_bytes = 2 * 32768 + 16 * 5 * 64 * 22 * 8 + 16 * 64 * 2 * 8
     budget = 2 * job_bytes + (64 << 10)
     jobs = [make_job(100 + i, 4, 4, 256, 64) for i in range(10)]
     with oco.Server(ctx, max(budget, 1 << 20), max_nz=256, mode=oco.MODE_MIXED) as srv:
