@@ -1543,19 +1543,26 @@ int oco_serve_win_rows(int max_nz) { return 2 * (serve_rows(max_nz) + 3); }
 
 size_t oco_cells_chunked_doubles(long ncol, int nz) { return (size_t)ncol * serve_rows(nz) * CR_NFIELD * OCO_WAVE; }
 
+/* Thread i takes slot (column, row in chunk j, lane l), lane fastest, so a
+ * wavefront's field stores are one 512-byte line each (cell-order threads
+ * would store 64 lines 11 KB apart); padding slots (l*m + j >= nz) are
+ * skipped. */
 __global__ __launch_bounds__(256) void cells_chunked_fp64(const double* __restrict__ rho, const double* __restrict__ T,
-                                                          long ncells, int nz, double fully_ionized_T,
+                                                          long ncol, int nz, double fully_ionized_T,
                                                           double* __restrict__ cells)
 {
     __shared__ oco_mtab smt;
     load_mtab(&smt);
     const int m = serve_rows(nz);
-    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < ncells; i += (long)gridDim.x * blockDim.x) {
+    const long per_col = (long)m * OCO_WAVE, nslots = ncol * per_col;
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < nslots; i += (long)gridDim.x * blockDim.x) {
+        const long col = i / per_col;
+        const int r = (int)(i - col * per_col), j = r / OCO_WAVE, l = r - j * OCO_WAVE, z = l * m + j;
+        if (z >= nz) continue;
+        const long c = col * nz + z;
         double rec[CR_NFIELD];
-        oco_cell_record(rho[i], T[i], fully_ionized_T, &smt, rec);
-        const long col = i / nz;
-        const int z = (int)(i - col * nz), l = z / m, j = z - l * m;
-        double* out = cells + col * (long)m * CR_NFIELD * OCO_WAVE + (long)j * CR_NFIELD * OCO_WAVE + l;
+        oco_cell_record(rho[c], T[c], fully_ionized_T, &smt, rec);
+        double* out = cells + col * per_col * CR_NFIELD + (long)j * CR_NFIELD * OCO_WAVE + l;
 #pragma unroll
         for (int k = 0; k < CR_NFIELD; k++) out[k * OCO_WAVE] = rec[k];
     }
@@ -1565,16 +1572,29 @@ hipError_t oco_launch_cells_chunked(const double* rho, const double* T, long nce
                                     double* cells, hipStream_t stream)
 {
     (void)hipGetLastError();
-    if (ncells <= 0) return hipSuccess;
-    long blocks = (ncells + 255) / 256;
+    if (ncells <= 0 || nz <= 0) return hipSuccess;
+    const long ncol = ncells / nz, nslots = ncol * serve_rows(nz) * OCO_WAVE;
+    long blocks = (nslots + 255) / 256;
     if (blocks > 65536) blocks = 65536;
-    hipLaunchKernelGGL(cells_chunked_fp64, dim3((unsigned)blocks), dim3(256), 0, stream, rho, T, ncells, nz,
+    hipLaunchKernelGGL(cells_chunked_fp64, dim3((unsigned)blocks), dim3(256), 0, stream, rho, T, ncol, nz,
                        fully_ionized_T, cells);
     return hipGetLastError();
 }
 
+/* Waves of the serving kernel per SIMD the compiler is asked to fit: 3
+ * (168 VGPRs, some spilled) serves 7.7e10 points/s on the serving workload
+ * where 2 (211 VGPRs) serves 7.2e10; its chunk loop waits on each row's
+ * record loads, and a third wavefront hides more of that. */
+#ifndef OCO_SERVE_WAVES_PER_EU
+#define OCO_SERVE_WAVES_PER_EU 3
+#endif
+#if OCO_SERVE_WAVES_PER_EU > 0
+#define OCO_SERVE_ATTR __attribute__((amdgpu_waves_per_eu(OCO_SERVE_WAVES_PER_EU)))
+#else
+#define OCO_SERVE_ATTR
+#endif
 template <int K>
-__global__ __launch_bounds__(OCO_WAVE * OCO_SERVE_WG) void serve_wave(const oco_serve_args a)
+__global__ __launch_bounds__(OCO_WAVE * OCO_SERVE_WG) OCO_SERVE_ATTR void serve_wave(const oco_serve_args a)
 {
     extern __shared__ double win[];   /* per wavefront: K systems x win_rows rows x (D or x, W, Z, alpha) */
     __shared__ int tile_s;
