@@ -1019,15 +1019,15 @@ __device__ static inline void wave_points(const double* cc, int nz, const double
  * x[i] = D x[i+1] + W xL + Z for the chunk's rows, xL the previous chunk's last
  * unknown, the deficit E = 1 - D - W carried positively. */
 struct rh_chunk {
-    double Wp, Zp, Ep;       /* the previous row's W, Z, E (the parameter row x = xL: 1, 0, 0) */
+    double Dp, Wp, Zp, Ep;   /* the latest row's D, W, Z, E (the parameter row x = xL: -, 1, 0, 0) */
     double pr, sD, sE;       /* the chunk's first row: pr G + (..) xL + sD, deficit sE */
-    double DL, WL, ZL, EL;   /* its last row */
+    double DL, WL, ZL, EL;   /* its last row (rh_chunk_close) */
     double rlo;              /* 1 / mean alpha of the interface below the current row */
 };
 
 __device__ static inline void rh_chunk_init(rh_chunk& c)
 {
-    c.Wp = 1.0; c.Zp = 0.0; c.Ep = 0.0;
+    c.Dp = 0.0; c.Wp = 1.0; c.Zp = 0.0; c.Ep = 0.0;
     c.pr = 1.0; c.sD = 0.0; c.sE = 0.0;
     c.DL = 0.0; c.WL = 0.0; c.ZL = 0.0; c.EL = 0.0;
     c.rlo = 0.0;
@@ -1035,15 +1035,24 @@ __device__ static inline void rh_chunk_init(rh_chunk& c)
 
 /* Row z of the system (source/auxiliary.f90:147-197 in difference form) and
  * its elimination step: al, om, S the cell's point, anext the alpha of cell
- * z + 1 (read for z < nz - 1), last whether z is the chunk's last row.  The
- * interface reciprocal is shared by this row's A and the previous row's C
- * (the same sum, so the same bits as rh_interior_row); a chunk starting above
- * row 0 sets c.rlo from its first cell and the cell below before its first
- * row.  D, W, Z: the row's coefficients of x[z+1], xL and 1. */
-__device__ static inline void rh_chunk_row(rh_chunk& c, int z, int nz, bool last, double al, double om, double S,
+ * z + 1 (read for z < nz - 1), first whether z is the chunk's first row.  The
+ * first-row summary takes in every row but the chunk's last: each row adds
+ * the previous one's, so the test is on the first row (the same for every
+ * lane of the wavefront) rather than the last, and the last row's values are
+ * the state left behind (rh_chunk_close).  The interface reciprocal is
+ * shared by this row's A and the previous row's C (the same sum, so the same
+ * bits as rh_interior_row); a chunk starting above row 0 sets c.rlo from its
+ * first cell and the cell below before its first row.  D, W, Z: the row's
+ * coefficients of x[z+1], xL and 1. */
+__device__ static inline void rh_chunk_row(rh_chunk& c, int z, int nz, bool first, double al, double om, double S,
                                            double anext, double dz, double dz2, double& D, double& W, double& Z)
 {
 #pragma clang fp contract(fast)   /* mixed mode: error bounds, not the reference's bits */
+    if (!first) {
+        c.sD += c.pr * c.Zp;
+        c.sE += c.pr * c.Ep;
+        c.pr *= c.Dp;
+    }
     const double al2 = al * al;
     double Ac, Cc, H, R;
     if (z == 0 || z == nz - 1) {
@@ -1065,15 +1074,14 @@ __device__ static inline void rh_chunk_row(rh_chunk& c, int z, int nz, bool last
     D = Cc * r;
     W = (Ac * c.Wp) * r;
     Z = (R + Ac * c.Zp) * r;
-    const double E = (Ac * c.Ep + H) * r;
-    if (!last) {
-        c.sD += c.pr * Z;
-        c.sE += c.pr * E;
-        c.pr *= D;
-    } else {
-        c.DL = D; c.WL = W; c.ZL = Z; c.EL = E;
-    }
-    c.Wp = W; c.Zp = Z; c.Ep = E;
+    c.Ep = (Ac * c.Ep + H) * r;
+    c.Dp = D; c.Wp = W; c.Zp = Z;
+}
+
+/* after the chunk's last row */
+__device__ static inline void rh_chunk_close(rh_chunk& c)
+{
+    c.DL = c.Dp; c.WL = c.Wp; c.ZL = c.Zp; c.EL = c.Ep;
 }
 
 /* The reduced system in the chunks' last unknowns, all lanes of the wavefront:
@@ -1168,9 +1176,10 @@ __device__ static inline void solve_system_wave(const oco_sys& d, double* lds)
         const int z = z0 + j;
         Dr[j] = 0.0; Wr[j] = 0.0; Zr[j] = 0.0;
         if (z < z1)
-            rh_chunk_row(c, z, nz, z == z1 - 1, LA[z], LW[z], LS[z + 3], z < nz - 1 ? LA[z + 1] : 0.0, dz, dz2,
+            rh_chunk_row(c, z, nz, j == 0, LA[z], LW[z], LS[z + 3], z < nz - 1 ? LA[z + 1] : 0.0, dz, dz2,
                          Dr[j], Wr[j], Zr[j]);
     }
+    rh_chunk_close(c);
     /* 3. the reduced system */
     double G, GL;
     rh_reduced(c, z1 > z0, G, GL);
@@ -1666,7 +1675,7 @@ __global__ __launch_bounds__(OCO_WAVE * OCO_SERVE_WG) OCO_SERVE_ATTR void serve_
                                 if (z0 > 0) st[k].rlo = mx_rcp(0.5 * (p.alpha + anb[k]));
                             } else {   /* row z - 1, now that alpha(z) is known */
                                 double D, W, Z;
-                                rh_chunk_row(st[k], z - 1, nz, false, ap[k], op[k], Sp[k], p.alpha, dz, dz2, D, W, Z);
+                                rh_chunk_row(st[k], z - 1, nz, j == 1, ap[k], op[k], Sp[k], p.alpha, dz, dz2, D, W, Z);
                                 if (win) {
                                     double* q = wslot(k, z - 1);
                                     q[0] = D; q[1] = W; q[2] = Z; q[3] = ap[k];
@@ -1684,7 +1693,9 @@ __global__ __launch_bounds__(OCO_WAVE * OCO_SERVE_WG) OCO_SERVE_ATTR void serve_
                     const double an = lane_get(a0[k], dn1);   /* alpha of cell z1, the next chunk's first */
                     if (z0 < z1) {   /* the chunk's last row */
                         double D, W, Z;
-                        rh_chunk_row(st[k], z1 - 1, nz, true, ap[k], op[k], Sp[k], z1 < nz ? an : 0.0, dz, dz2, D, W, Z);
+                        rh_chunk_row(st[k], z1 - 1, nz, z1 - 1 == z0, ap[k], op[k], Sp[k], z1 < nz ? an : 0.0, dz,
+                                     dz2, D, W, Z);
+                        rh_chunk_close(st[k]);
                         if (win) {
                             double* q = wslot(k, z1 - 1);
                             q[0] = D; q[1] = W; q[2] = Z; q[3] = ap[k];

@@ -110,7 +110,10 @@ __device__ static inline oco_pt oco_point_mixed(const double* c, const double* w
     sm = sm + w[WT_G5 * WS] * c[CR_E5];
     sm = sm + w[WT_G6 * WS] * c[CR_E6];
     double kHbf = (((OCO_AHBF * stim) * fn) * w[WT_WA3 * WS]) * (c[CR_UNSOLD] + sm);
-    double kHmbf = (w[WT_HMBF_ON * WS] != 0.0) ? ((c[CR_HMBF] * stim) * fn) * w[WT_SCOEFF * WS] : 0.0;
+    /* H- bound-free below 16000 A: the switch selects the wavelength's
+     * coefficient (0 above), not the term, so no per-point select */
+    const double sco = w[WT_HMBF_ON * WS] != 0.0 ? w[WT_SCOEFF * WS] : 0.0;
+    double kHmbf = ((c[CR_HMBF] * stim) * fn) * sco;
     double flam = ((w[WT_F1 * WS] + w[WT_F2 * WS] * c[CR_LGT]) + w[WT_F3 * WS] * c[CR_LGT2]) - 26.0;
     double kHmff = c[CR_PHMFF] * mx_exp10(flam);
     double kappa = (((kHbf + kHmbf) + kHmff) + c[CR_ES]) * OCO_MP1;
@@ -133,8 +136,7 @@ __device__ static inline oco_pt oco_point_mixed(const double* c, const double* w
     p.alpha = alpha;
     p.alpha1 = mx_rcp(alpha);
     p.alpha2 = alpha * alpha;
-    double damp = c[CR_DT1] * w[WT_WCM1 * WS];
-    if (damp > OCO_LOG_OVF) damp = OCO_LOG_OVF;
+    const double damp = __builtin_fmin(c[CR_DT1] * w[WT_WCM1 * WS], OCO_LOG_OVF);
     p.S = mx_div(w[WT_SFW5 * WS], mx_expm1(damp));
     return p;
 }
