@@ -128,8 +128,9 @@ def main():
                          "for scaling the FETCH_SIZE / WRITE_SIZE counters")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py: this build measures one GPU; the path shards by columns/wavelengths with no "
-              "exchange (see DESIGN.md), multi-GPU runs are not part of it", file=sys.stderr)
+        print("bench.py: measures one GPU only (--gpus 1); the path would shard by wavelength ranges or "
+              "whole blocks, with no data exchanged between GPUs, but multi-GPU runs are not implemented",
+              file=sys.stderr)
         sys.exit(2)
     if args.mode is None:
         args.mode = "fp64" if args.config == "c2" else "both"
@@ -638,9 +639,11 @@ def bench_batch(args):
         t1 = time.perf_counter()
         ms_per_step = (t1 - t0) * 1e3 / args.steps
         variant, nblocks, _ = b.tuning(code)
-        # mixed variants 3, 4, 5: the wavefront-per-system kernel in workgroups of 1, 2, 4
+        # mixed variants 3 + 3a + b: the wavefront-per-system kernel, columns of up to 512 cells in
+        # workgroups of (1, 2, 4)[a] wavefronts, longer ones of (1, 2, 4)[b]
         kern = ("batch_fp64 (graph)" if mode == "fp64" else
-                "batch_mixed_wave, %d wavefronts/workgroup (graph)" % (1 << (variant - 3)) if variant >= 3 else
+                "batch_mixed_wave, %d/%d wavefronts per workgroup (short/long columns) (graph)"
+                % ((1, 2, 4)[(variant - 3) // 3], (1, 2, 4)[(variant - 3) % 3]) if variant >= 3 else
                 "batch_mixed (graph)")
         res[mode] = {"value": points / (ms_per_step * 1e-3), "ms_per_step": ms_per_step,
                      "roofline": roofline_of(kern, points, point_weights(mode, fion), float(np.mean(ms))),

@@ -872,10 +872,14 @@ struct oco_batch {
      * the field-major records */
     DevBuf witems, wtickets, cells_soa;
     bool wave_ok = false;
-    long wio[3] = {0, 0, 0}, wqo[3] = {0, 0, 0};
+    /* two classes of columns for that kernel, each launched for its own
+     * longest column: nz <= OCO_BATCH_SHORT_NZ, and the rest */
+    int cls_max_nz[2] = {0, 0};
+    long wio[2][3] = {{0}}, wqo[2][3] = {{0}};
     /* per precision mode: autotuned kernel variant and grid, captured graph
-     * (mixed variants 0..2: lane per system; 3, 4, 5: wavefront per system in
-     * workgroups of 1, 2, 4) */
+     * (mixed variants 0..2: lane per system; 3 + 3 a + b: wavefront per system,
+     * the short class in workgroups of 1, 2, 4 wavefronts for a = 0, 1, 2 and
+     * the long class for b) */
     int nblocks[2] = {0, 0}, variant[2] = {1, 1}, tuned[2] = {0, 0};
     float tune_ms[2][16] = {{0}};
     hipGraph_t graph[2] = {nullptr, nullptr};
@@ -930,17 +934,46 @@ static oco_batch_args batch_args(oco_batch* b)
 
 static const int BATCH_WAVE_K[3] = {1, 2, 4};
 #define OCO_BATCH_LANE_VARIANTS 3
+/* Columns up to this many cells form the short class of the batched
+ * wavefront kernel: 9 rows per lane, and 3 workgroups of 4 wavefronts fit a
+ * CU's LDS. */
+#define OCO_BATCH_SHORT_NZ 512
+
+/* workgroups of the batched wavefront kernel in wavefront variant v (the
+ * classes' sizes 3 v / 3 and v % 3 of BATCH_WAVE_K): the larger of the two
+ * classes' resident grids; 0 if a class does not fit */
+static int batch_wave_blocks(const oco_batch* b, int v, int device)
+{
+    int g = 0;
+    const int iv[2] = {v / 3, v % 3};
+    for (int cl = 0; cl < 2; cl++) {
+        if (b->cls_max_nz[cl] == 0) continue;
+        const int gc = oco_batch_wave_grid(b->cls_max_nz[cl], BATCH_WAVE_K[iv[cl]], device);
+        if (gc <= 0) return 0;
+        g = std::max(g, gc);
+    }
+    return g;
+}
 
 /* enqueue one full batch solve with the given kernel variant / grid */
 static hipError_t batch_enqueue(oco_batch* b, int mixed, int variant, int nblocks)
 {
     oco_ctx* c = b->ctx;
     if (mixed && variant >= OCO_BATCH_LANE_VARIANTS) {
-        const int i = variant - OCO_BATCH_LANE_VARIANTS;
+        (void)nblocks;   /* each class takes its own resident grid */
+        const int v = variant - OCO_BATCH_LANE_VARIANTS, iv[2] = {v / 3, v % 3};
         oco_batch_args a = batch_args(b);
-        a.witems = (const int*)b->witems.p + b->wio[i];
-        a.wq_off = (const int*)b->witems.p + b->wqo[i];
-        return oco_launch_batch_wave(a, b->ncol, c->fully_ionized_T, BATCH_WAVE_K[i], nblocks, c->stream);
+        hipError_t e = oco_launch_batch_wave_records(a, b->ncol, c->fully_ionized_T, c->stream);
+        for (int cl = 0; cl < 2 && e == hipSuccess; cl++) {
+            if (b->cls_max_nz[cl] == 0) continue;
+            const int i = iv[cl], K = BATCH_WAVE_K[i];
+            a.max_nz = b->cls_max_nz[cl];
+            a.witems = (const int*)b->witems.p + b->wio[cl][i];
+            a.wq_off = (const int*)b->witems.p + b->wqo[cl][i];
+            a.wtickets = (unsigned long long*)b->wtickets.p + cl * OCO_TICKET_QUEUES * OCO_TICKET_STRIDE;
+            e = oco_launch_batch_wave(a, K, oco_batch_wave_grid(a.max_nz, K, c->device), c->stream);
+        }
+        return e;
     }
     hipError_t e = oco_launch_cells((const double*)b->rho.p, (const double*)b->T.p, b->ncells, c->fully_ionized_T,
                                     (double*)b->cells.p, 0, c->stream);
@@ -1015,27 +1048,35 @@ int oco_batch_create(oco_ctx* c, int ncol, const int* nz, const int* nw, const d
     std::vector<int> witems;
     b->wave_ok = c->mixed_sweep != 2 && oco_batch_wave_grid(b->max_nz, 1, c->device) > 0;
     if (b->wave_ok) {
-        for (int i = 0; i < 3; i++) {
-            const int K = BATCH_WAVE_K[i];
-            std::vector<int> q[OCO_TICKET_QUEUES];
-            for (int p = 0; p < ncol; p++) {
-                const int col = order[p];
-                for (int w0 = 0; w0 < nw[col]; w0 += K) {
-                    q[p % OCO_TICKET_QUEUES].push_back(col);
-                    q[p % OCO_TICKET_QUEUES].push_back(w0);
+        for (int i : order) {
+            const int cl = nz[i] <= OCO_BATCH_SHORT_NZ ? 0 : 1;
+            b->cls_max_nz[cl] = std::max(b->cls_max_nz[cl], nz[i]);
+        }
+        for (int cl = 0; cl < 2; cl++) {
+            for (int i = 0; i < 3; i++) {
+                const int K = BATCH_WAVE_K[i];
+                std::vector<int> q[OCO_TICKET_QUEUES];
+                int p = 0;   /* position among the class's columns, longest first */
+                for (int col : order) {
+                    if ((nz[col] <= OCO_BATCH_SHORT_NZ ? 0 : 1) != cl) continue;
+                    for (int w0 = 0; w0 < nw[col]; w0 += K) {
+                        q[p % OCO_TICKET_QUEUES].push_back(col);
+                        q[p % OCO_TICKET_QUEUES].push_back(w0);
+                    }
+                    p++;
                 }
+                b->wio[cl][i] = (long)witems.size();
+                int n = 0;
+                std::vector<int> qoff(OCO_TICKET_QUEUES + 1);
+                for (int k = 0; k < OCO_TICKET_QUEUES; k++) {
+                    qoff[k] = n;
+                    n += (int)(q[k].size() / 2);
+                    witems.insert(witems.end(), q[k].begin(), q[k].end());
+                }
+                qoff[OCO_TICKET_QUEUES] = n;
+                b->wqo[cl][i] = (long)witems.size();
+                witems.insert(witems.end(), qoff.begin(), qoff.end());
             }
-            b->wio[i] = (long)witems.size();
-            int n = 0;
-            std::vector<int> qoff(OCO_TICKET_QUEUES + 1);
-            for (int k = 0; k < OCO_TICKET_QUEUES; k++) {
-                qoff[k] = n;
-                n += (int)(q[k].size() / 2);
-                witems.insert(witems.end(), q[k].begin(), q[k].end());
-            }
-            qoff[OCO_TICKET_QUEUES] = n;
-            b->wqo[i] = (long)witems.size();
-            witems.insert(witems.end(), qoff.begin(), qoff.end());
         }
     }
     int dev_cus = 256;
@@ -1066,7 +1107,7 @@ int oco_batch_create(oco_ctx* c, int ncol, const int* nz, const int* nw, const d
     if (!st) st = ensure(c, b->fail, sizeof(unsigned long long));
     if (!st && b->wave_ok) {
         st = ensure(c, b->witems, witems.size() * sizeof(int));
-        if (!st) st = ensure(c, b->wtickets, OCO_TICKET_BYTES);
+        if (!st) st = ensure(c, b->wtickets, 2 * OCO_TICKET_BYTES);
         if (!st) st = ensure(c, b->cells_soa, (size_t)nc * CR_NFIELD * sizeof(double));
     }
     if (!st) {
@@ -1144,14 +1185,20 @@ static int batch_prepare(oco_batch* b, int m)
             dev_cus = prop.multiProcessorCount;
         float best = 1e30f;
         int k = 0;
-        const int nvar = oco_batch_variants() + (m == OCO_MODE_MIXED && b->wave_ok ? 3 : 0);
+        const int nvar = oco_batch_variants() + (m == OCO_MODE_MIXED && b->wave_ok ? 9 : 0);
         /* OCO_MIXED_SWEEP=wave: only the wavefront variants */
         const int v0 = m == OCO_MODE_MIXED && b->wave_ok && c->mixed_sweep == 1 ? OCO_BATCH_LANE_VARIANTS : 0;
         for (int v = v0; v < nvar; v++) {
             const bool wave = v >= OCO_BATCH_LANE_VARIANTS;
+            /* wavefront variants: a class absent from the batch makes its
+             * size irrelevant, so only the first of those variants is tried */
+            if (wave) {
+                const int wv = v - OCO_BATCH_LANE_VARIANTS;
+                if ((b->cls_max_nz[0] == 0 && wv / 3 != 0) || (b->cls_max_nz[1] == 0 && wv % 3 != 0)) continue;
+            }
             for (int wps = 2; wps <= (wave ? 2 : 4); wps++) {
-                /* wavefront variants: the occupancy grid (one try) */
-                int nb = wave ? oco_batch_wave_grid(b->max_nz, BATCH_WAVE_K[v - OCO_BATCH_LANE_VARIANTS], c->device)
+                /* wavefront variants: the occupancy grids (one try) */
+                int nb = wave ? batch_wave_blocks(b, v - OCO_BATCH_LANE_VARIANTS, c->device)
                               : std::min(b->ntiles, dev_cus * 4 * wps);
                 if (nb <= 0) continue;
                 HIPCHK(hipEventRecord(c->ev[0], c->stream));

@@ -135,10 +135,13 @@ int oco_serve_grid(int mixed, int max_nz, int device);
 hipError_t oco_launch_fail_reset(unsigned long long* fail, int n, hipStream_t stream);
 /* mixed != 0 selects the mixed-precision kernels (non-grey only). */
 hipError_t oco_launch_batch(const oco_batch_args& a, int mixed, int variant, int nblocks, hipStream_t stream);
-/* batched mixed mode, wavefront per system, k (1, 2, 4) wavefronts per
- * workgroup: per-column field-major records, then the sweep */
-hipError_t oco_launch_batch_wave(const oco_batch_args& a, int ncol, double fully_ionized_T, int k, int nblocks,
-                                 hipStream_t stream);
+/* batched mixed mode, wavefront per system: the per-column field-major
+ * records (and the failure key), then the sweep of one class of columns (its
+ * items, tickets and longest column in a) with k (1, 2, 4) wavefronts per
+ * workgroup */
+hipError_t oco_launch_batch_wave_records(const oco_batch_args& a, int ncol, double fully_ionized_T,
+                                         hipStream_t stream);
+hipError_t oco_launch_batch_wave(const oco_batch_args& a, int k, int nblocks, hipStream_t stream);
 /* resident workgroups of that sweep for columns of at most max_nz cells; 0 if
  * such columns do not fit it */
 int oco_batch_wave_grid(int max_nz, int k, int device);

@@ -1906,15 +1906,22 @@ static void launch_batch_wave(const oco_batch_args& a, int nblocks, hipStream_t 
     else hipLaunchKernelGGL((batch_mixed_wave<17, K>), g, b, sh, stream, a);
 }
 
-hipError_t oco_launch_batch_wave(const oco_batch_args& a, int ncol, double fully_ionized_T, int k, int nblocks,
-                                 hipStream_t stream)
+hipError_t oco_launch_batch_wave_records(const oco_batch_args& a, int ncol, double fully_ionized_T,
+                                         hipStream_t stream)
 {
     (void)hipGetLastError();   /* a pending not-ready status from an event query is not this launch's */
-    if (!a.wtickets || !a.cells_soa || ncol <= 0 || nblocks <= 0) return hipErrorInvalidValue;
+    if (!a.cells_soa || ncol <= 0) return hipErrorInvalidValue;
     hipLaunchKernelGGL(batch_reset, dim3(1), dim3(64), 0, stream, a.counter, a.fail);
-    hipLaunchKernelGGL(tickets_reset, dim3(1), dim3(64), 0, stream, a.wtickets);
     hipLaunchKernelGGL(cells_cols_fp64, dim3(ncol), dim3(256), 0, stream, a.rho, a.T, a.nz, a.cell_off,
                        fully_ionized_T, (double*)a.cells_soa);
+    return hipGetLastError();
+}
+
+hipError_t oco_launch_batch_wave(const oco_batch_args& a, int k, int nblocks, hipStream_t stream)
+{
+    (void)hipGetLastError();
+    if (!a.wtickets || !a.cells_soa || nblocks <= 0) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(tickets_reset, dim3(1), dim3(64), 0, stream, a.wtickets);
     if (k == 4) launch_batch_wave<4>(a, nblocks, stream);
     else if (k == 2) launch_batch_wave<2>(a, nblocks, stream);
     else launch_batch_wave<1>(a, nblocks, stream);
