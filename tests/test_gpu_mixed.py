@@ -250,14 +250,15 @@ def test_wave_groups_equal(nz, nw):
         assert np.array_equal(A, out[0][2]), g
 
 
-@pytest.mark.parametrize("kind", ["lane", "wave"])
-def test_batch_mixed_equals_block(kind):
+@pytest.mark.parametrize("kind,nz", [("lane", 96), ("wave", 96), ("wave", 600)])
+def test_batch_mixed_equals_block(kind, nz):
     """Batched (persistent, graph-replayed) mixed solve == the block mixed
     solve with the same kernel kind (OCO_MIXED_SWEEP restricts the batch's
-    autotune to it), bit for bit, on every replay."""
+    autotune to it), bit for bit, on every replay; nz 96 and 600 put every
+    column in the batched wavefront kernel's short and long class."""
     import ocotillo_amd as oco
     from ocotillo_amd.workloads import config2
-    rho, T, dz, waves = config2(2, 3, 96, 80)
+    rho, T, dz, waves = config2(2, 3, nz, 80)
     ncol = 6
     nz, nw = rho.shape[2], waves.size
     ctx = sweep_context(kind)
