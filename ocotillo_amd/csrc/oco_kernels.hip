@@ -257,14 +257,17 @@ __device__ static inline void boundary_row(const oco_pt& pc, double dz, double d
 /* The same rows in difference form (mixed mode): -A x[i-1] + (A + C + H) x[i]
  * - C x[i+1] = R with A, C, H, R >= 0; H is the row's excess over A + C
  * (zeta inside; alpha^2 (dz + zeta) at the boundary rows, whose coupling is
- * alpha). */
+ * alpha).  An interior row is scaled by the product of its interface means
+ * k- k+ (A = alpha^2 k+, C = alpha^2 k-, H and R times k- k+): no reciprocal,
+ * and the elimination's ratios within the row are unchanged. */
 __device__ static inline void rh_interior_row(double am, const oco_pt& pc, double ap, double dz2, double& A,
                                               double& C, double& H, double& R)
 {
 #pragma clang fp contract(fast)
-    A = pc.alpha2 * mx_rcp(0.5 * (pc.alpha + am));
-    C = pc.alpha2 * mx_rcp(0.5 * (ap + pc.alpha));
-    H = (dz2 * (pc.alpha2 * pc.alpha)) * (1.0 - pc.albedo);
+    const double km = 0.5 * (pc.alpha + am), kp = 0.5 * (ap + pc.alpha);
+    A = pc.alpha2 * kp;
+    C = pc.alpha2 * km;
+    H = ((dz2 * (pc.alpha2 * pc.alpha)) * (1.0 - pc.albedo)) * (km * kp);
     R = pc.S * H;
 }
 
@@ -1022,7 +1025,7 @@ struct rh_chunk {
     double Dp, Wp, Zp, Ep;   /* the latest row's D, W, Z, E (the parameter row x = xL: -, 1, 0, 0) */
     double pr, sD, sE;       /* the chunk's first row: pr G + (..) xL + sD, deficit sE */
     double DL, WL, ZL, EL;   /* its last row (rh_chunk_close) */
-    double rlo;              /* 1 / mean alpha of the interface below the current row */
+    double mlo;              /* mean alpha of the interface below the current row */
 };
 
 __device__ static inline void rh_chunk_init(rh_chunk& c)
@@ -1030,7 +1033,7 @@ __device__ static inline void rh_chunk_init(rh_chunk& c)
     c.Dp = 0.0; c.Wp = 1.0; c.Zp = 0.0; c.Ep = 0.0;
     c.pr = 1.0; c.sD = 0.0; c.sE = 0.0;
     c.DL = 0.0; c.WL = 0.0; c.ZL = 0.0; c.EL = 0.0;
-    c.rlo = 0.0;
+    c.mlo = 0.0;
 }
 
 /* Row z of the system (source/auxiliary.f90:147-197 in difference form) and
@@ -1039,11 +1042,13 @@ __device__ static inline void rh_chunk_init(rh_chunk& c)
  * first-row summary takes in every row but the chunk's last: each row adds
  * the previous one's, so the test is on the first row (the same for every
  * lane of the wavefront) rather than the last, and the last row's values are
- * the state left behind (rh_chunk_close).  The interface reciprocal is
- * shared by this row's A and the previous row's C (the same sum, so the same
- * bits as rh_interior_row); a chunk starting above row 0 sets c.rlo from its
- * first cell and the cell below before its first row.  D, W, Z: the row's
- * coefficients of x[z+1], xL and 1. */
+ * the state left behind (rh_chunk_close).  An interior row is scaled by the
+ * product of its two interface means m-, m+ (A = alpha^2 m+, C = alpha^2 m-,
+ * H and R times m- m+), which takes the two reciprocals out of the row; D, W,
+ * Z and E are ratios within one row, so the scale leaves them as they are.
+ * The mean below is the one above of the previous row; a chunk starting above
+ * row 0 sets c.mlo from its first cell and the cell below before its first
+ * row.  D, W, Z: the row's coefficients of x[z+1], xL and 1. */
 __device__ static inline void rh_chunk_row(rh_chunk& c, int z, int nz, bool first, double al, double om, double S,
                                            double anext, double dz, double dz2, double& D, double& W, double& Z)
 {
@@ -1061,14 +1066,14 @@ __device__ static inline void rh_chunk_row(rh_chunk& c, int z, int nz, bool firs
         rh_boundary_row(pc, dz, dz2, H, R);
         Ac = z == 0 ? 0.0 : al;
         Cc = z == 0 ? al : 0.0;
-        if (z == 0) c.rlo = mx_rcp(0.5 * (anext + al));
+        if (z == 0) c.mlo = 0.5 * (anext + al);
     } else {
-        const double rhi = mx_rcp(0.5 * (anext + al));
-        Ac = al2 * c.rlo;
-        Cc = al2 * rhi;
-        H = (dz2 * (al2 * al)) * (1.0 - om);
+        const double mhi = 0.5 * (anext + al);
+        Ac = al2 * mhi;
+        Cc = al2 * c.mlo;
+        H = ((dz2 * (al2 * al)) * (1.0 - om)) * (c.mlo * mhi);
         R = S * H;
-        c.rlo = rhi;
+        c.mlo = mhi;
     }
     const double r = mx_rcp((Ac * (c.Wp + c.Ep) + Cc) + H);   /* 1 - D[i-1] = W + E */
     D = Cc * r;
@@ -1170,7 +1175,7 @@ __device__ static inline void solve_system_wave(const oco_sys& d, double* lds)
     double Dr[MM], Wr[MM], Zr[MM];
     rh_chunk c;
     rh_chunk_init(c);
-    if (z0 > 0 && z0 < z1) c.rlo = mx_rcp(0.5 * (LA[z0] + LA[z0 - 1]));
+    if (z0 > 0 && z0 < z1) c.mlo = 0.5 * (LA[z0] + LA[z0 - 1]);
 #pragma unroll
     for (int j = 0; j < MM; j++) {
         const int z = z0 + j;
@@ -1672,7 +1677,7 @@ __global__ __launch_bounds__(OCO_WAVE * OCO_SERVE_WG) OCO_SERVE_ATTR void serve_
                             const oco_pt p = oco_point_mixed<1>(rec, w);
                             if (j == 0) {
                                 a0[k] = p.alpha;
-                                if (z0 > 0) st[k].rlo = mx_rcp(0.5 * (p.alpha + anb[k]));
+                                if (z0 > 0) st[k].mlo = 0.5 * (p.alpha + anb[k]);
                             } else {   /* row z - 1, now that alpha(z) is known */
                                 double D, W, Z;
                                 rh_chunk_row(st[k], z - 1, nz, j == 1, ap[k], op[k], Sp[k], p.alpha, dz, dz2, D, W, Z);
