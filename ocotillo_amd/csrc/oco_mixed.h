@@ -8,9 +8,11 @@
  * v_exp_f32 / v_log_f32 on an FP64-reduced argument, with the FP64 exponent
  * put back by ldexp ("exponent scaling"), and divides use v_rcp_f64 with one
  * Newton step.  Relative error: about 1.2e-7 per exp (the FP32 core), about 1e-7
- * absolute per log, ~1e-16 per divide.  The three cancelling forms 1 - 10^(-x), exp(d) - 1 and
- * 1 - exp(-y) switch to an FP64 series below 0.5 so they keep full relative
- * accuracy at long wavelengths and high temperature.
+ * absolute per log, ~1e-16 per divide.  The cancelling forms 1 - exp(-y)
+ * (1 - 10^(-x) is one) and exp(d) - 1 need no transcendental at all: one
+ * FP64 series on the argument reduced by multiples of ln 2, scaled back by
+ * 2^n, keeps full relative accuracy at long wavelengths and high
+ * temperature.
  *
  * The Saha state, number densities and every other temperature-only factor
  * come from the FP64 cell kernel, unchanged.
@@ -52,8 +54,8 @@ __device__ static inline double mx_log(double x)
 }
 
 /* 1 - exp(-y) = y (1 - y/2 + y^2/6 - ...) for |y| < 1/2: the Taylor series to
- * degree 8 (truncation below 1.1e-8 relative, under the 1.2e-7 of the FP32
- * exp core it replaces), Horner with FMA */
+ * degree 9 (truncation below 1e-9 relative at 1/2, 2e-11 on the reduced
+ * range |y| <= ln2/2 where the callers use it), Horner with FMA */
 __device__ static inline double mx_series_omexp(double y)
 {
     double q = (1.0 / 362880);
@@ -68,20 +70,28 @@ __device__ static inline double mx_series_omexp(double y)
     return y * q;
 }
 
-/* 1 - exp(-y), y >= 0 (1 - exp(-1000) is 1 in double: larger y are clamped) */
+/* 1 - exp(-y), y >= 0 (1 - exp(-1000) is 1 in double: larger y are
+ * clamped).  y = n ln2 + r with |r| <= ln2/2 gives
+ * 1 - exp(-y) = (1 - 2^-n) + 2^-n (1 - exp(-r)): for n = 0 the series alone,
+ * for n >= 1 a first term of at least 1/2, so nothing cancels. */
 __device__ static inline double mx_omexp(double y)
 {
-    double s = mx_series_omexp(y);
-    double l = 1.0 - mx_exp(-__builtin_fmin(y, 1000.0));
-    return y < 0.5 ? s : l;
+    y = __builtin_fmin(y, 1000.0);
+    const double n = __builtin_rint(y * OCO_LOG2E);
+    const double r = __builtin_fma(-n, OCO_LN2, y);
+    const double t = __builtin_amdgcn_ldexp(1.0, -(int)n);
+    return __builtin_fma(t, mx_series_omexp(r), 1.0 - t);
 }
 
-/* exp(d) - 1, d >= 0 */
+/* exp(d) - 1, 0 <= d <= 708: d = n ln2 + r, |r| <= ln2/2, gives
+ * exp(d) - 1 = 2^n (exp(r) - 1) + (2^n - 1), with exp(r) - 1 = -(1 - exp(r))
+ * from the series; for n = 0 the series alone */
 __device__ static inline double mx_expm1(double d)
 {
-    double s = mx_series_omexp(-d);   /* 1 - exp(d) */
-    double l = mx_exp(d) - 1.0;
-    return d < 0.5 ? -s : l;
+    const double n = __builtin_rint(d * OCO_LOG2E);
+    const double r = __builtin_fma(-n, OCO_LN2, d);
+    const double t = __builtin_amdgcn_ldexp(1.0, (int)n);
+    return __builtin_fma(t, -mx_series_omexp(-r), t - 1.0);
 }
 
 /* 1/y and x/y: v_rcp_f64 refined by one Newton step */
