@@ -8,11 +8,12 @@
  * v_exp_f32 / v_log_f32 on an FP64-reduced argument, with the FP64 exponent
  * put back by ldexp ("exponent scaling"), and divides use v_rcp_f64 with one
  * Newton step.  Relative error: about 1.2e-7 per exp (the FP32 core), about 1e-7
- * absolute per log, ~1e-16 per divide.  The cancelling forms 1 - exp(-y)
- * (1 - 10^(-x) is one) and exp(d) - 1 need no transcendental at all: one
- * FP64 series on the argument reduced by multiples of ln 2, scaled back by
- * 2^n, keeps full relative accuracy at long wavelengths and high
- * temperature.
+ * absolute per log, ~1e-16 per divide.  The cancelling forms need no
+ * transcendental at all: exp(d) - 1 is one FP64 series on the argument
+ * reduced by multiples of ln 2, scaled back by 2^n, which keeps full
+ * relative accuracy at long wavelengths and high temperature, and the
+ * 1 - exp(-y) of stimulated emission (1 - 10^(-x)) and of the Gaunt factor
+ * follow from it, their y being the same h nu / kT.
  *
  * The Saha state, number densities and every other temperature-only factor
  * come from the FP64 cell kernel, unchanged.
@@ -30,7 +31,7 @@
 /* 2^t for |t| < 2^31: t = n + f, |f| <= 1/2, 2^f in FP32, 2^n by ldexp
  * (which gives 0 or inf beyond the double range).  Callers bound their
  * arguments: every exponent of the opacity is bounded by the cell's
- * temperature-only factors, except 1 - exp(-y), which clamps y. */
+ * temperature-only factors. */
 __device__ static inline double mx_exp2(double t)
 {
     double n = __builtin_rint(t);
@@ -70,19 +71,6 @@ __device__ static inline double mx_series_omexp(double y)
     return y * q;
 }
 
-/* 1 - exp(-y), y >= 0 (1 - exp(-1000) is 1 in double: larger y are
- * clamped).  y = n ln2 + r with |r| <= ln2/2 gives
- * 1 - exp(-y) = (1 - 2^-n) + 2^-n (1 - exp(-r)): for n = 0 the series alone,
- * for n >= 1 a first term of at least 1/2, so nothing cancels. */
-__device__ static inline double mx_omexp(double y)
-{
-    y = __builtin_fmin(y, 1000.0);
-    const double n = __builtin_rint(y * OCO_LOG2E);
-    const double r = __builtin_fma(-n, OCO_LN2, y);
-    const double t = __builtin_amdgcn_ldexp(1.0, -(int)n);
-    return __builtin_fma(t, mx_series_omexp(r), 1.0 - t);
-}
-
 /* exp(d) - 1, 0 <= d <= 708: d = n ln2 + r, |r| <= ln2/2, gives
  * exp(d) - 1 = 2^n (exp(r) - 1) + (2^n - 1), with exp(r) - 1 = -(1 - exp(r))
  * from the series; for n = 0 the series alone */
@@ -102,6 +90,23 @@ __device__ static inline double mx_rcp(double y)
 }
 __device__ static inline double mx_div(double x, double y) { return x * mx_rcp(y); }
 
+/* 1 - exp(-y) for y close to d, from E = exp(d) - 1 and r1 = 1 / (1 + E):
+ * 1 - exp(-y) = 1 - exp(-d) exp(-delta) = (E + (1 - exp(-delta))) r1 with
+ * delta = y - d, the bracket by its series to third order.  The opacity's
+ * exponents h nu / kT of stimulated emission (chi theta ln10, from the
+ * rounded 1.2398e4 eV A and 5040 K) and of the bremsstrahlung Gaunt factor
+ * (h nu k1 / T) are the Planck term's d times 1 + 1e-5 and 1 + 1e-16; the
+ * truncation, delta^4 / 24 against E, is then below 1e-19.  When d is
+ * clamped to 308, delta may be large, but the result is 1 in double as long
+ * as the series (y clamped to 1000) stays far below E = exp(308). */
+__device__ static inline double mx_omexp_near(double y, double d, double E, double r1)
+{
+#pragma clang fp contract(fast)
+    const double dl = __builtin_fmin(y, 1000.0) - d;
+    const double q = dl * __builtin_fma(dl, __builtin_fma(dl, 1.0 / 6.0, -0.5), 1.0);
+    return (E + q) * r1;
+}
+
 /* the opacity, albedo and source function of one (cell, wavelength), mixed
  * precision; same formulas as oco_point_s (source/continuous_opacity.f90) */
 template <int WS>
@@ -112,7 +117,14 @@ __device__ static inline oco_pt oco_point_mixed(const double* c, const double* w
 #pragma clang fp contract(fast)
     oco_pt p;
     double theta = c[CR_THETA], fn = c[CR_FN];
-    double stim = mx_omexp((w[WT_CHI * WS] * theta) * OCO_LN10);
+    /* the Planck term's exp(d) - 1, and 1/E, 1/(1 + E) from one reciprocal
+     * of E (1 + E) (below 1e268 for d <= 308) */
+    const double damp = __builtin_fmin(c[CR_DT1] * w[WT_WCM1 * WS], OCO_LOG_OVF);
+    const double E = mx_expm1(damp);
+    const double E1 = 1.0 + E;
+    const double rP = mx_rcp(E * E1);
+    const double rE = E1 * rP, r1 = E * rP;
+    double stim = mx_omexp_near((w[WT_CHI * WS] * theta) * OCO_LN10, damp, E, r1);
     double sm = w[WT_G1 * WS];
     sm = sm + w[WT_G2 * WS] * c[CR_E2];
     sm = sm + w[WT_G3 * WS] * c[CR_E3];
@@ -137,7 +149,7 @@ __device__ static inline oco_pt oco_point_mixed(const double* c, const double* w
     } else {
         double inner = w[WT_NU9 * WS] * c[CR_U0];
         double gB = mx_log(mx_exp(5.960 - (OCO_SQRT3 * OCO_PI1) * mx_log(inner)) + OCO_EXP1);
-        double lnQ = (c[CR_U1] - w[WT_L3NU * WS]) + mx_log(mx_omexp(w[WT_HNK * WS] * c[CR_U2]));
+        double lnQ = (c[CR_U1] - w[WT_L3NU * WS]) + mx_log(mx_omexp_near(w[WT_HNK * WS] * c[CR_U2], damp, E, r1));
         double kRho = (OCO_C_BREMS * mx_exp(lnQ)) * gB;
         alpha = alpha + kRho;
         kappa = kappa + alpha * c[CR_U3];
@@ -146,8 +158,7 @@ __device__ static inline oco_pt oco_point_mixed(const double* c, const double* w
     p.alpha = alpha;
     p.alpha1 = mx_rcp(alpha);
     p.alpha2 = alpha * alpha;
-    const double damp = __builtin_fmin(c[CR_DT1] * w[WT_WCM1 * WS], OCO_LOG_OVF);
-    p.S = mx_div(w[WT_SFW5 * WS], mx_expm1(damp));
+    p.S = w[WT_SFW5 * WS] * rE;
     return p;
 }
 
