@@ -158,6 +158,8 @@ void oco_build_wtab(const std::vector<double>& waves, std::vector<double>& wt)
         w[WT_HNK] = (OCO_H_PLANCK * nu) * OCO_K1_CGS;
         w[WT_WCM1] = wcm1;
         w[WT_SFW5] = sfc * ((((wcm1 * wcm1) * wcm1) * wcm1) * wcm1);
+        w[WT_HBFW] = OCO_AHBF * w[WT_WA3];
+        w[WT_HFFW] = OCO_ALPHA0 * w[WT_WA3];
     }
 }
 

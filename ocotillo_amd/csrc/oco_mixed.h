@@ -124,37 +124,41 @@ __device__ static inline oco_pt oco_point_mixed(const double* c, const double* w
     const double E1 = 1.0 + E;
     const double rP = mx_rcp(E * E1);
     const double rE = E1 * rP, r1 = E * rP;
-    double stim = mx_omexp_near((w[WT_CHI * WS] * theta) * OCO_LN10, damp, E, r1);
+    /* The same sums, regrouped: stim*fn is common to the three
+     * stimulated-emission terms, A_Hbf wa^3 and alpha0 wa^3 come from the
+     * table, and the neutral albedo is e_scatter over the sum before the
+     * factor 1/mp (which cancels). */
+    const double sf = mx_omexp_near((w[WT_CHI * WS] * theta) * OCO_LN10, damp, E, r1) * fn;
     double sm = w[WT_G1 * WS];
     sm = sm + w[WT_G2 * WS] * c[CR_E2];
     sm = sm + w[WT_G3 * WS] * c[CR_E3];
     sm = sm + w[WT_G4 * WS] * c[CR_E4];
     sm = sm + w[WT_G5 * WS] * c[CR_E5];
     sm = sm + w[WT_G6 * WS] * c[CR_E6];
-    double kHbf = (((OCO_AHBF * stim) * fn) * w[WT_WA3 * WS]) * (c[CR_UNSOLD] + sm);
     /* H- bound-free below 16000 A: the switch selects the wavelength's
      * coefficient (0 above), not the term, so no per-point select */
     const double sco = w[WT_HMBF_ON * WS] != 0.0 ? w[WT_SCOEFF * WS] : 0.0;
-    double kHmbf = ((c[CR_HMBF] * stim) * fn) * sco;
+    const double kbf = w[WT_HBFW * WS] * (c[CR_UNSOLD] + sm) + c[CR_HMBF] * sco;   /* (H bf + H- bf) / (stim fn) */
     double flam = ((w[WT_F1 * WS] + w[WT_F2 * WS] * c[CR_LGT]) + w[WT_F3 * WS] * c[CR_LGT2]) - 26.0;
-    double kHmff = c[CR_PHMFF] * mx_exp10(flam);
-    double kappa = (((kHbf + kHmbf) + kHmff) + c[CR_ES]) * OCO_MP1;
-    double rho = c[CR_RHO];
-    double alpha = kappa * rho;
+    const double s1 = sf * kbf + (c[CR_PHMFF] * mx_exp10(flam) + c[CR_ES]);   /* kappa mp before free-free */
+    const double rho = c[CR_RHO];
+    double alpha, kappa;
     if (c[CR_NEUTRAL] != 0.0) {
+        /* kappa mp += H ff; alpha = (s1 + s2) rho / mp (the double count of
+         * source/continuous_opacity.f90:194); albedo = e_scatter / s2 */
         double g_ff = 1.0 + w[WT_GFF * WS] * ((c[CR_U0] * w[WT_CHI1 * WS]) + 0.5);
-        double kHff = ((((stim * fn) * OCO_ALPHA0) * w[WT_WA3 * WS]) * g_ff) * c[CR_U1];
-        kappa = kappa + kHff * OCO_MP1;
-        alpha = alpha + kappa * rho;
+        const double s2 = s1 + ((sf * w[WT_HFFW * WS]) * g_ff) * c[CR_U1];
+        alpha = (s1 + s2) * (rho * OCO_MP1);
+        kappa = s2;
     } else {
         double inner = w[WT_NU9 * WS] * c[CR_U0];
         double gB = mx_log(mx_exp(5.960 - (OCO_SQRT3 * OCO_PI1) * mx_log(inner)) + OCO_EXP1);
         double lnQ = (c[CR_U1] - w[WT_L3NU * WS]) + mx_log(mx_omexp_near(w[WT_HNK * WS] * c[CR_U2], damp, E, r1));
         double kRho = (OCO_C_BREMS * mx_exp(lnQ)) * gB;
-        alpha = alpha + kRho;
-        kappa = kappa + alpha * c[CR_U3];
+        alpha = s1 * (rho * OCO_MP1) + kRho;
+        kappa = s1 + alpha * (c[CR_U3] * (1.0 / OCO_MP1));   /* kappa mp */
     }
-    p.albedo = mx_div(c[CR_ESM], kappa);
+    p.albedo = mx_div(c[CR_ES], kappa);
     p.alpha = alpha;
     p.alpha1 = mx_rcp(alpha);
     p.alpha2 = alpha * alpha;
