@@ -160,6 +160,7 @@ void oco_build_wtab(const std::vector<double>& waves, std::vector<double>& wt)
         w[WT_SFW5] = sfc * ((((wcm1 * wcm1) * wcm1) * wcm1) * wcm1);
         w[WT_HBFW] = OCO_AHBF * w[WT_WA3];
         w[WT_HFFW] = OCO_ALPHA0 * w[WT_WA3];
+        w[WT_F1M26] = w[WT_F1] - 26.0;
     }
 }
 

@@ -129,7 +129,8 @@ __device__ static inline oco_pt oco_point_mixed(const double* c, const double* w
      * table, and the neutral albedo is e_scatter over the sum before the
      * factor 1/mp (which cancels). */
     const double sf = mx_omexp_near((w[WT_CHI * WS] * theta) * OCO_LN10, damp, E, r1) * fn;
-    double sm = w[WT_G1 * WS];
+    /* Unsold tail + sum_n g(n) n^-3 exp(-chi_n/kT), from the cell's term */
+    double sm = c[CR_UNSOLD] + w[WT_G1 * WS];
     sm = sm + w[WT_G2 * WS] * c[CR_E2];
     sm = sm + w[WT_G3 * WS] * c[CR_E3];
     sm = sm + w[WT_G4 * WS] * c[CR_E4];
@@ -138,8 +139,8 @@ __device__ static inline oco_pt oco_point_mixed(const double* c, const double* w
     /* H- bound-free below 16000 A: the switch selects the wavelength's
      * coefficient (0 above), not the term, so no per-point select */
     const double sco = w[WT_HMBF_ON * WS] != 0.0 ? w[WT_SCOEFF * WS] : 0.0;
-    const double kbf = w[WT_HBFW * WS] * (c[CR_UNSOLD] + sm) + c[CR_HMBF] * sco;   /* (H bf + H- bf) / (stim fn) */
-    double flam = ((w[WT_F1 * WS] + w[WT_F2 * WS] * c[CR_LGT]) + w[WT_F3 * WS] * c[CR_LGT2]) - 26.0;
+    const double kbf = w[WT_HBFW * WS] * sm + c[CR_HMBF] * sco;   /* (H bf + H- bf) / (stim fn) */
+    double flam = (w[WT_F1M26 * WS] + w[WT_F2 * WS] * c[CR_LGT]) + w[WT_F3 * WS] * c[CR_LGT2];
     const double s1 = sf * kbf + (c[CR_PHMFF] * mx_exp10(flam) + c[CR_ES]);   /* kappa mp before free-free */
     const double rho = c[CR_RHO];
     double alpha, kappa;

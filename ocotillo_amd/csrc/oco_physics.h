@@ -77,10 +77,11 @@ enum {
     WT_L3NU,                 /* 3*log(nu) */
     WT_HNK,                  /* (h*nu)*k1 */
     WT_WCM1, WT_SFW5,        /* 1/wcm, source_function_const*w1cm5 */
-    WT_HBFW, WT_HFFW,        /* mixed mode only: A_Hbf*wa^3, alpha0*wa^3 */
+    WT_HBFW, WT_HFFW,        /* mixed mode only: A_Hbf*wa^3, alpha0*wa^3, */
+    WT_F1M26,                /* f_coeff(1) - 26 */
     WT_NFIELD
 };
-#define OCO_WT_STRIDE 22
+#define OCO_WT_STRIDE 23
 
 /* wavelength_independent_pillars (source/gas_state.f90:30-110 and
  * source/continuous_opacity.f90:143-163, 233-244, 262-296): fills rec. */
