@@ -639,12 +639,17 @@ def bench_batch(args):
         t1 = time.perf_counter()
         ms_per_step = (t1 - t0) * 1e3 / args.steps
         variant, nblocks, _ = b.tuning(code)
-        # mixed variants 3 + 3a + b: the wavefront-per-system kernel, columns of up to 512 cells in
-        # workgroups of (1, 2, 4)[a] wavefronts, longer ones of (1, 2, 4)[b]
-        kern = ("batch_fp64 (graph)" if mode == "fp64" else
-                "batch_mixed_wave, %d/%d wavefronts per workgroup (short/long columns) (graph)"
-                % ((1, 2, 4)[(variant - 3) // 3], (1, 2, 4)[(variant - 3) % 3]) if variant >= 3 else
-                "batch_mixed (graph)")
+        # mixed variants 3 + 9a + 3b + c: the wavefront-per-system kernel, columns of up to 512
+        # cells in workgroups of (1, 2, 4)[a] wavefronts, up to 768 of (1, 2, 4)[b], longer ones
+        # of (1, 2, 4)[c]
+        if mode == "fp64":
+            kern = "batch_fp64 (graph)"
+        elif variant >= 3:
+            v = variant - 3
+            kern = ("batch_mixed_wave, %d/%d/%d wavefronts per workgroup (columns up to 512/768/1088 "
+                    "cells) (graph)" % ((1, 2, 4)[v // 9], (1, 2, 4)[v // 3 % 3], (1, 2, 4)[v % 3]))
+        else:
+            kern = "batch_mixed (graph)"
         res[mode] = {"value": points / (ms_per_step * 1e-3), "ms_per_step": ms_per_step,
                      "roofline": roofline_of(kern, points, point_weights(mode, fion), float(np.mean(ms))),
                      "variant": variant, "workgroups": nblocks}

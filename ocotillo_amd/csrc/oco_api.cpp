@@ -863,6 +863,11 @@ int oco_athena_slab(oco_ctx* c, int nz, int nyloc, int nxloc, int nzl, int iz0, 
 /* Batched uneven columns                                                  */
 /* ---------------------------------------------------------------------- */
 
+/* column classes of the batched wavefront kernel (batch_class) and its
+ * variants: one of BATCH_WAVE_K's three workgroup sizes per class */
+#define OCO_BATCH_CLASSES 3
+#define OCO_BATCH_WAVE_VARIANTS 27
+
 struct oco_batch {
     oco_ctx* ctx = nullptr;
     int ncol = 0, max_nz = 0, ntiles = 0;
@@ -875,10 +880,10 @@ struct oco_batch {
      * the field-major records */
     DevBuf witems, wtickets, cells_soa;
     bool wave_ok = false;
-    /* two classes of columns for that kernel, each launched for its own
-     * longest column: nz <= OCO_BATCH_SHORT_NZ, and the rest */
-    int cls_max_nz[2] = {0, 0};
-    long wio[2][3] = {{0}}, wqo[2][3] = {{0}};
+    /* OCO_BATCH_CLASSES classes of columns for that kernel (batch_class),
+     * each launched for its own longest column */
+    int cls_max_nz[OCO_BATCH_CLASSES] = {0};
+    long wio[OCO_BATCH_CLASSES][3] = {{0}}, wqo[OCO_BATCH_CLASSES][3] = {{0}};
     /* per precision mode: autotuned kernel variant and grid, captured graph
      * (mixed variants 0..2: lane per system; 3 + 3 a + b: wavefront per system,
      * the short class in workgroups of 1, 2, 4 wavefronts for a = 0, 1, 2 and
@@ -937,19 +942,31 @@ static oco_batch_args batch_args(oco_batch* b)
 
 static const int BATCH_WAVE_K[3] = {1, 2, 4};
 #define OCO_BATCH_LANE_VARIANTS 3
-/* Columns up to this many cells form the short class of the batched
- * wavefront kernel: 9 rows per lane, and 3 workgroups of 4 wavefronts fit a
- * CU's LDS. */
-#define OCO_BATCH_SHORT_NZ 512
+/* The column classes of the batched wavefront kernel.  Each class is
+ * launched with LDS for its longest column, which sets how many wavefronts a
+ * CU holds: up to 512 cells, 12 (3 per SIMD, where the sweep stops gaining);
+ * up to 768, 8; above, 6 (for 2-wavefront workgroups).  Without the middle
+ * class, columns of 513-768 cells would run at 6. */
+static int batch_class(int nz) { return nz <= 512 ? 0 : nz <= 768 ? 1 : 2; }
 
-/* workgroups of the batched wavefront kernel in wavefront variant v (the
- * classes' sizes 3 v / 3 and v % 3 of BATCH_WAVE_K): the larger of the two
- * classes' resident grids; 0 if a class does not fit */
+/* the classes' indices into BATCH_WAVE_K in wavefront variant v (base 3,
+ * class 0 the most significant digit) */
+static void batch_wave_sizes(int v, int iv[OCO_BATCH_CLASSES])
+{
+    for (int cl = OCO_BATCH_CLASSES - 1; cl >= 0; cl--) {
+        iv[cl] = v % 3;
+        v /= 3;
+    }
+}
+
+/* workgroups of the batched wavefront kernel in wavefront variant v: the
+ * largest of the classes' resident grids; 0 if a class does not fit */
 static int batch_wave_blocks(const oco_batch* b, int v, int device)
 {
     int g = 0;
-    const int iv[2] = {v / 3, v % 3};
-    for (int cl = 0; cl < 2; cl++) {
+    int iv[OCO_BATCH_CLASSES];
+    batch_wave_sizes(v, iv);
+    for (int cl = 0; cl < OCO_BATCH_CLASSES; cl++) {
         if (b->cls_max_nz[cl] == 0) continue;
         const int gc = oco_batch_wave_grid(b->cls_max_nz[cl], BATCH_WAVE_K[iv[cl]], device);
         if (gc <= 0) return 0;
@@ -964,10 +981,11 @@ static hipError_t batch_enqueue(oco_batch* b, int mixed, int variant, int nblock
     oco_ctx* c = b->ctx;
     if (mixed && variant >= OCO_BATCH_LANE_VARIANTS) {
         (void)nblocks;   /* each class takes its own resident grid */
-        const int v = variant - OCO_BATCH_LANE_VARIANTS, iv[2] = {v / 3, v % 3};
+        int iv[OCO_BATCH_CLASSES];
+        batch_wave_sizes(variant - OCO_BATCH_LANE_VARIANTS, iv);
         oco_batch_args a = batch_args(b);
         hipError_t e = oco_launch_batch_wave_records(a, b->ncol, c->fully_ionized_T, c->stream);
-        for (int cl = 0; cl < 2 && e == hipSuccess; cl++) {
+        for (int cl = 0; cl < OCO_BATCH_CLASSES && e == hipSuccess; cl++) {
             if (b->cls_max_nz[cl] == 0) continue;
             const int i = iv[cl], K = BATCH_WAVE_K[i];
             a.max_nz = b->cls_max_nz[cl];
@@ -1052,16 +1070,16 @@ int oco_batch_create(oco_ctx* c, int ncol, const int* nz, const int* nw, const d
     b->wave_ok = c->mixed_sweep != 2 && oco_batch_wave_grid(b->max_nz, 1, c->device) > 0;
     if (b->wave_ok) {
         for (int i : order) {
-            const int cl = nz[i] <= OCO_BATCH_SHORT_NZ ? 0 : 1;
+            const int cl = batch_class(nz[i]);
             b->cls_max_nz[cl] = std::max(b->cls_max_nz[cl], nz[i]);
         }
-        for (int cl = 0; cl < 2; cl++) {
+        for (int cl = 0; cl < OCO_BATCH_CLASSES; cl++) {
             for (int i = 0; i < 3; i++) {
                 const int K = BATCH_WAVE_K[i];
                 std::vector<int> q[OCO_TICKET_QUEUES];
                 int p = 0;   /* position among the class's columns, longest first */
                 for (int col : order) {
-                    if ((nz[col] <= OCO_BATCH_SHORT_NZ ? 0 : 1) != cl) continue;
+                    if (batch_class(nz[col]) != cl) continue;
                     for (int w0 = 0; w0 < nw[col]; w0 += K) {
                         q[p % OCO_TICKET_QUEUES].push_back(col);
                         q[p % OCO_TICKET_QUEUES].push_back(w0);
@@ -1110,7 +1128,7 @@ int oco_batch_create(oco_ctx* c, int ncol, const int* nz, const int* nw, const d
     if (!st) st = ensure(c, b->fail, sizeof(unsigned long long));
     if (!st && b->wave_ok) {
         st = ensure(c, b->witems, witems.size() * sizeof(int));
-        if (!st) st = ensure(c, b->wtickets, 2 * OCO_TICKET_BYTES);
+        if (!st) st = ensure(c, b->wtickets, OCO_BATCH_CLASSES * OCO_TICKET_BYTES);
         if (!st) st = ensure(c, b->cells_soa, (size_t)nc * CR_NFIELD * sizeof(double));
     }
     if (!st) {
@@ -1188,7 +1206,7 @@ static int batch_prepare(oco_batch* b, int m)
             dev_cus = prop.multiProcessorCount;
         float best = 1e30f;
         int k = 0;
-        const int nvar = oco_batch_variants() + (m == OCO_MODE_MIXED && b->wave_ok ? 9 : 0);
+        const int nvar = oco_batch_variants() + (m == OCO_MODE_MIXED && b->wave_ok ? OCO_BATCH_WAVE_VARIANTS : 0);
         /* OCO_MIXED_SWEEP=wave: only the wavefront variants */
         const int v0 = m == OCO_MODE_MIXED && b->wave_ok && c->mixed_sweep == 1 ? OCO_BATCH_LANE_VARIANTS : 0;
         for (int v = v0; v < nvar; v++) {
@@ -1196,8 +1214,11 @@ static int batch_prepare(oco_batch* b, int m)
             /* wavefront variants: a class absent from the batch makes its
              * size irrelevant, so only the first of those variants is tried */
             if (wave) {
-                const int wv = v - OCO_BATCH_LANE_VARIANTS;
-                if ((b->cls_max_nz[0] == 0 && wv / 3 != 0) || (b->cls_max_nz[1] == 0 && wv % 3 != 0)) continue;
+                int iv[OCO_BATCH_CLASSES];
+                batch_wave_sizes(v - OCO_BATCH_LANE_VARIANTS, iv);
+                bool skip = false;
+                for (int cl = 0; cl < OCO_BATCH_CLASSES; cl++) skip = skip || (b->cls_max_nz[cl] == 0 && iv[cl] != 0);
+                if (skip) continue;
             }
             for (int wps = 2; wps <= (wave ? 2 : 4); wps++) {
                 /* wavefront variants: the occupancy grids (one try) */

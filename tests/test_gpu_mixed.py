@@ -255,12 +255,13 @@ def test_wave_groups_equal(nz, nw):
         assert np.array_equal(A, out[0][2]), g
 
 
-@pytest.mark.parametrize("kind,nz", [("lane", 96), ("wave", 96), ("wave", 600)])
+@pytest.mark.parametrize("kind,nz", [("lane", 96), ("wave", 96), ("wave", 600), ("wave", 900)])
 def test_batch_mixed_equals_block(kind, nz):
     """Batched (persistent, graph-replayed) mixed solve == the block mixed
     solve with the same kernel kind (OCO_MIXED_SWEEP restricts the batch's
-    autotune to it), bit for bit, on every replay; nz 96 and 600 put every
-    column in the batched wavefront kernel's short and long class."""
+    autotune to it), bit for bit, on every replay; nz 96, 600 and 900 put
+    every column in one of the batched wavefront kernel's three classes
+    (up to 512, 768 and 1088 cells)."""
     import ocotillo_amd as oco
     from ocotillo_amd.workloads import config2
     rho, T, dz, waves = config2(2, 3, nz, 80)
@@ -292,13 +293,13 @@ def test_batch_mixed_equals_block(kind, nz):
 
 def test_batch_mixed_wave_uneven_columns():
     """Batched mixed mode on the wavefront-per-system kernel, uneven columns
-    (nz 4..700, nw 1..140; the kernel is instantiated for the longest
-    column): every column equals that column solved as a block of its own by
+    (nz 4..1000 across the three column classes, nw 1..140; each class's
+    kernel is instantiated for its longest column): every column equals that column solved as a block of its own by
     the block wavefront sweep, bit for bit, on every graph replay, with the
     context's own eager work between replays."""
     import ocotillo_amd as oco
     from ocotillo_amd.workloads import config3
-    nz, nw, waves, dz, rho, T = config3(ncol=14, seed=78, nz_range=(4, 700), nw_range=(1, 140))
+    nz, nw, waves, dz, rho, T = config3(ncol=14, seed=78, nz_range=(4, 1000), nw_range=(1, 140))
     print("batch nz", nz.tolist(), "nw", nw.tolist(), flush=True)
     with sweep_context("wave") as c:
         c.set_physics(2e4, False, 0.4)
