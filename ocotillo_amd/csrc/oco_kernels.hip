@@ -1098,7 +1098,11 @@ __device__ static inline void rh_chunk_close(rh_chunk& c)
  * last row's D, and after each reduction step A = 0 for lanes < s, C = 0 for
  * lanes > 63 - s), and every row's A + C + H is positive and finite, so the
  * wrapped-around values are multiplied by zero.  Empty chunks are identity
- * rows.  G: this chunk's last unknown, GL the previous chunk's. */
+ * rows.  Each step, every lane divides its row by its own diagonal
+ * A + C + H and shares the quotients: the neighbour above needs a, h, r and
+ * the one below c, h, r (k1 A- = A a-, k1 H- = A h-, ...), so one reciprocal
+ * and six shuffled doubles per step.  G: this chunk's last unknown, GL the
+ * previous chunk's. */
 __device__ static inline void rh_reduced(const rh_chunk& c, bool rows, double& G, double& GL)
 {
 #pragma clang fp contract(fast)
@@ -1115,14 +1119,14 @@ __device__ static inline void rh_reduced(const rh_chunk& c, bool rows, double& G
 #pragma unroll
     for (int sft = 1; sft < OCO_WAVE; sft <<= 1) {
         const int up = ((lane - sft) & (OCO_WAVE - 1)) * 4, dn = ((lane + sft) & (OCO_WAVE - 1)) * 4;
-        const double Am = lane_get(Ar, up), Cm = lane_get(Cr, up), Hm = lane_get(Hr, up), Rm = lane_get(Rr, up);
-        const double Ap = lane_get(Ar, dn), Cp = lane_get(Cr, dn), Hp = lane_get(Hr, dn), Rp = lane_get(Rr, dn);
-        const double k1 = Ar * mx_rcp((Am + Cm) + Hm);
-        const double k2 = Cr * mx_rcp((Ap + Cp) + Hp);
-        Ar = k1 * Am;
-        Cr = k2 * Cp;
-        Hr = (Hr + k1 * Hm) + k2 * Hp;
-        Rr = (Rr + k1 * Rm) + k2 * Rp;
+        const double di = mx_rcp((Ar + Cr) + Hr);
+        const double an = Ar * di, cn = Cr * di, hn = Hr * di, rn = Rr * di;
+        const double am = lane_get(an, up), hm = lane_get(hn, up), rm = lane_get(rn, up);
+        const double cp = lane_get(cn, dn), hp = lane_get(hn, dn), rp = lane_get(rn, dn);
+        Hr = (Hr + Ar * hm) + Cr * hp;
+        Rr = (Rr + Ar * rm) + Cr * rp;
+        Ar = Ar * am;
+        Cr = Cr * cp;
     }
     G = Rr * mx_rcp((Ar + Cr) + Hr);
     GL = lane_get(G, up1);
