@@ -967,6 +967,37 @@ __device__ static inline double lane_get(double v, int addr)
     return __builtin_bit_cast(double, r);
 }
 
+/* v from the neighbouring lane without an LDS round trip: lane - 1 (0 in
+ * lane 0) and lane + 1 (0 in lane 63) by DPP whole-wavefront shifts */
+__device__ static inline double lane_up1(double v)
+{
+    const int2 w = __builtin_bit_cast(int2, v);
+    int2 r;
+    r.x = __builtin_amdgcn_mov_dpp(w.x, 0x138, 0xf, 0xf, true);   /* wave_shr:1, bound_ctrl */
+    r.y = __builtin_amdgcn_mov_dpp(w.y, 0x138, 0xf, 0xf, true);
+    return __builtin_bit_cast(double, r);
+}
+__device__ static inline double lane_dn1(double v)
+{
+    const int2 w = __builtin_bit_cast(int2, v);
+    int2 r;
+    r.x = __builtin_amdgcn_mov_dpp(w.x, 0x130, 0xf, 0xf, true);   /* wave_shl:1, bound_ctrl */
+    r.y = __builtin_amdgcn_mov_dpp(w.y, 0x130, 0xf, 0xf, true);
+    return __builtin_bit_cast(double, r);
+}
+
+/* v across the two halves of the wavefront (v_permlane32_swap): up holds
+ * lane - 32 in lanes 32..63, dn lane + 32 in lanes 0..31; the other half of
+ * each holds the lane's own v */
+__device__ static inline void lane_swap32(double v, double& up, double& dn)
+{
+    const int2 w = __builtin_bit_cast(int2, v);
+    const auto lo = __builtin_amdgcn_permlane32_swap((unsigned)w.x, (unsigned)w.x, false, false);
+    const auto hi = __builtin_amdgcn_permlane32_swap((unsigned)w.y, (unsigned)w.y, false, false);
+    up = __builtin_bit_cast(double, int2{(int)lo[0], (int)hi[0]});
+    dn = __builtin_bit_cast(double, int2{(int)lo[1], (int)hi[1]});
+}
+
 /* rows per lane of the wavefront sweep for columns of nz cells */
 __host__ __device__ static inline int wave_rows(int nz)
 {
@@ -1101,14 +1132,17 @@ __device__ static inline void rh_chunk_close(rh_chunk& c)
  * rows.  Each step, every lane divides its row by its own diagonal
  * A + C + H and shares the quotients: the neighbour above needs a, h, r and
  * the one below c, h, r (k1 A- = A a-, k1 H- = A h-, ...), so one reciprocal
- * and six shuffled doubles per step.  G: this chunk's last unknown, GL the
- * previous chunk's. */
+ * and six shuffled doubles per step.  Shifts by one lane are DPP moves and
+ * the step across the halves one v_permlane32_swap: where these give a lane
+ * 0 or its own value instead of the wrapped-around neighbour's, its
+ * coupling to that neighbour is exactly zero as well (before the last step,
+ * A = 0 in lanes 0..31 and C = 0 in lanes 32..63).  G: this chunk's last
+ * unknown, GL the previous chunk's. */
 __device__ static inline void rh_reduced(const rh_chunk& c, bool rows, double& G, double& GL)
 {
 #pragma clang fp contract(fast)
     const int lane = lane_id();
-    const int dn1 = ((lane + 1) & (OCO_WAVE - 1)) * 4, up1 = ((lane - 1) & (OCO_WAVE - 1)) * 4;
-    const double Gn = lane_get(c.pr, dn1), Dn = lane_get(c.sD, dn1), En = lane_get(c.sE, dn1);
+    const double Gn = lane_dn1(c.pr), Dn = lane_dn1(c.sD), En = lane_dn1(c.sE);
     double Ar = 0.0, Cr = 0.0, Hr = 1.0, Rr = 0.0;
     if (rows) {
         Ar = c.WL;
@@ -1121,15 +1155,27 @@ __device__ static inline void rh_reduced(const rh_chunk& c, bool rows, double& G
         const int up = ((lane - sft) & (OCO_WAVE - 1)) * 4, dn = ((lane + sft) & (OCO_WAVE - 1)) * 4;
         const double di = mx_rcp((Ar + Cr) + Hr);
         const double an = Ar * di, cn = Cr * di, hn = Hr * di, rn = Rr * di;
-        const double am = lane_get(an, up), hm = lane_get(hn, up), rm = lane_get(rn, up);
-        const double cp = lane_get(cn, dn), hp = lane_get(hn, dn), rp = lane_get(rn, dn);
+        double am, hm, rm, cp, hp, rp;
+        if (sft == 1) {
+            am = lane_up1(an); hm = lane_up1(hn); rm = lane_up1(rn);
+            cp = lane_dn1(cn); hp = lane_dn1(hn); rp = lane_dn1(rn);
+        } else if (sft == OCO_WAVE / 2) {
+            double x;
+            lane_swap32(an, am, x);
+            lane_swap32(cn, x, cp);
+            lane_swap32(hn, hm, hp);
+            lane_swap32(rn, rm, rp);
+        } else {
+            am = lane_get(an, up); hm = lane_get(hn, up); rm = lane_get(rn, up);
+            cp = lane_get(cn, dn); hp = lane_get(hn, dn); rp = lane_get(rn, dn);
+        }
         Hr = (Hr + Ar * hm) + Cr * hp;
         Rr = (Rr + Ar * rm) + Cr * rp;
         Ar = Ar * am;
         Cr = Cr * cp;
     }
     G = Rr * mx_rcp((Ar + Cr) + Hr);
-    GL = lane_get(G, up1);
+    GL = lane_up1(G);
 }
 
 /* x[z] from x[z+1] and the previous chunk's last unknown */
