@@ -1741,11 +1741,12 @@ __global__ __launch_bounds__(OCO_WAVE * OCO_SERVE_WG) OCO_SERVE_ATTR void serve_
                     }
                 }
             }
-            const int dn1 = ((lane + 1) & (OCO_WAVE - 1)) * 4;
 #pragma unroll
             for (int k = 0; k < K; k++) {
                 if (k < nk) {
-                    const double an = lane_get(a0[k], dn1);   /* alpha of cell z1, the next chunk's first */
+                    /* alpha of cell z1, the next chunk's first (read only where z1 < nz,
+                     * so never in lane 63, whose shifted-in value is 0) */
+                    const double an = lane_dn1(a0[k]);
                     if (z0 < z1) {   /* the chunk's last row */
                         double D, W, Z;
                         rh_chunk_row(st[k], z1 - 1, nz, z1 - 1 == z0, ap[k], op[k], Sp[k], z1 < nz ? an : 0.0, dz,
