@@ -1276,11 +1276,13 @@ __device__ static inline void solve_system_wave(const oco_sys& d, double* lds)
 #define OCO_WAVE_ATTR
 #endif
 
-/* Systems a workgroup takes per ticket of the wavefront sweep (measured on
- * MI355X with one wavefront per workgroup, 256 x 256 columns, nz = 512,
- * nw = 128: 1 56.4 ms, 2 55.8, 4 54.0, 8 53.2, 16 55.5). */
+/* Systems a workgroup takes per ticket of the wavefront sweep: a ticket is
+ * 32 / K groups of K consecutive wavelengths of one column.  Measured on
+ * MI355X with workgroups of 4 wavefronts, 256 x 256 columns, nz = 512,
+ * nw = 128: 8 systems 36.6-36.7 ms, 16 35.7-35.8, 32 35.4-35.5; on 64 x 64
+ * columns of 256 cells, 1.39 and 1.40 ms for 8 and 32. */
 #ifndef OCO_WAVE_TICKET
-#define OCO_WAVE_TICKET 8
+#define OCO_WAVE_TICKET 32
 #endif
 
 template <int MM, int K>
