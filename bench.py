@@ -55,9 +55,9 @@ WM_IONISED = 143 + 7 * (WM_DIV - 1) + WM_OMEXP + WM_EXP10 + WM_EXPM1 + 3 * WM_LO
 # rocprofv3 FETCH_SIZE / WRITE_SIZE passes over one launch, scaled by the
 # copy-kernel calibration (x2.0 / x1.0); workloads not listed report None
 SWEEP_TRAFFIC_BYTES = {
-    (256, 256, 512, 128, "sweep_mixed_wave"): 1.11250e11,
+    (256, 256, 512, 128, "sweep_mixed_wave"): 1.13610e11,
     (256, 256, 512, 128, "sweep_mixed"): 3.40102e11,
-    (256, 256, 512, 128, "sweep_fp64"): 3.04746e11,
+    (256, 256, 512, 128, "sweep_fp64"): 3.04761e11,
 }
 
 
