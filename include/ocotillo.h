@@ -37,10 +37,18 @@ enum {
     OCO_E_IO = 9
 };
 
-/* Precision modes of oco_solve_block*. */
+/* Precision modes of oco_solve_block*.  MIXED and ACCURATE solve the same
+ * rows (source/auxiliary.f90:147-197) in difference form, which keeps the
+ * row excess zeta that the reference's b = -((a + c) + zeta) rounds away in
+ * optically thin cells, and form the emergent flux V(1), V(nz) from the
+ * rows' interface differences instead of differences of U (:64-78, 96-109);
+ * both are held to the exact solution of the reference's discrete problem.
+ * Their columns take at most 1088 cells (batches and blocks; the server
+ * takes any nz up to its max_nz). */
 enum {
-    OCO_MODE_FP64 = 0,   /* drop-in: the reference's FP64 arithmetic, operation by operation */
-    OCO_MODE_MIXED = 1   /* FP32 opacity with exponent scaling, FP64 Saha and elimination */
+    OCO_MODE_FP64 = 0,     /* drop-in: the reference's FP64 arithmetic, operation by operation */
+    OCO_MODE_MIXED = 1,    /* FP32-core opacity with exponent scaling, FP64 Saha and elimination */
+    OCO_MODE_ACCURATE = 2  /* the drop-in's FP64 opacity (alpha bit for bit), difference-form solve */
 };
 
 typedef struct oco_ctx oco_ctx;

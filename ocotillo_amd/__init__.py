@@ -18,13 +18,13 @@ import os
 import numpy as np
 
 __all__ = ["OcoError", "Context", "Batch", "Server", "Diag", "AthenaUnits", "load_library", "LIB_PATH",
-           "MODE_FP64", "MODE_MIXED", "EXPORTS"]
+           "MODE_FP64", "MODE_MIXED", "MODE_ACCURATE", "EXPORTS"]
 
 PKG = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(PKG, "libocotillo.so")
 
 OK, E_PIVOT0_STAGE1, E_PIVOT0_STAGE2, E_ARG, E_OOM, E_HIP, E_STATE, E_NODEVICE, E_BUSY, E_IO = range(10)
-MODE_FP64, MODE_MIXED = 0, 1
+MODE_FP64, MODE_MIXED, MODE_ACCURATE = 0, 1, 2
 
 # every symbol include/ocotillo.h declares
 EXPORTS = ["oco_create", "oco_destroy", "oco_set_wavelengths", "oco_set_physics", "oco_solve_block",

@@ -41,8 +41,8 @@ struct oco_ctx {
     double fully_ionized_T = 2e4;
     int lgrey = 0;
     double sigma_grey = 0.4;
-    int mixed_sweep = 0;  /* mixed mode: 0 by column length, 1 wavefront per system, 2 lane per system */
-    int wave_group = 0;   /* wavefronts per workgroup of the wavefront sweep: 1, 2, 4, 0 = default */
+    int wave_lanes = 0;   /* lanes per system of the difference-form sweep: 8, 16, 32, 64, 0 = by nz */
+    int wave_group = 0;   /* wavefronts per workgroup of the difference-form sweep: 1, 2, 4, 0 = default */
     DevBuf wtab, rho, T, cells, scratch, U, V, A, fail, canary, raw, tg, tickets;
     /* block outputs to files: copy stream, per-slot events, pinned staging */
     hipStream_t copy_stream = nullptr;
@@ -203,13 +203,11 @@ int oco_create(int device, size_t hbm_budget, oco_ctx** out)
     if (!c) return OCO_E_OOM;
     c->device = device;
     c->budget = hbm_budget;
-    /* OCO_MIXED_SWEEP=wave|lane forces one of the two mixed-mode sweeps (both
-     * give results within the mixed mode's stated bounds); default: by
-     * column length */
-    if (const char* e = getenv("OCO_MIXED_SWEEP"))
-        c->mixed_sweep = strcmp(e, "wave") == 0 ? 1 : strcmp(e, "lane") == 0 ? 2 : 0;
-    /* OCO_WAVE_GROUP=1|2|4: wavefronts per workgroup of the wavefront sweep
-     * (same results; default: the library's measured choice) */
+    /* OCO_WAVE_LANES=8|16|32|64: lanes per system of the difference-form
+     * sweep (results within the same bounds; default: by column length) */
+    if (const char* e = getenv("OCO_WAVE_LANES")) c->wave_lanes = atoi(e);
+    /* OCO_WAVE_GROUP=1|2|4: wavefronts per workgroup of the difference-form
+     * sweep (the same bits; default: the library's measured choice) */
     if (const char* e = getenv("OCO_WAVE_GROUP")) c->wave_group = atoi(e);
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
         delete c;
@@ -279,26 +277,35 @@ int oco_set_physics(oco_ctx* c, double fully_ionized_T, int lgrey, double sigma_
 
 /* Kernel and grid for sweeping nw_run wavelengths of ncol columns. */
 struct SweepPlan {
-    int wave = 0, nblocks = 0;
-    int wg = 1;   /* lane sweeps: wavefronts per workgroup (each with its own scratch slot) */
+    int wave = 0, nblocks = 0;   /* wave: wavefronts per workgroup of the difference-form sweep, 0 for the drop-in */
+    int lanes = 64, pk = 0;      /* difference-form sweep: lanes per system, point kind */
+    int wg = 1;                  /* drop-in sweep: wavefronts per workgroup (each with its own scratch slot) */
+    int ok = 1;                  /* 0: the mode cannot take columns of nz cells */
 };
 
-static SweepPlan sweep_plan(const oco_ctx* c, int nz, long ncol, int nw_run, int mixed)
+static bool mode_ok(int mode) { return mode == OCO_MODE_FP64 || mode == OCO_MODE_MIXED || mode == OCO_MODE_ACCURATE; }
+
+static SweepPlan sweep_plan(const oco_ctx* c, int nz, long ncol, int nw_run, int mode)
 {
     SweepPlan p;
     const long nsys = ncol * nw_run;
     const long ntiles = (nsys + 63) / 64;
-    /* mixed, non-grey: one wavefront per system when its LDS fits */
-    p.wave = mixed && !c->lgrey && c->mixed_sweep != 2 && oco_wave_fits(nz, c->device) &&
-             (c->mixed_sweep == 1 || oco_wave_preferred(nz));
-    if (p.wave) {
-        p.wave = oco_wave_group(nz, c->wave_group, c->device);
-        const long ngroups = ncol * ((nw_run + p.wave - 1) / p.wave);
-        const int grid = oco_wave_grid(nz, p.wave, c->device);
+    if (mode != OCO_MODE_FP64) {
+        /* difference form: grey points from rho and T in both modes */
+        p.pk = c->lgrey ? OCO_PK_GREY : mode == OCO_MODE_MIXED ? OCO_PK_MIXED : OCO_PK_FP64;
+        p.lanes = oco_wave_lanes(nz, c->wave_lanes);
+        if (!oco_wave_fits(nz, p.lanes, p.pk, c->device)) {
+            p.ok = 0;
+            return p;
+        }
+        p.wave = p.pk == OCO_PK_GREY ? 1 : oco_wave_group(nz, p.lanes, p.pk, c->wave_group, c->device);
+        const int S = p.wave * (64 / p.lanes);
+        const long ngroups = ncol * ((nw_run + S - 1) / S);
+        const int grid = oco_wave_grid(nz, p.wave, p.lanes, p.pk, c->device);
         p.nblocks = (int)(ngroups < grid ? ngroups : grid);
     } else {
-        const int grid = oco_sweep_grid(c->lgrey, mixed, (nw_run % 64) == 0, c->device);
-        p.wg = oco_sweep_wg(mixed && !c->lgrey);
+        const int grid = oco_sweep_grid(c->lgrey, (nw_run % 64) == 0, c->device);
+        p.wg = oco_sweep_wg();
         const long nb = (ntiles + p.wg - 1) / p.wg;
         p.nblocks = (int)(nb < grid ? nb : grid);
     }
@@ -390,14 +397,14 @@ int oco_solve_block_device(oco_ctx* c, int nz, int nyloc, int nxloc, double dz, 
 {
     if (!c) return OCO_E_ARG;
     if (nz < 4 || nyloc < 1 || nxloc < 1 || !d_rho || !d_T || !d_U || !d_V || !d_A) return OCO_E_ARG;
-    if (mode != OCO_MODE_FP64 && mode != OCO_MODE_MIXED) return OCO_E_ARG;
+    if (!mode_ok(mode)) return OCO_E_ARG;
     if (c->nw <= 0) return OCO_E_STATE;
     if (c->lgrey && c->nw != 1) return OCO_E_ARG;   /* source/flux_feautrier.f90:84-87 */
     HIPCHK(hipSetDevice(c->device));
     const long ncol = (long)nxloc * nyloc;
     const long ncells = ncol * nz;
-    const int mixed = mode == OCO_MODE_MIXED;
-    const SweepPlan p = sweep_plan(c, nz, ncol, c->nw, mixed);
+    const SweepPlan p = sweep_plan(c, nz, ncol, c->nw, mode);
+    if (!p.ok) return OCO_E_ARG;
     int st;
     if ((st = sweep_buffers(c, p, nz, ncells))) return st;
 
@@ -414,7 +421,7 @@ int oco_solve_block_device(oco_ctx* c, int nz, int nyloc, int nxloc, double dz, 
         HIPCHK(oco_launch_cells(d_rho, d_T, ncells, c->fully_ionized_T, (double*)c->cells.p, p.wave ? nz : 0,
                                 c->stream));
     HIPCHK(hipEventRecord(c->ev[1], c->stream));
-    HIPCHK(oco_launch_sweep(a, c->lgrey, mixed, p.wave, p.nblocks, c->stream));
+    HIPCHK(oco_launch_sweep(a, c->lgrey, p.wave, p.lanes, p.pk, p.nblocks, c->stream));
     HIPCHK(hipEventRecord(c->ev[2], c->stream));
 
     unsigned long long key = none;
@@ -487,7 +494,7 @@ int oco_opacity_points(oco_ctx* c, int npts, const double* rho, const double* T,
                        double* albedo, double* S)
 {
     if (!c || npts < 1 || !rho || !T || !alpha) return OCO_E_ARG;
-    if (mode != OCO_MODE_FP64 && mode != OCO_MODE_MIXED) return OCO_E_ARG;
+    if (!mode_ok(mode)) return OCO_E_ARG;   /* OCO_MODE_ACCURATE: the FP64 opacity */
     if (c->nw <= 0) return OCO_E_STATE;
     for (int i = 0; i < npts; i++)
         if (!(rho[i] > 0.0) || !(T[i] > 0.0)) return OCO_E_ARG;
@@ -590,11 +597,11 @@ int oco_solve_block_files(oco_ctx* c, int nz, int nyloc, int nxloc, double dz, d
     const auto t_start = std::chrono::steady_clock::now();
     if (!c || !rho3d || !T3d || !dir || !snapshot) return OCO_E_ARG;
     if (nz < 4 || nyloc < 1 || nxloc < 1) return OCO_E_ARG;
-    if (mode != OCO_MODE_FP64 && mode != OCO_MODE_MIXED) return OCO_E_ARG;
+    if (!mode_ok(mode)) return OCO_E_ARG;
     if (c->nw <= 0) return OCO_E_STATE;
     if (c->lgrey && c->nw != 1) return OCO_E_ARG;   /* source/flux_feautrier.f90:84-87 */
     HIPCHK(hipSetDevice(c->device));
-    const int nw = c->nw, mz = nz + 6, mixed = mode == OCO_MODE_MIXED;
+    const int nw = c->nw, mz = nz + 6;
     const long ncol = (long)nxloc * nyloc;
     const size_t ncells = (size_t)ncol * nz;
     const size_t bU = (size_t)ncol * mz * sizeof(double), bV = (size_t)ncol * nz * sizeof(double);
@@ -603,8 +610,9 @@ int oco_solve_block_files(oco_ctx* c, int nz, int nyloc, int nxloc, double dz, d
     const int nwc = (int)std::min<size_t>((size_t)nw, std::max<size_t>(1, stage_bytes / per_w));
     const int nch = (nw + nwc - 1) / nwc;
     const int nlast = nw - (nch - 1) * nwc;
-    const SweepPlan p_full = sweep_plan(c, nz, ncol, nwc, mixed);
-    const SweepPlan p_last = sweep_plan(c, nz, ncol, nlast, mixed);
+    const SweepPlan p_full = sweep_plan(c, nz, ncol, nwc, mode);
+    const SweepPlan p_last = sweep_plan(c, nz, ncol, nlast, mode);
+    if (!p_full.ok || !p_last.ok) return OCO_E_ARG;
     SweepPlan p_max = p_full;
     p_max.nblocks = std::max(p_full.nblocks, p_last.nblocks);
     const size_t slot_bytes = (size_t)nwc * per_w;
@@ -730,7 +738,7 @@ int oco_solve_block_files(oco_ctx* c, int nz, int nyloc, int nxloc, double dz, d
         const SweepPlan& pk = n == nwc ? p_full : p_last;
         char* sb = pin + slot * slot_bytes;
         he = hipMemcpyAsync(c->fail.p, none, sizeof(*none), hipMemcpyHostToDevice, c->stream);
-        if (he == hipSuccess) he = oco_launch_sweep(a, c->lgrey, mixed, pk.wave, pk.nblocks, c->stream);
+        if (he == hipSuccess) he = oco_launch_sweep(a, c->lgrey, pk.wave, pk.lanes, pk.pk, pk.nblocks, c->stream);
         if (he == hipSuccess)
             he = hipMemcpyAsync(keys + k, c->fail.p, sizeof(*keys), hipMemcpyDeviceToHost, c->stream);
         if (he == hipSuccess && k == 0)
@@ -863,10 +871,9 @@ int oco_athena_slab(oco_ctx* c, int nz, int nyloc, int nxloc, int nzl, int iz0, 
 /* Batched uneven columns                                                  */
 /* ---------------------------------------------------------------------- */
 
-/* column classes of the batched wavefront kernel (batch_class) and its
- * variants: one of BATCH_WAVE_K's three workgroup sizes per class */
-#define OCO_BATCH_CLASSES 3
-#define OCO_BATCH_WAVE_VARIANTS 27
+/* column classes of the batched difference-form kernel (batch_class); each
+ * class's workgroup size (one of BATCH_WAVE_K's three) is tuned on its own */
+#define OCO_BATCH_CLASSES 6
 
 struct oco_batch {
     oco_ctx* ctx = nullptr;
@@ -881,17 +888,21 @@ struct oco_batch {
     DevBuf witems, wtickets, cells_soa;
     bool wave_ok = false;
     /* OCO_BATCH_CLASSES classes of columns for that kernel (batch_class),
-     * each launched for its own longest column */
+     * each launched for its own longest column, and each class's tuned index
+     * into BATCH_WAVE_K */
     int cls_max_nz[OCO_BATCH_CLASSES] = {0};
     long wio[OCO_BATCH_CLASSES][3] = {{0}}, wqo[OCO_BATCH_CLASSES][3] = {{0}};
+    int cls_k[OCO_BATCH_CLASSES] = {0};
     /* per precision mode: autotuned kernel variant and grid, captured graph
-     * (mixed variants 0..2: lane per system; 3 + 3 a + b: wavefront per system,
-     * the short class in workgroups of 1, 2, 4 wavefronts for a = 0, 1, 2 and
-     * the long class for b) */
-    int nblocks[2] = {0, 0}, variant[2] = {1, 1}, tuned[2] = {0, 0};
-    float tune_ms[2][16] = {{0}};
+     * and the context's fully_ionized_T it was captured with (FP64 variants
+     * 0..2: lane per system with 1, 2, 4 back-substitution rows per chunk;
+     * mixed: the classes' indices into BATCH_WAVE_K as base-3 digits, class 0
+     * the most significant) */
+    int nblocks[2] = {0, 0}, variant[2] = {1, 0}, tuned[2] = {0, 0};
+    float tune_ms[2][32] = {{0}};
     hipGraph_t graph[2] = {nullptr, nullptr};
     hipGraphExec_t exec[2] = {nullptr, nullptr};
+    double graph_fiT[2] = {0.0, 0.0};
 };
 
 static void batch_release(oco_batch* b)
@@ -941,65 +952,53 @@ static oco_batch_args batch_args(oco_batch* b)
 }
 
 static const int BATCH_WAVE_K[3] = {1, 2, 4};
-#define OCO_BATCH_LANE_VARIANTS 3
-/* The column classes of the batched wavefront kernel.  Each class is
- * launched with LDS for its longest column, which sets how many wavefronts a
- * CU holds: up to 512 cells, 12 (3 per SIMD, where the sweep stops gaining);
- * up to 768, 8; above, 6 (for 2-wavefront workgroups).  Without the middle
- * class, columns of 513-768 cells would run at 6. */
-static int batch_class(int nz) { return nz <= 512 ? 0 : nz <= 768 ? 1 : 2; }
-
-/* the classes' indices into BATCH_WAVE_K in wavefront variant v (base 3,
- * class 0 the most significant digit) */
-static void batch_wave_sizes(int v, int iv[OCO_BATCH_CLASSES])
+/* The column classes of the batched difference-form kernel, each launched
+ * with LDS for its longest column and its own lanes per system: up to 72,
+ * 144 and 288 cells on 8, 16 and 32 lanes (at most 9 rows per lane, as the
+ * block sweep chooses), then 64 lanes up to 512, 768 and 1088 cells.  The LDS
+ * of a class's longest column sets how many wavefronts a CU holds, so the
+ * 64-lane columns are split three ways (at 512 cells, 12 wavefronts, 3 per
+ * SIMD, where the sweep stops gaining; without the 768 class, columns of
+ * 513-768 cells would run at 6). */
+static const int BATCH_CLASS_NZ[OCO_BATCH_CLASSES] = {72, 144, 288, 512, 768, 1088};
+static const int BATCH_CLASS_LANES[OCO_BATCH_CLASSES] = {8, 16, 32, 64, 64, 64};
+static int batch_class(int nz)
 {
-    for (int cl = OCO_BATCH_CLASSES - 1; cl >= 0; cl--) {
-        iv[cl] = v % 3;
-        v /= 3;
-    }
+    int cl = 0;
+    while (cl < OCO_BATCH_CLASSES - 1 && nz > BATCH_CLASS_NZ[cl]) cl++;
+    return cl;
 }
 
-/* workgroups of the batched wavefront kernel in wavefront variant v: the
- * largest of the classes' resident grids; 0 if a class does not fit */
-static int batch_wave_blocks(const oco_batch* b, int v, int device)
+/* enqueue the sweep of class cl with workgroups of BATCH_WAVE_K[i] */
+static hipError_t batch_class_enqueue(oco_batch* b, int cl, int i)
 {
-    int g = 0;
-    int iv[OCO_BATCH_CLASSES];
-    batch_wave_sizes(v, iv);
-    for (int cl = 0; cl < OCO_BATCH_CLASSES; cl++) {
-        if (b->cls_max_nz[cl] == 0) continue;
-        const int gc = oco_batch_wave_grid(b->cls_max_nz[cl], BATCH_WAVE_K[iv[cl]], device);
-        if (gc <= 0) return 0;
-        g = std::max(g, gc);
-    }
-    return g;
+    oco_ctx* c = b->ctx;
+    oco_batch_args a = batch_args(b);
+    const int K = BATCH_WAVE_K[i], L = BATCH_CLASS_LANES[cl];
+    a.max_nz = b->cls_max_nz[cl];
+    a.witems = (const int*)b->witems.p + b->wio[cl][i];
+    a.wq_off = (const int*)b->witems.p + b->wqo[cl][i];
+    a.wtickets = (unsigned long long*)b->wtickets.p + cl * OCO_TICKET_QUEUES * OCO_TICKET_STRIDE;
+    return oco_launch_batch_wave(a, K, L, oco_batch_wave_grid(a.max_nz, K, L, c->device), c->stream);
 }
 
-/* enqueue one full batch solve with the given kernel variant / grid */
+/* enqueue one full batch solve with the given kernel variant / grid (mixed:
+ * each class with its tuned workgroup size) */
 static hipError_t batch_enqueue(oco_batch* b, int mixed, int variant, int nblocks)
 {
     oco_ctx* c = b->ctx;
-    if (mixed && variant >= OCO_BATCH_LANE_VARIANTS) {
+    if (mixed) {
+        (void)variant;
         (void)nblocks;   /* each class takes its own resident grid */
-        int iv[OCO_BATCH_CLASSES];
-        batch_wave_sizes(variant - OCO_BATCH_LANE_VARIANTS, iv);
-        oco_batch_args a = batch_args(b);
-        hipError_t e = oco_launch_batch_wave_records(a, b->ncol, c->fully_ionized_T, c->stream);
-        for (int cl = 0; cl < OCO_BATCH_CLASSES && e == hipSuccess; cl++) {
-            if (b->cls_max_nz[cl] == 0) continue;
-            const int i = iv[cl], K = BATCH_WAVE_K[i];
-            a.max_nz = b->cls_max_nz[cl];
-            a.witems = (const int*)b->witems.p + b->wio[cl][i];
-            a.wq_off = (const int*)b->witems.p + b->wqo[cl][i];
-            a.wtickets = (unsigned long long*)b->wtickets.p + cl * OCO_TICKET_QUEUES * OCO_TICKET_STRIDE;
-            e = oco_launch_batch_wave(a, K, oco_batch_wave_grid(a.max_nz, K, c->device), c->stream);
-        }
+        hipError_t e = oco_launch_batch_wave_records(batch_args(b), b->ncol, c->fully_ionized_T, c->stream);
+        for (int cl = 0; cl < OCO_BATCH_CLASSES && e == hipSuccess; cl++)
+            if (b->cls_max_nz[cl] > 0) e = batch_class_enqueue(b, cl, b->cls_k[cl]);
         return e;
     }
     hipError_t e = oco_launch_cells((const double*)b->rho.p, (const double*)b->T.p, b->ncells, c->fully_ionized_T,
                                     (double*)b->cells.p, 0, c->stream);
     if (e != hipSuccess) return e;
-    return oco_launch_batch(batch_args(b), mixed, variant, nblocks, c->stream);
+    return oco_launch_batch(batch_args(b), variant, nblocks, c->stream);
 }
 
 int oco_batch_create(oco_ctx* c, int ncol, const int* nz, const int* nw, const double* waves, const double* dz,
@@ -1063,11 +1062,13 @@ int oco_batch_create(oco_ctx* c, int ncol, const int* nz, const int* nw, const d
         oco_build_wtab(g, one);
         std::copy(one.begin(), one.end(), wt.begin() + woff[i] * OCO_WT_STRIDE);
     }
-    /* mixed mode, wavefront per system: for workgroups of K wavefronts, items
-     * (column, first of K wavelengths); column p of the longest-first order
-     * goes to queue p mod 8, so each XCD's L2 holds its own columns' records */
+    /* mixed mode, difference form: for workgroups of K wavefronts, items
+     * (column, first of the S = K 64 / L wavelengths of a workgroup); column
+     * p of the longest-first order goes to queue p mod 8, so each XCD's L2
+     * holds its own columns' records */
     std::vector<int> witems;
-    b->wave_ok = c->mixed_sweep != 2 && oco_batch_wave_grid(b->max_nz, 1, c->device) > 0;
+    b->wave_ok = b->max_nz <= BATCH_CLASS_NZ[OCO_BATCH_CLASSES - 1] &&
+                 oco_batch_wave_grid(b->max_nz, 1, BATCH_CLASS_LANES[batch_class(b->max_nz)], c->device) > 0;
     if (b->wave_ok) {
         for (int i : order) {
             const int cl = batch_class(nz[i]);
@@ -1075,7 +1076,7 @@ int oco_batch_create(oco_ctx* c, int ncol, const int* nz, const int* nw, const d
         }
         for (int cl = 0; cl < OCO_BATCH_CLASSES; cl++) {
             for (int i = 0; i < 3; i++) {
-                const int K = BATCH_WAVE_K[i];
+                const int K = BATCH_WAVE_K[i] * (64 / BATCH_CLASS_LANES[cl]);   /* wavelengths per item */
                 std::vector<int> q[OCO_TICKET_QUEUES];
                 int p = 0;   /* position among the class's columns, longest first */
                 for (int col : order) {
@@ -1194,52 +1195,85 @@ int oco_batch_tuning(oco_batch* b, int mode, int* variant, int* nblocks, int* tu
     return OCO_OK;
 }
 
-/* first run in a mode: time every kernel variant x grid size once and keep
- * the best; then capture the whole solve (reset, cells, sweep) as one hipGraph */
+/* First run in a mode: time every kernel variant x grid size three times
+ * and keep the one with the best median; then capture the whole solve
+ * (reset, cells, sweep) as one hipGraph.  The kernels take fully_ionized_T
+ * by value, so a graph captured under another value of the context's is
+ * captured again. */
 static int batch_prepare(oco_batch* b, int m)
 {
     oco_ctx* c = b->ctx;
+    if (m == OCO_MODE_MIXED && !b->wave_ok) return OCO_E_ARG;
     if (!b->tuned[m]) {
         int dev_cus = 256;
         hipDeviceProp_t prop;
         if (hipGetDeviceProperties(&prop, c->device) == hipSuccess && prop.multiProcessorCount > 0)
             dev_cus = prop.multiProcessorCount;
-        float best = 1e30f;
         int k = 0;
-        const int nvar = oco_batch_variants() + (m == OCO_MODE_MIXED && b->wave_ok ? OCO_BATCH_WAVE_VARIANTS : 0);
-        /* OCO_MIXED_SWEEP=wave: only the wavefront variants */
-        const int v0 = m == OCO_MODE_MIXED && b->wave_ok && c->mixed_sweep == 1 ? OCO_BATCH_LANE_VARIANTS : 0;
-        for (int v = v0; v < nvar; v++) {
-            const bool wave = v >= OCO_BATCH_LANE_VARIANTS;
-            /* wavefront variants: a class absent from the batch makes its
-             * size irrelevant, so only the first of those variants is tried */
-            if (wave) {
-                int iv[OCO_BATCH_CLASSES];
-                batch_wave_sizes(v - OCO_BATCH_LANE_VARIANTS, iv);
-                bool skip = false;
-                for (int cl = 0; cl < OCO_BATCH_CLASSES; cl++) skip = skip || (b->cls_max_nz[cl] == 0 && iv[cl] != 0);
-                if (skip) continue;
-            }
-            for (int wps = 2; wps <= (wave ? 2 : 4); wps++) {
-                /* wavefront variants: the occupancy grids (one try) */
-                int nb = wave ? batch_wave_blocks(b, v - OCO_BATCH_LANE_VARIANTS, c->device)
-                              : std::min(b->ntiles, dev_cus * 4 * wps);
-                if (nb <= 0) continue;
+        /* the median of three timings of a launch sequence */
+        auto time3 = [&](auto&& enqueue, float& ms) -> int {
+            float t3[3];
+            for (int r = 0; r < 3; r++) {
                 HIPCHK(hipEventRecord(c->ev[0], c->stream));
-                HIPCHK(batch_enqueue(b, m, v, nb));
+                HIPCHK(enqueue());
                 HIPCHK(hipEventRecord(c->ev[1], c->stream));
                 HIPCHK(hipEventSynchronize(c->ev[1]));
-                float ms = 0.f;
-                HIPCHK(hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
-                if (k < 16) b->tune_ms[m][k++] = ms;
-                if (ms < best) {
-                    best = ms;
-                    b->variant[m] = v;
-                    b->nblocks[m] = nb;
+                t3[r] = 0.f;
+                HIPCHK(hipEventElapsedTime(&t3[r], c->ev[0], c->ev[1]));
+            }
+            std::sort(t3, t3 + 3);
+            ms = t3[1];
+            if (k < 32) b->tune_ms[m][k++] = ms;
+            return OCO_OK;
+        };
+        if (m == OCO_MODE_MIXED) {
+            /* each class on its own: its launch with each workgroup size
+             * (the records once, before) */
+            HIPCHK(oco_launch_batch_wave_records(batch_args(b), b->ncol, c->fully_ionized_T, c->stream));
+            int code = 0;
+            for (int cl = 0; cl < OCO_BATCH_CLASSES; cl++) {
+                float best = 1e30f;
+                b->cls_k[cl] = 0;
+                if (b->cls_max_nz[cl] > 0) {
+                    for (int i = 0; i < 3; i++) {
+                        const int L = BATCH_CLASS_LANES[cl];
+                        if (oco_batch_wave_grid(b->cls_max_nz[cl], BATCH_WAVE_K[i], L, c->device) <= 0) continue;
+                        float ms = 0.f;
+                        int st = time3([&] { return batch_class_enqueue(b, cl, i); }, ms);
+                        if (st) return st;
+                        if (ms < best) {
+                            best = ms;
+                            b->cls_k[cl] = i;
+                        }
+                    }
+                }
+                code = 3 * code + b->cls_k[cl];
+            }
+            b->variant[m] = code;
+            b->nblocks[m] = 0;
+        } else {
+            float best = 1e30f;
+            for (int v = 0; v < oco_batch_variants(); v++) {
+                for (int wps = 2; wps <= 4; wps++) {
+                    const int nb = std::min(b->ntiles, dev_cus * 4 * wps);
+                    float ms = 0.f;
+                    int st = time3([&] { return batch_enqueue(b, m, v, nb); }, ms);
+                    if (st) return st;
+                    if (ms < best) {
+                        best = ms;
+                        b->variant[m] = v;
+                        b->nblocks[m] = nb;
+                    }
                 }
             }
         }
         b->tuned[m] = 1;
+    }
+    if (b->exec[m] && b->graph_fiT[m] != c->fully_ionized_T) {
+        (void)hipGraphExecDestroy(b->exec[m]);
+        (void)hipGraphDestroy(b->graph[m]);
+        b->exec[m] = nullptr;
+        b->graph[m] = nullptr;
     }
     if (!b->exec[m]) {
         HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
@@ -1252,6 +1286,7 @@ static int batch_prepare(oco_batch* b, int m)
         }
         b->graph[m] = g;
         HIPCHK(hipGraphInstantiate(&b->exec[m], g, nullptr, nullptr, 0));
+        b->graph_fiT[m] = c->fully_ionized_T;
     }
     return OCO_OK;
 }

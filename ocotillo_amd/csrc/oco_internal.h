@@ -134,36 +134,41 @@ int oco_serve_win_rows(int max_nz);
 int oco_serve_grid(int mixed, int max_nz, int device);
 hipError_t oco_launch_fail_reset(unsigned long long* fail, int n, hipStream_t stream);
 /* mixed != 0 selects the mixed-precision kernels (non-grey only). */
-hipError_t oco_launch_batch(const oco_batch_args& a, int mixed, int variant, int nblocks, hipStream_t stream);
-/* batched mixed mode, wavefront per system: the per-column field-major
- * records (and the failure key), then the sweep of one class of columns (its
- * items, tickets and longest column in a) with k (1, 2, 4) wavefronts per
- * workgroup */
+/* the FP64 lane-per-system batch kernel, variant 0..2 */
+hipError_t oco_launch_batch(const oco_batch_args& a, int variant, int nblocks, hipStream_t stream);
+/* batched mixed mode, difference form: the per-column field-major records
+ * (and the failure key), then the sweep of one class of columns (its items,
+ * tickets and longest column in a) with k (1, 2, 4) wavefronts per workgroup
+ * and `lanes` (8, 16, 32, 64) lanes per system */
 hipError_t oco_launch_batch_wave_records(const oco_batch_args& a, int ncol, double fully_ionized_T,
                                          hipStream_t stream);
-hipError_t oco_launch_batch_wave(const oco_batch_args& a, int k, int nblocks, hipStream_t stream);
+hipError_t oco_launch_batch_wave(const oco_batch_args& a, int k, int lanes, int nblocks, hipStream_t stream);
 /* resident workgroups of that sweep for columns of at most max_nz cells; 0 if
  * such columns do not fit it */
-int oco_batch_wave_grid(int max_nz, int k, int device);
+int oco_batch_wave_grid(int max_nz, int k, int lanes, int device);
 int oco_batch_variants(void);
 /* soa_nz > 0: field-major records per column of soa_nz cells (field k of
  * cell z of column c at (c*CR_NFIELD + k)*soa_nz + z); 0: cell-major */
 hipError_t oco_launch_cells(const double* rho, const double* T, long ncells, double fully_ionized_T,
                             double* cells, int soa_nz, hipStream_t stream);
-/* wave != 0 (mixed, non-grey): the wavefront-per-system sweep with `wave`
- * (1, 2 or 4) wavefronts per workgroup, which reads field-major cell records
- * and needs no scratch */
-hipError_t oco_launch_sweep(const oco_sweep_args& a, int lgrey, int mixed, int wave, int nblocks, hipStream_t stream);
-int oco_wave_fits(int nz, int device);
-/* 1 when the wavefront sweep is the faster mixed sweep for nz (measured) */
-int oco_wave_preferred(int nz);
-/* wavefronts per workgroup of the wavefront sweep (want: 1, 2, 4, or 0 for the default) */
-int oco_wave_group(int nz, int want, int device);
-int oco_wave_grid(int nz, int k, int device);
+/* wave == 0: the FP64 drop-in's lane-per-system sweep; wave = k (1, 2 or 4):
+ * the difference-form sweep with k wavefronts per workgroup and `lanes` (8,
+ * 16, 32, 64) lanes per system on points of kind pk (OCO_PK_*), which reads
+ * field-major cell records (grey: rho and T) and needs no scratch */
+hipError_t oco_launch_sweep(const oco_sweep_args& a, int lgrey, int wave, int lanes, int pk, int nblocks,
+                            hipStream_t stream);
+/* point kinds of the difference-form sweep */
+enum { OCO_PK_MIXED = 0, OCO_PK_FP64 = 1, OCO_PK_GREY = 2 };
+/* lanes per system for nz cells (want: 8, 16, 32, 64, or 0 for the default) */
+int oco_wave_lanes(int nz, int want);
+int oco_wave_fits(int nz, int lanes, int pk, int device);
+/* wavefronts per workgroup of the difference-form sweep (want: 1, 2, 4, or 0 for the default) */
+int oco_wave_group(int nz, int lanes, int pk, int want, int device);
+int oco_wave_grid(int nz, int k, int lanes, int pk, int device);
 /* workgroups resident on the device, and wavefronts per workgroup, of the
- * lane-per-system sweep of a mode (grey: the FP64 kernel in both modes) */
-int oco_sweep_grid(int lgrey, int mixed, int uniform, int device);
-int oco_sweep_wg(int mixed);
+ * FP64 drop-in's lane-per-system sweep */
+int oco_sweep_grid(int lgrey, int uniform, int device);
+int oco_sweep_wg(void);
 
 /* host helpers shared by the C-ABI files */
 #ifdef __cplusplus

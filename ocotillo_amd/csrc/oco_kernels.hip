@@ -16,10 +16,12 @@
  *               back-substitution fused with the ghost extrapolation, the
  *               6th-order derivative (7-point register window) and V.
  *
+ *   sweep_mixed_wave  the difference-form sweep of OCO_MODE_MIXED (opacity
+ *               of oco_mixed.h) and OCO_MODE_ACCURATE (the drop-in's FP64
+ *               opacity): an aligned group of L lanes per system.
+ *
  * The FP64 path is compiled with -ffp-contract=off and its elementary
  * functions are oco_math.h's, read from an LDS copy of their tables.
- * sweep_mixed / batch_mixed are the same kernels with the mixed-precision
- * opacity of oco_mixed.h (OCO_MODE_MIXED).
  */
 #include "oco_internal.h"
 #include "oco_physics.h"
@@ -263,7 +265,7 @@ __device__ static inline void boundary_row(const oco_pt& pc, double dz, double d
 __device__ static inline void rh_interior_row(double am, const oco_pt& pc, double ap, double dz2, double& A,
                                               double& C, double& H, double& R)
 {
-#pragma clang fp contract(fast)
+#pragma clang fp contract(on)
     const double km = 0.5 * (pc.alpha + am), kp = 0.5 * (ap + pc.alpha);
     A = pc.alpha2 * kp;
     C = pc.alpha2 * km;
@@ -273,7 +275,7 @@ __device__ static inline void rh_interior_row(double am, const oco_pt& pc, doubl
 
 __device__ static inline void rh_boundary_row(const oco_pt& pc, double dz, double dz2, double& H, double& R)
 {
-#pragma clang fp contract(fast)
+#pragma clang fp contract(on)
     const double ze = ((pc.alpha * dz2) * (1.0 - pc.albedo)) * 0.25;
     H = (dz + ze) * pc.alpha2;
     R = (pc.S * ze) * pc.alpha2;
@@ -860,15 +862,6 @@ __global__ __launch_bounds__(OCO_WAVE * OCO_SWEEP_WG) OCO_SWEEP_ATTR void sweep_
     sweep_tiles<GREY, UNIFORM, false>(a, &smt, &stg[threadIdx.x / OCO_WAVE]);
 }
 
-/* Mixed-precision sweep (non-grey): FP32-core opacity, FP64 elimination.
- * Its elementary functions use no tables. */
-template <bool UNIFORM>
-__global__ __launch_bounds__(OCO_WAVE) OCO_SWEEP_ATTR void sweep_mixed(const oco_sweep_args a)
-{
-    __shared__ oco_stage_mixed stg;
-    sweep_tiles<false, UNIFORM, true>(a, nullptr, &stg);
-}
-
 /* First-pillar canaries (source/flux_feautrier.f90:198-201): sum(a), sum(b),
  * sum(c), sum(d) of system (ix,iy,iw) = (1,1,1), from the FP64 opacity in
  * either precision mode (the cell records are recomputed here, so the kernel
@@ -998,48 +991,76 @@ __device__ static inline void lane_swap32(double v, double& up, double& dn)
     dn = __builtin_bit_cast(double, int2{(int)lo[1], (int)hi[1]});
 }
 
-/* rows per lane of the wavefront sweep for columns of nz cells */
-__host__ __device__ static inline int wave_rows(int nz)
+/* rows per lane of the wavefront sweep with L lanes per system for columns
+ * of nz cells (odd: the lanes' chunk strides then hit distinct LDS banks) */
+__host__ __device__ static inline int wave_rows(int nz, int L = OCO_WAVE)
 {
-    return ((nz + OCO_WAVE - 1) / OCO_WAVE) | 1;
+    return ((nz + L - 1) / L) | 1;
 }
 
-/* Words of LDS a wavefront's system takes: alpha [nz], albedo [nz], S -> U
- * with ghosts [nz + 6]. */
-__host__ __device__ static inline long wave_lds_words(int nz) { return 3L * nz + 6; }
+/* Words of LDS a system of the wavefront sweep takes: alpha [nz], albedo
+ * [nz], S -> U with ghosts [nz + 6], and S of the three cells at each
+ * boundary [6], kept for the emergent flux after U has replaced S. */
+__host__ __device__ static inline long wave_lds_words(int nz) { return 3L * nz + 12; }
 
-/* 1. points of nk <= K systems of one column, one per wavefront of the
- * workgroup (source/continuous_opacity.f90:175-201): thread t takes cells
- * z = t, t + 64 K, ..., loads the cell's record once and evaluates it for
- * each of the nk wavelengths iw0 .. iw0 + nk - 1, into wavefront k's LDS
- * region (alpha, albedo, S; regions lstr >= wave_lds_words(nz) doubles
- * apart) and the alpha output of system k.  The column's
- * field-major records (field k of cell z at cc[k*nz + z]) are read through a
- * buffer descriptor: the field offset is a scalar, the lane offset a vector
- * register, no per-field address arithmetic (loads past the column's records
- * return 0). */
-template <int K>
-__device__ static inline void wave_points(const double* cc, int nz, const double* wtab, int iw0, int nk,
-                                          double* A0, long astride, double* lds, long lstr)
+/* Point kinds of the difference-form sweeps (OCO_PK_*): the mixed-precision
+ * opacity (oco_point_mixed, OCO_MODE_MIXED), the drop-in's FP64 opacity
+ * (oco_point, the same alpha, albedo and S bit for bit, OCO_MODE_ACCURATE),
+ * and the grey parameters from rho and T (oco_grey_point, both modes). */
+
+/* 1. points of nk <= S systems of one column, S consecutive wavelengths iw0
+ * .. iw0 + nk - 1 (source/continuous_opacity.f90:175-201, or :377-390 grey):
+ * each cell's record is loaded once per part of the workgroup and evaluated
+ * for that part's wavelengths, into system k's LDS region (alpha, albedo,
+ * S; regions lstr >= wave_lds_words(nz) doubles apart) and the alpha output
+ * of system k.  The K wavefronts split the cells when a column has at least
+ * 64 K / 2 of them (one part: thread t takes cells t, t + 64 K, ...);
+ * shorter columns need only W < K wavefronts per pass over their cells, so
+ * the workgroup splits into P = K / W parts, part p taking wavelengths p, p
+ * + P, ...  A wavefront's wavelengths are the same for all its lanes, so its
+ * table rows are scalar loads.  The column's field-major records (field k of
+ * cell z at cc[k*nz + z]) are read through a buffer descriptor: the field
+ * offset is a scalar, the lane offset a vector register, no per-field
+ * address arithmetic.  Grey columns read rho at cc[z] and T at cT[z]. */
+template <int K, int PK>
+__device__ static inline void wave_points(const double* cc, const double* cT, int nz, const double* wtab, int iw0,
+                                          int nk, double* A0, long astride, double* lds, long lstr,
+                                          double sigma_grey, const oco_mtab* M)
 {
     const __amdgpu_buffer_rsrc_t crs =
         __builtin_amdgcn_make_buffer_rsrc((void*)cc, (short)0, CR_NFIELD * nz * (int)sizeof(double), OCO_BUF_DWORD3);
+    int W = 1;
+    while (W < K && W * OCO_WAVE * 2 <= nz + OCO_WAVE - 1) W <<= 1;   /* wavefronts per pass over the cells */
+    const int wid = __builtin_amdgcn_readfirstlane((int)threadIdx.x / OCO_WAVE);
+    const int P = K / W, part = wid / W;
+    const int t0 = (wid - part * W) * OCO_WAVE + lane_id();
 #ifdef OCO_WAVE_POINTS_UNROLL
 #pragma unroll OCO_WAVE_POINTS_UNROLL
 #endif
-    for (int z = threadIdx.x; z < nz; z += OCO_WAVE * K) {
+    for (int z = t0; z < nz; z += OCO_WAVE * W) {
         double rec[CR_NFIELD];
-        const int zo = z * (int)sizeof(double);
+        if constexpr (PK == OCO_PK_GREY) {
+            rec[0] = cc[z];
+            rec[1] = cT[z];
+        } else {
+            const int zo = z * (int)sizeof(double);
 #pragma unroll
-        for (int k = 0; k < CR_NFIELD; k++)
-            rec[k] = __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(crs, zo,
-                                                                                    k * nz * (int)sizeof(double), 0));
-        for (int k = 0; k < nk; k++) {
-            double w[OCO_WT_STRIDE];
-            oco_cptr wp = (oco_cptr)(wtab + (long)(iw0 + k) * OCO_WT_STRIDE);
+            for (int k = 0; k < CR_NFIELD; k++)
+                rec[k] = __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(crs, zo,
+                                                                                        k * nz * (int)sizeof(double), 0));
+        }
+        for (int k = part; k < nk; k += P) {
+            oco_pt p;
+            if constexpr (PK == OCO_PK_GREY) {
+                p = oco_grey_point(rec[0], rec[1], sigma_grey, M);
+            } else {
+                double w[OCO_WT_STRIDE];
+                oco_cptr wp = (oco_cptr)(wtab + (long)(iw0 + k) * OCO_WT_STRIDE);
 #pragma unroll
-            for (int f = 0; f < OCO_WT_STRIDE; f++) w[f] = wp[f];
-            const oco_pt p = oco_point_mixed<1>(rec, w);
+                for (int f = 0; f < OCO_WT_STRIDE; f++) w[f] = wp[f];
+                if constexpr (PK == OCO_PK_MIXED) p = oco_point_mixed<1>(rec, w);
+                else p = oco_point(rec, w, M);
+            }
             double* L = lds + k * lstr;
             L[z] = p.alpha;
             L[nz + z] = p.albedo;
@@ -1083,7 +1104,7 @@ __device__ static inline void rh_chunk_init(rh_chunk& c)
 __device__ static inline void rh_chunk_row(rh_chunk& c, int z, int nz, bool first, double al, double om, double S,
                                            double anext, double dz, double dz2, double& D, double& W, double& Z)
 {
-#pragma clang fp contract(fast)   /* mixed mode: error bounds, not the reference's bits */
+#pragma clang fp contract(on)   /* mixed mode: error bounds, not the reference's bits */
     if (!first) {
         c.sD += c.pr * c.Zp;
         c.sE += c.pr * c.Ep;
@@ -1137,10 +1158,15 @@ __device__ static inline void rh_chunk_close(rh_chunk& c)
  * 0 or its own value instead of the wrapped-around neighbour's, its
  * coupling to that neighbour is exactly zero as well (before the last step,
  * A = 0 in lanes 0..31 and C = 0 in lanes 32..63).  G: this chunk's last
- * unknown, GL the previous chunk's. */
+ * unknown, GL the previous chunk's.  With L < 64 lanes per system the
+ * wavefront holds 64 / L systems in aligned groups of L lanes and only the
+ * steps below L run; the shifts and rotations still cross the groups, where
+ * the couplings are exactly zero for the same reason (a system's first chunk
+ * has W = 0, its last D = 0, and an absent system is identity rows). */
+template <int L>
 __device__ static inline void rh_reduced(const rh_chunk& c, bool rows, double& G, double& GL)
 {
-#pragma clang fp contract(fast)
+#pragma clang fp contract(on)
     const int lane = lane_id();
     const double Gn = lane_dn1(c.pr), Dn = lane_dn1(c.sD), En = lane_dn1(c.sE);
     double Ar = 0.0, Cr = 0.0, Hr = 1.0, Rr = 0.0;
@@ -1151,7 +1177,7 @@ __device__ static inline void rh_reduced(const rh_chunk& c, bool rows, double& G
         Rr = c.ZL + c.DL * Dn;
     }
 #pragma unroll
-    for (int sft = 1; sft < OCO_WAVE; sft <<= 1) {
+    for (int sft = 1; sft < L; sft <<= 1) {
         const int up = ((lane - sft) & (OCO_WAVE - 1)) * 4, dn = ((lane + sft) & (OCO_WAVE - 1)) * 4;
         const double di = mx_rcp((Ar + Cr) + Hr);
         const double an = Ar * di, cn = Cr * di, hn = Hr * di, rn = Rr * di;
@@ -1159,7 +1185,7 @@ __device__ static inline void rh_reduced(const rh_chunk& c, bool rows, double& G
         if (sft == 1) {
             am = lane_up1(an); hm = lane_up1(hn); rm = lane_up1(rn);
             cp = lane_dn1(cn); hp = lane_dn1(hn); rp = lane_dn1(rn);
-        } else if (sft == OCO_WAVE / 2) {
+        } else if (L == OCO_WAVE && sft == OCO_WAVE / 2) {
             double x;
             lane_swap32(an, am, x);
             lane_swap32(cn, x, cp);
@@ -1181,14 +1207,14 @@ __device__ static inline void rh_reduced(const rh_chunk& c, bool rows, double& G
 /* x[z] from x[z+1] and the previous chunk's last unknown */
 __device__ static inline double rh_back(double D, double W, double Z, double x, double GL)
 {
-#pragma clang fp contract(fast)
+#pragma clang fp contract(on)
     return (D * x + W * GL) + Z;
 }
 
 /* 6th-order derivative at e from U(e-3 .. e+3) (source/auxiliary.f90:64-78) */
 __device__ static inline double der7(double um3, double um2, double um1, double up1, double up2, double up3)
 {
-#pragma clang fp contract(fast)
+#pragma clang fp contract(on)
     const double c60 = 1.0 / 60.0;
     return c60 * (((45.0 * (up1 - um1)) - (9.0 * (up2 - um2))) + (up3 - um3));
 }
@@ -1202,23 +1228,63 @@ __device__ static inline double flux_v(double al, double df, double dz1)
 /* ghost f(n -+ i) = 2 f(n) - f(n +- i) (source/auxiliary.f90:49-62) */
 __device__ static inline double ghost(double fn, double fi)
 {
-#pragma clang fp contract(fast)
+#pragma clang fp contract(on)
     return 2.0 * fn - fi;
 }
 
-/* 2.-4. one system, the whole wavefront, from its points in `lds` (written by
- * wave_points; the caller has synchronised the workgroup since) */
-template <int MM>
-__device__ static inline void solve_system_wave(const oco_sys& d, double* lds)
+/* The derivative der of the difference-form solves at a boundary cell,
+ * counted inward, without differences of U.  With the ghosts of
+ * source/auxiliary.f90:49-62, der (:64-78) at a boundary cell is
+ *     der = (37 d0 - 8 d1 + d2) / 30,   dk = U(k + 1) - U(k),
+ * cells k = 0, 1, 2, 3 counted inward from the boundary (at the top, der is
+ * minus this).  On an optically thin cell the U agree to ~13 digits, so dk
+ * formed from them keeps few: the emergent flux of an FP64 U rounded to the
+ * last bit is off by up to 3e-3 on a 1e-15 g/cm^3, 2000 K floor.  The rows
+ * give each difference without that subtraction, from U and S, which are
+ * accurate to full precision:
+ *   boundary row (:187-197):  d0 = a0 (dz U0 + ze (U0 - S0)),
+ *                             ze = a0 dz^2 (1 - w0) / 4;
+ *   interior row (:159-165), c (U(k+1) - U(k)) - a (U(k) - U(k-1)) = zeta (U(k) - S(k)):
+ *     dk = kap(k+1/2) (d(k-1) / kap(k-1/2) + dz^2 ak (1 - wk) (Uk - Sk)),
+ * kap the interface means of alpha.  In a thick cell (alpha dz > 1), U is
+ * close to S and Uk - Sk is the smaller, less accurate difference, so there
+ * dk = U(k+1) - U(k) directly.  Both are the same der in exact arithmetic;
+ * measured on C4-profile columns, V from this is within 1e-11 of the exact
+ * solution where der(U) in FP64 is off by 3e-3. */
+__device__ static inline double rh_boundary_der(double u0, double u1, double u2, double u3, double a0, double a1,
+                                                double a2, double a3, double w0, double w1, double w2, double s0,
+                                                double s1, double s2, double dz, double dz2)
 {
-    const int nz = d.nz, lane = lane_id();
+#pragma clang fp contract(on)
+    const double ze = ((a0 * dz2) * (1.0 - w0)) * 0.25;
+    const double k01 = 0.5 * (a0 + a1), k12 = 0.5 * (a1 + a2), k23 = 0.5 * (a2 + a3);
+    const double d0 = a0 * dz <= 1.0 ? a0 * (dz * u0 + ze * (u0 - s0)) : u1 - u0;
+    const double d1 = a1 * dz <= 1.0 ? k12 * (d0 * mx_rcp(k01) + ((dz2 * a1) * (1.0 - w1)) * (u1 - s1)) : u2 - u1;
+    const double d2 = a2 * dz <= 1.0 ? k23 * (d1 * mx_rcp(k12) + ((dz2 * a2) * (1.0 - w2)) * (u2 - s2)) : u3 - u2;
+    return ((37.0 * d0 - 8.0 * d1) + d2) * (1.0 / 30.0);
+}
+
+/* 2.-4. one system on an aligned group of L lanes, from its points in `lds`
+ * (written by wave_points; the caller has synchronised the workgroup since).
+ * Every lane of the wavefront calls this; a group without a system
+ * (valid false) takes empty chunks, which makes its reduced rows identity
+ * rows, and stores nothing. */
+template <int MM, int L>
+__device__ static inline void solve_system_wave(const oco_sys& d, double* lds, bool valid)
+{
+    const int nz = d.nz, ll = lane_id() & (L - 1);
     const double dz = d.dz, dz1 = d.dz1, dz2 = d.dz2;
     double* LA = lds;              /* alpha */
     double* LW = lds + nz;         /* albedo */
     double* LS = lds + 2 * nz;     /* extended, e = z + 3: S, then U with ghosts */
-    const int m = wave_rows(nz);
-    const int z0 = lane * m < nz ? lane * m : nz;
+    double* LB = lds + 3 * nz + 6; /* S of cells 0, 1, 2, nz - 1, nz - 2, nz - 3 */
+    const int m = wave_rows(nz, L);
+    const int z0 = valid && ll * m < nz ? ll * m : nz;
     const int z1 = z0 + m < nz ? z0 + m : nz;
+    /* boundary lanes: lane 0 of the group the bottom (cells 0, 1, 2, ...
+     * upward), lane 1 the top */
+    const int bz = ll == 0 ? 0 : nz - 1, bs = ll == 0 ? 1 : -1;
+    if (valid && ll < 6) LB[ll] = LS[3 + (ll < 3 ? ll : nz + 2 - ll)];
 
     /* 2. rows (source/auxiliary.f90:147-197) in difference form and the
      * chunk's forward elimination */
@@ -1237,7 +1303,7 @@ __device__ static inline void solve_system_wave(const oco_sys& d, double* lds)
     rh_chunk_close(c);
     /* 3. the reduced system */
     double G, GL;
-    rh_reduced(c, z1 > z0, G, GL);
+    rh_reduced<L>(c, z1 > z0, G, GL);
     /* 4. back-substitution inside the chunk */
     double x = G;
 #pragma unroll
@@ -1250,24 +1316,38 @@ __device__ static inline void solve_system_wave(const oco_sys& d, double* lds)
     }
     wave_sync();
     /* ghosts (source/auxiliary.f90:49-62) */
-    if (lane < 3) {
-        const int i = lane + 1;   /* f(n2+i) = 2 f(n2) - f(n2-i) */
-        LS[nz + 2 + i] = ghost(LS[nz + 2], LS[nz + 2 - i]);
-    } else if (lane < 6) {
-        const int i = lane - 2;   /* f(n1-i) = 2 f(n1) - f(n1+i) */
-        LS[3 - i] = ghost(LS[3], LS[3 + i]);
+    if (valid) {
+        if (ll < 3) {
+            const int i = ll + 1;   /* f(n2+i) = 2 f(n2) - f(n2-i) */
+            LS[nz + 2 + i] = ghost(LS[nz + 2], LS[nz + 2 - i]);
+        } else if (ll < 6) {
+            const int i = ll - 2;   /* f(n1-i) = 2 f(n1) - f(n1+i) */
+            LS[3 - i] = ghost(LS[3], LS[3 + i]);
+        }
     }
     wave_sync();
-    /* der and calc_flux (:64-78, :96-109) */
-    for (int z = lane; z < nz; z += OCO_WAVE) {
+    if (!valid) return;
+    /* der and calc_flux (:64-78, :96-109): the emergent V(1), V(nz) from
+     * rh_boundary_der, every other V from der of U */
+    if (ll < 2) {
+        const double* u = LS + 3 + bz;
+        const double* al = LA + bz;
+        const double* om = LW + bz;
+        const double* sb = LB + 3 * ll;
+        const double df = rh_boundary_der(u[0], u[bs], u[2 * bs], u[3 * bs], al[0], al[bs], al[2 * bs], al[3 * bs],
+                                          om[0], om[bs], om[2 * bs], sb[0], sb[1], sb[2], dz, dz2);
+        out_store(d.V + bz, flux_v(al[0], ll == 0 ? df : -df, dz1));
+    }
+    for (int z = ll + 1; z < nz - 1; z += L) {
         const double* u = LS + z + 3;
         out_store(d.V + z, flux_v(LA[z], der7(u[-3], u[-2], u[-1], u[1], u[2], u[3]), dz1));
     }
-    for (int e = lane; e < nz + 6; e += OCO_WAVE) out_store(d.U + e, LS[e]);
+    for (int e = ll; e < nz + 6; e += L) out_store(d.U + e, LS[e]);
 }
 
-/* Mixed-precision sweep, one wavefront per system, K wavefronts per
- * workgroup on K wavelengths of one column. */
+/* The difference-form sweep of the block (OCO_MODE_MIXED, OCO_MODE_ACCURATE):
+ * L lanes per system, so a wavefront holds 64 / L systems, and K wavefronts
+ * per workgroup on S = K 64 / L consecutive wavelengths of one column. */
 /* Tuning knob: OCO_WAVE_WAVES_PER_EU=n asks the compiler to fit n waves of
  * the wavefront sweep per SIMD. */
 #ifdef OCO_WAVE_WAVES_PER_EU
@@ -1277,23 +1357,29 @@ __device__ static inline void solve_system_wave(const oco_sys& d, double* lds)
 #endif
 
 /* Systems a workgroup takes per ticket of the wavefront sweep: a ticket is
- * 32 / K groups of K consecutive wavelengths of one column.  Measured on
- * MI355X with workgroups of 4 wavefronts, 256 x 256 columns, nz = 512,
- * nw = 128: 8 systems 36.6-36.7 ms, 16 35.7-35.8, 32 35.4-35.5; on 64 x 64
- * columns of 256 cells, 1.39 and 1.40 ms for 8 and 32. */
+ * 32 / S groups of S consecutive wavelengths of one column (at least one).
+ * Measured on MI355X with workgroups of 4 wavefronts of one system each,
+ * 256 x 256 columns, nz = 512, nw = 128: 8 systems 36.6-36.7 ms, 16
+ * 35.7-35.8, 32 35.4-35.5; on 64 x 64 columns of 256 cells, 1.39 and 1.40 ms
+ * for 8 and 32. */
 #ifndef OCO_WAVE_TICKET
 #define OCO_WAVE_TICKET 32
 #endif
 
-template <int MM, int K>
+template <int MM, int K, int L, int PK>
 __global__ __launch_bounds__(OCO_WAVE * K) OCO_WAVE_ATTR void sweep_mixed_wave(const oco_sweep_args a)
 {
-    extern __shared__ double lds[];
+    extern __shared__ double lds[];   /* S systems' regions, then the FP64 math tables (not in mixed mode) */
     __shared__ long ticket;
+    constexpr int S = K * (OCO_WAVE / L);   /* systems per workgroup */
     const int nw = a.nw, nz = a.nz;
-    const int ngrp = (nw + K - 1) / K;   /* wavelength groups per column */
+    const long lstr = wave_lds_words(nz);
+    oco_mtab* smt = PK == OCO_PK_MIXED ? nullptr : (oco_mtab*)(lds + S * lstr);
+    if constexpr (PK != OCO_PK_MIXED) load_mtab(smt);
+    const int ngrp = (nw + S - 1) / S;   /* wavelength groups per column */
     const int wid = (int)threadIdx.x / OCO_WAVE;
-    constexpr long TG = OCO_WAVE_TICKET / K > 0 ? OCO_WAVE_TICKET / K : 1;   /* groups per ticket */
+    const int g = wid * (OCO_WAVE / L) + lane_id() / L;   /* this lane group's system in the workgroup's S */
+    constexpr long TG = OCO_WAVE_TICKET / S > 0 ? OCO_WAVE_TICKET / S : 1;   /* groups per ticket */
     const long astride = a.ncol * (long)nz;   /* alpha output: one wavelength to the next */
     /* XCD-aware work queues: workgroups are dispatched to the 8 XCDs
      * round-robin, so workgroup b runs on XCD b mod 8.  With a grid that is a
@@ -1301,7 +1387,7 @@ __global__ __launch_bounds__(OCO_WAVE * K) OCO_WAVE_ATTR void sweep_mixed_wave(c
      * column's wavelength groups in turn, and the workgroups of XCD x draw
      * tickets of TG groups from it: a column's cell records (22 fields x nz
      * doubles) are brought into one L2 and reused there, and each record a
-     * thread loads serves the K wavelengths of its group.  A workgroup whose
+     * thread loads serves the wavelengths of its group.  A workgroup whose
      * queue is empty goes on to the next XCD's, so the grid drains together.
      * Any other grid: one queue in (column, wavelength) order.  Every
      * workgroup ends once it has found all queues empty (the ticket is read
@@ -1321,37 +1407,47 @@ __global__ __launch_bounds__(OCO_WAVE * K) OCO_WAVE_ATTR void sweep_mixed_wave(c
             const long t1 = t0 + TG < ngroups ? t0 + TG : ngroups;
             for (long t = t0; t < t1; t++) {
                 const long col = (t / ngrp) * nq + q;
-                const int iw0 = (int)(t % ngrp) * K;
-                const int nk = nw - iw0 < K ? nw - iw0 : K;
+                const int iw0 = (int)(t % ngrp) * S;
+                const int nk = nw - iw0 < S ? nw - iw0 : S;
                 if (t > t0) __syncthreads();   /* the previous group's systems are done with LDS */
-                wave_points<K>(a.cells + col * (long)CR_NFIELD * nz, nz, a.wtab, iw0, nk,
-                               a.A + ((long)iw0 * a.ncol + col) * nz, astride, lds, wave_lds_words(nz));
+                const double* cc = PK == OCO_PK_GREY ? a.rho + col * (long)nz : a.cells + col * (long)CR_NFIELD * nz;
+                wave_points<K, PK>(cc, a.T + col * (long)nz, nz, a.wtab, iw0, nk,
+                                   a.A + ((long)iw0 * a.ncol + col) * nz, astride, lds, lstr, a.sigma_grey, smt);
                 __syncthreads();
-                if (wid < nk) {
-                    const int iw = iw0 + wid;
-                    const long osys = (long)iw * a.ncol + col;
-                    oco_sys d;
-                    d.nz = nz;
-                    d.dz = a.dz; d.dz1 = a.dz1; d.dz2 = a.dz2; d.sigma_grey = 0.0;
-                    d.cells = nullptr; d.rho = nullptr; d.T = nullptr;
-                    d.U = a.U + osys * (nz + 6);
-                    d.V = a.V + osys * nz;
-                    d.A = a.A + osys * nz;
-                    d.key = col * nw + iw;
-                    solve_system_wave<MM>(d, lds + wid * wave_lds_words(nz));
-                }
+                const bool valid = g < nk;
+                const int iw = iw0 + (valid ? g : 0);
+                const long osys = (long)iw * a.ncol + col;
+                oco_sys d;
+                d.nz = nz;
+                d.dz = a.dz; d.dz1 = a.dz1; d.dz2 = a.dz2; d.sigma_grey = a.sigma_grey;
+                d.cells = nullptr; d.rho = nullptr; d.T = nullptr;
+                d.U = a.U + osys * (nz + 6);
+                d.V = a.V + osys * nz;
+                d.A = a.A + osys * nz;
+                d.key = col * nw + iw;
+                solve_system_wave<MM, L>(d, lds + g * lstr, valid);
             }
         }
     }
 }
 
-static size_t wave_lds_bytes(int nz, int k) { return (size_t)k * wave_lds_words(nz) * sizeof(double); }
+/* LDS of a workgroup of the block sweep: S = k 64 / lanes systems' regions,
+ * plus the FP64 math tables unless the points are mixed-precision */
+static size_t wave_lds_bytes(int nz, int k, int lanes = OCO_WAVE, int pk = OCO_PK_MIXED)
+{
+    return (size_t)k * (OCO_WAVE / lanes) * wave_lds_words(nz) * sizeof(double) +
+           (pk == OCO_PK_MIXED ? 0 : sizeof(oco_mtab));
+}
 
+/* Rows per lane: at most 17 with 64 lanes per system (columns up to 1088
+ * cells), 9 with fewer lanes (a longer column takes more lanes). */
 #define OCO_WAVE_MAX_ROWS 17
+#define OCO_WAVE_MAX_ROWS_SUB 9
+static int wave_max_rows(int lanes) { return lanes == OCO_WAVE ? OCO_WAVE_MAX_ROWS : OCO_WAVE_MAX_ROWS_SUB; }
 
 /* Wavefronts per workgroup of the wavefront sweep: each cell record a thread
- * loads serves K wavelengths.  Measured on MI355X (256 x 256 columns,
- * nz = 512, nw = 128), see oco_wave_group. */
+ * loads serves the workgroup's wavelengths.  Measured on MI355X (256 x 256
+ * columns, nz = 512, nw = 128), see oco_wave_group. */
 #ifndef OCO_WAVE_GROUP
 #define OCO_WAVE_GROUP 4
 #endif
@@ -1364,55 +1460,70 @@ static int lds_limit(int device)
     return lim;
 }
 
-/* 1 when the wavefront-per-system mixed sweep takes columns of nz cells:
- * at most OCO_WAVE_MAX_ROWS rows per lane and one wavefront's LDS within a
- * workgroup's */
-int oco_wave_fits(int nz, int device)
+static bool lanes_ok(int lanes) { return lanes == 8 || lanes == 16 || lanes == 32 || lanes == 64; }
+
+/* Lanes per system for columns of nz cells: `want` (8, 16, 32, 64), or for
+ * 0 the fewest lanes whose chunks are at most 9 rows (64 above 576 cells). */
+int oco_wave_lanes(int nz, int want)
 {
-    return nz >= 4 && wave_rows(nz) <= OCO_WAVE_MAX_ROWS && wave_lds_bytes(nz, 1) <= (size_t)lds_limit(device);
+    if (lanes_ok(want)) return want;
+    for (int l = 8; l < OCO_WAVE; l <<= 1)
+        if (wave_rows(nz, l) <= OCO_WAVE_MAX_ROWS_SUB) return l;
+    return OCO_WAVE;
+}
+
+/* 1 when the difference-form sweep with `lanes` lanes per system takes
+ * columns of nz cells: rows per lane within its limit and one wavefront's
+ * LDS within a workgroup's */
+int oco_wave_fits(int nz, int lanes, int pk, int device)
+{
+    return nz >= 4 && lanes_ok(lanes) && wave_rows(nz, lanes) <= wave_max_rows(lanes) &&
+           wave_lds_bytes(nz, 1, lanes, pk) <= (size_t)lds_limit(device);
 }
 
 /* Wavefronts per workgroup for columns of nz cells: `want` (1, 2 or 4; 0 for
  * OCO_WAVE_GROUP), halved until the workgroup's LDS fits. */
-int oco_wave_group(int nz, int want, int device)
+int oco_wave_group(int nz, int lanes, int pk, int want, int device)
 {
     int k = want == 1 || want == 2 || want == 4 ? want : OCO_WAVE_GROUP;
-    while (k > 1 && wave_lds_bytes(nz, k) > (size_t)lds_limit(device)) k /= 2;
+    while (k > 1 && wave_lds_bytes(nz, k, lanes, pk) > (size_t)lds_limit(device)) k /= 2;
     return k;
 }
 
-/* Measured on MI355X on a 128 x 128-column block, nw = 128, with workgroups
- * of 4 wavefronts (sweep ms, lane / wave): nz 64 2.17/3.75, 128 4.92/4.97,
- * 192 7.47/5.18, 256 10.05/6.25, 1024 39.8/37.9: the wavefront sweep is
- * chosen above 128 cells, up to its 17 rows per lane. */
-#ifndef OCO_WAVE_AUTO_MIN_NZ
-#define OCO_WAVE_AUTO_MIN_NZ 129
-#endif
-int oco_wave_preferred(int nz)
+/* The kernel of a variant: (rows, lanes) as instantiated, k wavefronts per
+ * workgroup, point kind pk (grey: one wavefront per workgroup). */
+template <int K, int PK>
+static const void* wave_kernel_kp(int nz, int lanes)
 {
-    return nz >= OCO_WAVE_AUTO_MIN_NZ && wave_rows(nz) <= OCO_WAVE_MAX_ROWS;
+    const int m = wave_rows(nz, lanes);
+    switch (lanes) {
+    case 8: return (const void*)sweep_mixed_wave<9, K, 8, PK>;
+    case 16: return (const void*)sweep_mixed_wave<9, K, 16, PK>;
+    case 32: return (const void*)sweep_mixed_wave<9, K, 32, PK>;
+    default:
+        if (m <= 5) return (const void*)sweep_mixed_wave<5, K, 64, PK>;
+        if (m <= 9) return (const void*)sweep_mixed_wave<9, K, 64, PK>;
+        return (const void*)sweep_mixed_wave<17, K, 64, PK>;
+    }
 }
 
-template <int K>
-static const void* wave_kernel_k(int nz)
+static const void* wave_kernel(int nz, int k, int lanes, int pk)
 {
-    const int m = wave_rows(nz);
-    if (m <= 5) return (const void*)sweep_mixed_wave<5, K>;
-    if (m <= 9) return (const void*)sweep_mixed_wave<9, K>;
-    return (const void*)sweep_mixed_wave<17, K>;
-}
-
-static const void* wave_kernel(int nz, int k)
-{
-    return k == 4 ? wave_kernel_k<4>(nz) : k == 2 ? wave_kernel_k<2>(nz) : wave_kernel_k<1>(nz);
+    if (pk == OCO_PK_GREY) return wave_kernel_kp<1, OCO_PK_GREY>(nz, lanes);
+    if (pk == OCO_PK_FP64)
+        return k == 4 ? wave_kernel_kp<4, OCO_PK_FP64>(nz, lanes)
+                      : k == 2 ? wave_kernel_kp<2, OCO_PK_FP64>(nz, lanes) : wave_kernel_kp<1, OCO_PK_FP64>(nz, lanes);
+    return k == 4 ? wave_kernel_kp<4, OCO_PK_MIXED>(nz, lanes)
+                  : k == 2 ? wave_kernel_kp<2, OCO_PK_MIXED>(nz, lanes) : wave_kernel_kp<1, OCO_PK_MIXED>(nz, lanes);
 }
 
 /* Resident workgroups of the wavefront sweep with k wavefronts each. */
-int oco_wave_grid(int nz, int k, int device)
+int oco_wave_grid(int nz, int k, int lanes, int pk, int device)
 {
+    if (pk == OCO_PK_GREY) k = 1;
     int per_cu = 0, cus = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, wave_kernel(nz, k), OCO_WAVE * k,
-                                                     wave_lds_bytes(nz, k)) != hipSuccess || per_cu < 1)
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, wave_kernel(nz, k, lanes, pk), OCO_WAVE * k,
+                                                     wave_lds_bytes(nz, k, lanes, pk)) != hipSuccess || per_cu < 1)
         per_cu = 1;
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess || cus < 1)
         cus = 256;
@@ -1510,13 +1621,6 @@ __global__ __launch_bounds__(OCO_WAVE) void batch_fp64(const oco_batch_args a)
     batch_tiles<BC, false>(a, &smt, &tile_s, &stg);
 }
 
-template <int BC>
-__global__ __launch_bounds__(OCO_WAVE) void batch_mixed(const oco_batch_args a)
-{
-    __shared__ int tile_s;
-    __shared__ oco_stage_batch_mixed stg;
-    batch_tiles<BC, true>(a, nullptr, &tile_s, &stg);
-}
 
 /* ---- serving: tiles of many jobs in one persistent launch ----
  * Each tile is (job slot, column, first of 64 wavelengths); the job's
@@ -1593,6 +1697,9 @@ __global__ __launch_bounds__(OCO_WAVE) void serve_tiles(const oco_serve_args a)
 #ifndef OCO_SERVE_K
 #define OCO_SERVE_K 2
 #endif
+/* Doubles per row of the serving kernel's boundary window: D (then x), W,
+ * Z, alpha, albedo, S */
+#define OCO_SERVE_SLOT 6
 /* Rows per lane of the serving solve: the wavefront sweep's, at least
  * OCO_SERVE_MIN_ROWS (short columns then leave lanes empty: longer sequential
  * chunks and a smaller reduced system) */
@@ -1662,12 +1769,12 @@ hipError_t oco_launch_cells_chunked(const double* rho, const double* T, long nce
 template <int K>
 __global__ __launch_bounds__(OCO_WAVE * OCO_SERVE_WG) OCO_SERVE_ATTR void serve_wave(const oco_serve_args a)
 {
-    extern __shared__ double win[];   /* per wavefront: K systems x win_rows rows x (D or x, W, Z, alpha) */
+    extern __shared__ double win[];   /* per wavefront: K systems x win_rows rows x (D or x, W, Z, alpha, albedo, S) */
     __shared__ int tile_s;
     /* the wavefront's index, wave-uniform (its wavelengths' table rows are then scalar loads) */
     const int lane = lane_id(), wid = __builtin_amdgcn_readfirstlane((int)threadIdx.x / OCO_WAVE);
     const int WR = a.win_rows;
-    double* lds = win + (long)wid * K * WR * 4;
+    double* lds = win + (long)wid * K * WR * OCO_SERVE_SLOT;
     oco_cptr_i tiles = (oco_cptr_i)a.tiles;
     for (;;) {
         __syncthreads();   /* every wavefront has read the last tile_s */
@@ -1688,7 +1795,9 @@ __global__ __launch_bounds__(OCO_WAVE * OCO_SERVE_WG) OCO_SERVE_ATTR void serve_
         const int nB = nz < (3 / m + 1) * m ? nz : (3 / m + 1) * m;
         const int t0 = ((nz - 4) / m) * m > nB ? ((nz - 4) / m) * m : nB;
         const bool win = z0 < z1 && (z0 < nB || z1 > t0);
-        auto wslot = [&](int k, int z) -> double* { return lds + ((long)k * WR + (z < nB ? z : nB + (z - t0))) * 4; };
+        auto wslot = [&](int k, int z) -> double* {
+            return lds + ((long)k * WR + (z < nB ? z : nB + (z - t0))) * OCO_SERVE_SLOT;
+        };
         const int wend = wt0 + OCO_WAVE < jb.nw ? wt0 + OCO_WAVE : jb.nw;
         for (int iw0 = wt0 + wid * K; iw0 < wend; iw0 += K * OCO_SERVE_WG) {
             const int nk = wend - iw0 < K ? wend - iw0 : K;
@@ -1735,7 +1844,7 @@ __global__ __launch_bounds__(OCO_WAVE * OCO_SERVE_WG) OCO_SERVE_ATTR void serve_
                                 rh_chunk_row(st[k], z - 1, nz, j == 1, ap[k], op[k], Sp[k], p.alpha, dz, dz2, D, W, Z);
                                 if (win) {
                                     double* q = wslot(k, z - 1);
-                                    q[0] = D; q[1] = W; q[2] = Z; q[3] = ap[k];
+                                    q[0] = D; q[1] = W; q[2] = Z; q[3] = ap[k]; q[4] = op[k]; q[5] = Sp[k];
                                 }
                             }
                             ap[k] = p.alpha; op[k] = p.albedo; Sp[k] = p.S;
@@ -1756,11 +1865,11 @@ __global__ __launch_bounds__(OCO_WAVE * OCO_SERVE_WG) OCO_SERVE_ATTR void serve_
                         rh_chunk_close(st[k]);
                         if (win) {
                             double* q = wslot(k, z1 - 1);
-                            q[0] = D; q[1] = W; q[2] = Z; q[3] = ap[k];
+                            q[0] = D; q[1] = W; q[2] = Z; q[3] = ap[k]; q[4] = op[k]; q[5] = Sp[k];
                         }
                     }
                     double G, GL;
-                    rh_reduced(st[k], z0 < z1, G, GL);
+                    rh_reduced<OCO_WAVE>(st[k], z0 < z1, G, GL);
                     if (win) {   /* back-substitution in the window's chunks; x replaces D */
                         double x = G;
                         wslot(k, z1 - 1)[0] = x;
@@ -1777,19 +1886,15 @@ __global__ __launch_bounds__(OCO_WAVE * OCO_SERVE_WG) OCO_SERVE_ATTR void serve_
 #pragma unroll
                 for (int k = 0; k < K; k++) {
                     if (k < nk) {
-                        double v;
-                        if (lane == 0) {   /* V(1): U(3 + i) = x[i], ghosts below */
-                            const double x0 = wslot(k, 0)[0], x1 = wslot(k, 1)[0], x2 = wslot(k, 2)[0],
-                                         x3 = wslot(k, 3)[0];
-                            v = flux_v(wslot(k, 0)[3],
-                                       der7(ghost(x0, x3), ghost(x0, x2), ghost(x0, x1), x1, x2, x3), dz1);
-                        } else {           /* V(nz): ghosts above */
-                            const double xn1 = wslot(k, nz - 1)[0], xn2 = wslot(k, nz - 2)[0],
-                                         xn3 = wslot(k, nz - 3)[0], xn4 = wslot(k, nz - 4)[0];
-                            v = flux_v(wslot(k, nz - 1)[3],
-                                       der7(xn4, xn3, xn2, ghost(xn1, xn2), ghost(xn1, xn3), ghost(xn1, xn4)), dz1);
-                        }
-                        jb.em[((long)(iw0 + k) * jb.ncol + col) * 2 + lane] = v;
+                        /* V(1) in lane 0, V(nz) in lane 1: the cells counted inward */
+                        const int bz = lane == 0 ? 0 : nz - 1, bs = lane == 0 ? 1 : -1;
+                        const double* q0 = wslot(k, bz);
+                        const double* q1 = wslot(k, bz + bs);
+                        const double* q2 = wslot(k, bz + 2 * bs);
+                        const double* q3 = wslot(k, bz + 3 * bs);
+                        const double df = rh_boundary_der(q0[0], q1[0], q2[0], q3[0], q0[3], q1[3], q2[3], q3[3],
+                                                          q0[4], q1[4], q2[4], q0[5], q1[5], q2[5], dz, dz2);
+                        jb.em[((long)(iw0 + k) * jb.ncol + col) * 2 + lane] = flux_v(q0[3], lane == 0 ? df : -df, dz1);
                     }
                 }
             }
@@ -1798,7 +1903,10 @@ __global__ __launch_bounds__(OCO_WAVE * OCO_SERVE_WG) OCO_SERVE_ATTR void serve_
     }
 }
 
-static size_t serve_wave_lds(int win_rows) { return (size_t)OCO_SERVE_WG * OCO_SERVE_K * win_rows * 4 * sizeof(double); }
+static size_t serve_wave_lds(int win_rows)
+{
+    return (size_t)OCO_SERVE_WG * OCO_SERVE_K * win_rows * OCO_SERVE_SLOT * sizeof(double);
+}
 
 hipError_t oco_launch_serve(const oco_serve_args& a, int mixed, int nblocks, hipStream_t stream)
 {
@@ -1824,23 +1932,15 @@ int oco_serve_grid(int mixed, int max_nz, int device)
     return per_cu * cus;
 }
 
-hipError_t oco_launch_batch(const oco_batch_args& a, int mixed, int variant, int nblocks, hipStream_t stream)
+hipError_t oco_launch_batch(const oco_batch_args& a, int variant, int nblocks, hipStream_t stream)
 {
     (void)hipGetLastError();   /* a pending not-ready status from an event query is not this launch's */
     hipLaunchKernelGGL(batch_reset, dim3(1), dim3(64), 0, stream, a.counter, a.fail);
     const dim3 g(nblocks), b(OCO_WAVE);
-    if (mixed) {
-        switch (variant) {
-        case 0: hipLaunchKernelGGL((batch_mixed<1>), g, b, 0, stream, a); break;
-        case 1: hipLaunchKernelGGL((batch_mixed<2>), g, b, 0, stream, a); break;
-        default: hipLaunchKernelGGL((batch_mixed<4>), g, b, 0, stream, a); break;
-        }
-    } else {
-        switch (variant) {
-        case 0: hipLaunchKernelGGL((batch_fp64<1>), g, b, 0, stream, a); break;
-        case 1: hipLaunchKernelGGL((batch_fp64<2>), g, b, 0, stream, a); break;
-        default: hipLaunchKernelGGL((batch_fp64<4>), g, b, 0, stream, a); break;
-        }
+    switch (variant) {
+    case 0: hipLaunchKernelGGL((batch_fp64<1>), g, b, 0, stream, a); break;
+    case 1: hipLaunchKernelGGL((batch_fp64<2>), g, b, 0, stream, a); break;
+    default: hipLaunchKernelGGL((batch_fp64<4>), g, b, 0, stream, a); break;
     }
     return hipGetLastError();
 }
@@ -1866,18 +1966,21 @@ __global__ __launch_bounds__(256) void cells_cols_fp64(const double* __restrict_
     }
 }
 
-/* Batched mixed mode, one wavefront per system, K wavefronts per workgroup on
- * K wavelengths of one column (the block sweep's wave_points and
- * solve_system_wave, so a column gives the same bits as in a block of its
- * own).  Items are drawn from per-XCD queues of whole columns, longest
+/* Batched mixed mode: the block sweep's wave_points and solve_system_wave
+ * (L lanes per system, K wavefronts per workgroup on S = K 64 / L
+ * wavelengths of one column), with per-column shapes, so a column gives the
+ * same bits as in a block of its own with the same L.  Items (column, first
+ * of S wavelengths) are drawn from per-XCD queues of whole columns, longest
  * first; a workgroup whose queue is empty goes on to the next XCD's.  Each
- * wavefront's LDS region is sized for the batch's longest column. */
-template <int MM, int K>
+ * system's LDS region is sized for the longest column of the launch. */
+template <int MM, int K, int L>
 __global__ __launch_bounds__(OCO_WAVE * K) void batch_mixed_wave(const oco_batch_args a)
 {
     extern __shared__ double lds[];
     __shared__ int ticket;
+    constexpr int S = K * (OCO_WAVE / L);
     const int wid = (int)threadIdx.x / OCO_WAVE;
+    const int g = wid * (OCO_WAVE / L) + lane_id() / L;
     oco_cptr_i items = (oco_cptr_i)a.witems;
     oco_cptr_i qoff = (oco_cptr_i)a.wq_off;
     oco_cptr_i nzs = (oco_cptr_i)a.nz;
@@ -1903,65 +2006,57 @@ __global__ __launch_bounds__(OCO_WAVE * K) void batch_mixed_wave(const oco_batch
             if (t >= i1) break;
             const int col = items[2 * t], iw0 = items[2 * t + 1];
             const int nz = nzs[col], nw = nws[col];
-            const int nk = nw - iw0 < K ? nw - iw0 : K;
+            const int nk = nw - iw0 < S ? nw - iw0 : S;
             const long c0 = coff[col], v0 = voff[col];
-            wave_points<K>(a.cells_soa + c0 * CR_NFIELD, nz, a.wtab + woff[col] * OCO_WT_STRIDE, iw0, nk,
-                           a.A + v0 + (long)iw0 * nz, nz, lds, lstr);
+            wave_points<K, OCO_PK_MIXED>(a.cells_soa + c0 * CR_NFIELD, nullptr, nz, a.wtab + woff[col] * OCO_WT_STRIDE,
+                                         iw0, nk, a.A + v0 + (long)iw0 * nz, nz, lds, lstr, 0.0, nullptr);
             __syncthreads();
-            if (wid < nk) {
-                const int iw = iw0 + wid;
-                oco_sys d;
-                d.nz = nz;
-                d.dz = dzs[3 * col]; d.dz1 = dzs[3 * col + 1]; d.dz2 = dzs[3 * col + 2]; d.sigma_grey = 0.0;
-                d.cells = nullptr; d.rho = nullptr; d.T = nullptr;
-                d.U = a.U + uoff[col] + (long)iw * (nz + 6);
-                d.V = a.V + v0 + (long)iw * nz;
-                d.A = a.A + v0 + (long)iw * nz;
-                d.key = woff[col] + iw;
-                solve_system_wave<MM>(d, lds + wid * lstr);
-            }
+            const bool valid = g < nk;
+            const int iw = iw0 + (valid ? g : 0);
+            oco_sys d;
+            d.nz = nz;
+            d.dz = dzs[3 * col]; d.dz1 = dzs[3 * col + 1]; d.dz2 = dzs[3 * col + 2]; d.sigma_grey = 0.0;
+            d.cells = nullptr; d.rho = nullptr; d.T = nullptr;
+            d.U = a.U + uoff[col] + (long)iw * (nz + 6);
+            d.V = a.V + v0 + (long)iw * nz;
+            d.A = a.A + v0 + (long)iw * nz;
+            d.key = woff[col] + iw;
+            solve_system_wave<MM, L>(d, lds + g * lstr, valid);
         }
     }
 }
 
 template <int K>
-static const void* batch_wave_kernel_k(int max_nz)
+static const void* batch_wave_kernel_k(int max_nz, int lanes)
 {
-    const int m = wave_rows(max_nz);
-    if (m <= 5) return (const void*)batch_mixed_wave<5, K>;
-    if (m <= 9) return (const void*)batch_mixed_wave<9, K>;
-    return (const void*)batch_mixed_wave<17, K>;
+    switch (lanes) {
+    case 8: return (const void*)batch_mixed_wave<9, K, 8>;
+    case 16: return (const void*)batch_mixed_wave<9, K, 16>;
+    case 32: return (const void*)batch_mixed_wave<9, K, 32>;
+    default:
+        return wave_rows(max_nz) <= 9 ? (const void*)batch_mixed_wave<9, K, 64>
+                                      : (const void*)batch_mixed_wave<17, K, 64>;
+    }
 }
 
-static const void* batch_wave_kernel(int max_nz, int k)
+static const void* batch_wave_kernel(int max_nz, int k, int lanes)
 {
-    return k == 4 ? batch_wave_kernel_k<4>(max_nz) : k == 2 ? batch_wave_kernel_k<2>(max_nz)
-                                                            : batch_wave_kernel_k<1>(max_nz);
+    return k == 4 ? batch_wave_kernel_k<4>(max_nz, lanes)
+                  : k == 2 ? batch_wave_kernel_k<2>(max_nz, lanes) : batch_wave_kernel_k<1>(max_nz, lanes);
 }
 
-int oco_batch_wave_grid(int max_nz, int k, int device)
+int oco_batch_wave_grid(int max_nz, int k, int lanes, int device)
 {
-    if (max_nz < 4 || wave_rows(max_nz) > OCO_WAVE_MAX_ROWS) return 0;
-    if (wave_lds_bytes(max_nz, k) > (size_t)lds_limit(device)) return 0;
+    if (!oco_wave_fits(max_nz, lanes, OCO_PK_MIXED, device)) return 0;
+    if (wave_lds_bytes(max_nz, k, lanes) > (size_t)lds_limit(device)) return 0;
     int per_cu = 0, cus = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, batch_wave_kernel(max_nz, k), OCO_WAVE * k,
-                                                     wave_lds_bytes(max_nz, k)) != hipSuccess || per_cu < 1)
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, batch_wave_kernel(max_nz, k, lanes), OCO_WAVE * k,
+                                                     wave_lds_bytes(max_nz, k, lanes)) != hipSuccess || per_cu < 1)
         per_cu = 1;
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess || cus < 1)
         cus = 256;
     const int g = per_cu * cus;
     return g >= OCO_XCDS ? g / OCO_XCDS * OCO_XCDS : g;
-}
-
-template <int K>
-static void launch_batch_wave(const oco_batch_args& a, int nblocks, hipStream_t stream)
-{
-    const int m = wave_rows(a.max_nz);
-    const dim3 g(nblocks), b(OCO_WAVE * K);
-    const size_t sh = wave_lds_bytes(a.max_nz, K);
-    if (m <= 5) hipLaunchKernelGGL((batch_mixed_wave<5, K>), g, b, sh, stream, a);
-    else if (m <= 9) hipLaunchKernelGGL((batch_mixed_wave<9, K>), g, b, sh, stream, a);
-    else hipLaunchKernelGGL((batch_mixed_wave<17, K>), g, b, sh, stream, a);
 }
 
 hipError_t oco_launch_batch_wave_records(const oco_batch_args& a, int ncol, double fully_ionized_T,
@@ -1975,14 +2070,14 @@ hipError_t oco_launch_batch_wave_records(const oco_batch_args& a, int ncol, doub
     return hipGetLastError();
 }
 
-hipError_t oco_launch_batch_wave(const oco_batch_args& a, int k, int nblocks, hipStream_t stream)
+hipError_t oco_launch_batch_wave(const oco_batch_args& a, int k, int lanes, int nblocks, hipStream_t stream)
 {
     (void)hipGetLastError();
-    if (!a.wtickets || !a.cells_soa || nblocks <= 0) return hipErrorInvalidValue;
+    if (!a.wtickets || !a.cells_soa || nblocks <= 0 || !lanes_ok(lanes)) return hipErrorInvalidValue;
     hipLaunchKernelGGL(tickets_reset, dim3(1), dim3(64), 0, stream, a.wtickets);
-    if (k == 4) launch_batch_wave<4>(a, nblocks, stream);
-    else if (k == 2) launch_batch_wave<2>(a, nblocks, stream);
-    else launch_batch_wave<1>(a, nblocks, stream);
+    const void* kern = batch_wave_kernel(a.max_nz, k, lanes);
+    void* args[] = {(void*)&a};
+    (void)hipLaunchKernel(kern, dim3(nblocks), dim3(OCO_WAVE * k), args, wave_lds_bytes(a.max_nz, k, lanes), stream);
     return hipGetLastError();
 }
 
@@ -2154,20 +2249,16 @@ hipError_t oco_launch_cells(const double* rho, const double* T, long ncells, dou
 /* Workgroups of the sweep kernel that fit on the device at once (occupancy x
  * CUs): the grid-stride sweep is launched with exactly that many, so every
  * workgroup is resident from the start and takes an equal share of tiles. */
-/* wavefronts per workgroup of the lane-per-system sweeps */
-int oco_sweep_wg(int mixed)
-{
-    return mixed ? 1 : OCO_SWEEP_WG;
-}
+/* wavefronts per workgroup of the lane-per-system sweep */
+int oco_sweep_wg(void) { return OCO_SWEEP_WG; }
 
-/* resident workgroups of a lane-per-system sweep */
-int oco_sweep_grid(int lgrey, int mixed, int uniform, int device)
+/* resident workgroups of the lane-per-system sweep (the FP64 drop-in) */
+int oco_sweep_grid(int lgrey, int uniform, int device)
 {
     const void* k;
     if (lgrey) k = (const void*)sweep_fp64<true, false>;
-    else if (mixed) k = uniform ? (const void*)sweep_mixed<true> : (const void*)sweep_mixed<false>;
     else k = uniform ? (const void*)sweep_fp64<false, true> : (const void*)sweep_fp64<false, false>;
-    const int wg = oco_sweep_wg(mixed && !lgrey);
+    const int wg = OCO_SWEEP_WG;
     int per_cu = 0, cus = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, OCO_WAVE * wg, 0) != hipSuccess || per_cu < 1)
         per_cu = 8 / wg > 0 ? 8 / wg : 1;
@@ -2176,18 +2267,16 @@ int oco_sweep_grid(int lgrey, int mixed, int uniform, int device)
     return per_cu * cus;
 }
 
-template <int K>
-static void launch_wave(const oco_sweep_args& a, int nblocks, hipStream_t stream)
+static void launch_wave(const oco_sweep_args& a, int k, int lanes, int pk, int nblocks, hipStream_t stream)
 {
-    const int m = wave_rows(a.nz);
-    const dim3 g(nblocks), b(OCO_WAVE * K);
-    const size_t sh = wave_lds_bytes(a.nz, K);
-    if (m <= 5) hipLaunchKernelGGL((sweep_mixed_wave<5, K>), g, b, sh, stream, a);
-    else if (m <= 9) hipLaunchKernelGGL((sweep_mixed_wave<9, K>), g, b, sh, stream, a);
-    else hipLaunchKernelGGL((sweep_mixed_wave<17, K>), g, b, sh, stream, a);
+    if (pk == OCO_PK_GREY) k = 1;
+    const void* kern = wave_kernel(a.nz, k, lanes, pk);
+    void* args[] = {(void*)&a};
+    (void)hipLaunchKernel(kern, dim3(nblocks), dim3(OCO_WAVE * k), args, wave_lds_bytes(a.nz, k, lanes, pk), stream);
 }
 
-hipError_t oco_launch_sweep(const oco_sweep_args& a, int lgrey, int mixed, int wave, int nblocks, hipStream_t stream)
+hipError_t oco_launch_sweep(const oco_sweep_args& a, int lgrey, int wave, int lanes, int pk, int nblocks,
+                            hipStream_t stream)
 {
     (void)hipGetLastError();   /* a pending not-ready status from an event query is not this launch's */
     const bool uniform = (a.nw % OCO_WAVE) == 0;
@@ -2199,17 +2288,11 @@ hipError_t oco_launch_sweep(const oco_sweep_args& a, int lgrey, int mixed, int w
     }
     if (!a.tickets) return hipErrorInvalidValue;
     hipLaunchKernelGGL(tickets_reset, dim3(1), dim3(64), 0, stream, a.tickets);
-    const dim3 g(nblocks), b(OCO_WAVE), bf(OCO_WAVE * OCO_SWEEP_WG);
-    if (lgrey) {
-        /* grey opacity has no per-wavelength transcendentals: one kernel for both modes */
+    const dim3 g(nblocks), bf(OCO_WAVE * OCO_SWEEP_WG);
+    if (wave) {
+        launch_wave(a, wave, lanes, pk, nblocks, stream);
+    } else if (lgrey) {
         hipLaunchKernelGGL((sweep_fp64<true, false>), g, bf, 0, stream, a);
-    } else if (mixed && wave) {
-        if (wave == 4) launch_wave<4>(a, nblocks, stream);
-        else if (wave == 2) launch_wave<2>(a, nblocks, stream);
-        else launch_wave<1>(a, nblocks, stream);
-    } else if (mixed) {
-        if (uniform) hipLaunchKernelGGL((sweep_mixed<true>), g, b, 0, stream, a);
-        else hipLaunchKernelGGL((sweep_mixed<false>), g, b, 0, stream, a);
     } else if (uniform) {
         hipLaunchKernelGGL((sweep_fp64<false, true>), g, bf, 0, stream, a);
     } else {

@@ -101,7 +101,7 @@ __device__ static inline double mx_div(double x, double y) { return x * mx_rcp(y
  * as the series (y clamped to 1000) stays far below E = exp(308). */
 __device__ static inline double mx_omexp_near(double y, double d, double E, double r1)
 {
-#pragma clang fp contract(fast)
+#pragma clang fp contract(on)
     const double dl = __builtin_fmin(y, 1000.0) - d;
     const double q = dl * __builtin_fma(dl, __builtin_fma(dl, 1.0 / 6.0, -0.5), 1.0);
     return (E + q) * r1;
@@ -114,7 +114,7 @@ __device__ static inline oco_pt oco_point_mixed(const double* c, const double* w
 {
     /* products feeding sums may be fused (one rounding instead of two): the
      * mixed mode is held to error bounds, not to the reference's bits */
-#pragma clang fp contract(fast)
+#pragma clang fp contract(on)
     oco_pt p;
     double theta = c[CR_THETA], fn = c[CR_FN];
     /* the Planck term's exp(d) - 1, and 1/E, 1/(1 + E) from one reciprocal

@@ -390,8 +390,72 @@ void oo_tridag_rh(int n, const double* A, const double* C, const double* H, cons
     for (int i = n - 2; i >= 0; i--) u[i] = Dw[i] * u[i + 1] + u[i];
 }
 
+/* The exact solution of the reference's discrete problem: the rows of
+ * source/auxiliary.f90:147-197, the Thomas elimination :14-47, update_ghosts
+ * :49-62, der :64-78 and calc_flux :96-109, in the reference's operation
+ * order but evaluated in IEEE binary128 (113-bit significand) from the FP64
+ * alpha, albedo and S of the reference's opacity; 1/alpha and alpha^2 are
+ * formed in binary128 too, dz, 1/dz and dz^2 are the FP64 constants the
+ * reference sets.  Where FP64 rounds the problem away -- zeta / (a + c) ~
+ * (alpha dz)^2 / 2 reaches 1e-27 on an optically thin floor, and the
+ * emergent flux is a difference of U values 1e-13 apart relative --
+ * binary128 (eps 1.9e-34) still carries ~20 digits.  U, V are rounded to FP64
+ * once, at the end.  Nothing here ever stops: a zero pivot cannot occur for
+ * these rows in binary128 (every pivot is -(a + c + zeta) plus a positive
+ * fraction of a, |pivot| >= zeta > 0). */
+typedef __float128 oo_q;
+
+void oo_exact_system(int nz, const double* al, const double* om, const double* S, double dz, double dz1,
+                     double dz2, double* U, double* V)
+{
+    oo_q* w = (oo_q*)malloc(sizeof(oo_q) * (size_t)(nz * 5 + 6));
+    oo_q *a = w, *b = a + nz, *c = b + nz, *d = c + nz, *u = d + nz;   /* u: mz values */
+    const oo_q qdz = dz, qdz1 = dz1, qdz2 = dz2;
+    for (int i = 1; i < nz - 1; i++) {
+        const oo_q ai = al[i], kp = (oo_q)0.5 * ((oo_q)al[i + 1] + ai), km = (oo_q)0.5 * (ai + (oo_q)al[i - 1]);
+        const oo_q a2 = ai * ai;
+        a[i] = a2 / km;
+        c[i] = a2 / kp;
+        const oo_q zeta = (qdz2 * (a2 * ai)) * ((oo_q)1 - (oo_q)om[i]);
+        b[i] = -((a[i] + c[i]) + zeta);
+        d[i] = -((oo_q)S[i] * zeta);
+    }
+    for (int k = 0; k < 2; k++) {
+        const int i = k ? nz - 1 : 0;
+        const oo_q ai = al[i];
+        const oo_q zeta = ((ai * qdz2) * ((oo_q)1 - (oo_q)om[i])) * (oo_q)0.25;
+        a[i] = k ? ai : 0;
+        c[i] = k ? 0 : ai;
+        b[i] = -(((oo_q)1 / ai + qdz) + zeta) * (ai * ai);
+        d[i] = -(((oo_q)S[i] * zeta) * (ai * ai));
+    }
+    /* Thomas: the gam of row j kept in a[j] once a[j] is used */
+    oo_q* x = u + 3;
+    oo_q bet = b[0];
+    x[0] = d[0] / bet;
+    for (int j = 1; j < nz; j++) {
+        const oo_q gam = c[j - 1] / bet;
+        bet = b[j] - a[j] * gam;
+        x[j] = (d[j] - a[j] * x[j - 1]) / bet;
+        a[j] = gam;
+    }
+    for (int j = nz - 2; j >= 0; j--) x[j] = x[j] - a[j + 1] * x[j + 1];
+    for (int i = 1; i <= 3; i++) {
+        x[-i] = (oo_q)2 * x[0] - x[i];
+        x[nz - 1 + i] = (oo_q)2 * x[nz - 1] - x[nz - 1 - i];
+    }
+    for (int i = 0; i < nz; i++) {
+        const oo_q* f = x + i;
+        const oo_q df = (((oo_q)45 * (f[1] - f[-1]) - (oo_q)9 * (f[2] - f[-2])) + (f[3] - f[-3])) / (oo_q)60;
+        V[i] = (double)(-(df / (oo_q)al[i]) * qdz1);
+    }
+    for (int e = 0; e < nz + 6; e++) U[e] = (double)u[e];
+    free(w);
+}
+
 /* One (column, wavelength) system: source/flux_feautrier.f90:183-216.
- * rh != 0 solves the rows in difference form instead (see oo_tridag_rh). */
+ * rh == 1 solves the rows in difference form instead (see oo_tridag_rh),
+ * rh == 2 exactly (oo_exact_system). */
 static int solve_system(int nz, const oo_tables* t, const oo_cell* cells, int iw, int lgrey,
                         double sigma_grey, double dz, double dz1, double dz2, double* work,
                         double* U, double* V, double* A, int* jfail, double sums[4], int rh)
@@ -410,6 +474,11 @@ static int solve_system(int nz, const oo_tables* t, const oo_cell* cells, int iw
         for (int i = 0; i < nz; i++) { sums[0] += a[i]; sums[1] += b[i]; sums[2] += c[i]; sums[3] += d[i]; }
     }
     int st = 0;
+    if (rh == 2) {
+        oo_exact_system(nz, al, om, S, dz, dz1, dz2, U, V);
+        memcpy(A, al, sizeof(double) * (size_t)nz);
+        return 0;
+    }
     if (rh) {
         oo_rh_coefficients(nz, al, om, S, dz, dz2, a, c, b, d);   /* a = A, c = C, b = H, d = R */
         oo_tridag_rh(nz, a, c, b, d, U + 3, gam);

@@ -91,6 +91,12 @@ int oo_solve_block_ex(int nz, int nyloc, int nxloc, const oo_tables* t, double d
                       const double* rho3d, const double* T3d, double fully_ionized_T, int lgrey,
                       double sigma_grey, double* U, double* V, double* alpha, oo_diag* diag, int nthreads,
                       int rh);
+/* rh == 2: every system solved exactly (oo_exact_system). */
+/* The reference's rows, Thomas, ghosts, der and flux of one system evaluated
+ * in binary128 from FP64 alpha, albedo, S: the exact solution of the
+ * reference's discrete problem, rounded once to FP64.  U has nz + 6 values. */
+void oo_exact_system(int nz, const double* al, const double* om, const double* S, double dz, double dz1,
+                     double dz2, double* U, double* V);
 void oo_rh_coefficients(int nz, const double* al, const double* om, const double* S, double dz, double dz2,
                         double* A, double* C, double* H, double* R);
 void oo_tridag_rh(int n, const double* A, const double* C, const double* H, const double* R, double* u,

@@ -67,6 +67,7 @@ def lib(portable=False):
         L.oo_solve_block_ex.argtypes = L.oo_solve_block.argtypes + [C.c_int]
         L.oo_rh_coefficients.argtypes = [C.c_int, DP, DP, DP, C.c_double, C.c_double, DP, DP, DP, DP]
         L.oo_tridag_rh.argtypes = [C.c_int, DP, DP, DP, DP, DP, DP]
+        L.oo_exact_system.argtypes = [C.c_int, DP, DP, DP, C.c_double, C.c_double, C.c_double, DP, DP]
         L.oo_tridag.argtypes = [C.c_int, DP, DP, DP, DP, DP, DP, C.POINTER(C.c_int)]
         L.oo_tridag_coefficients.argtypes = [C.c_int, DP, DP, DP, DP, DP, C.c_double, C.c_double,
                                              DP, DP, DP, DP]
@@ -130,11 +131,14 @@ class Oracle:
         return p
 
     def solve_block(self, rho3d, T3d, dz, fully_ionized_T=2e4, lgrey=False, sigma_grey=0.4,
-                    dz1=None, dz2=None, nthreads=0, rh=False):
+                    dz1=None, dz2=None, nthreads=0, rh=False, exact=False):
         """rho3d, T3d: arrays [ix, iy, z] (C order == Fortran (z,iy,ix)).
         Returns U [nw,ix,iy,mz], V and alpha [nw,ix,iy,nz], Diag.
         rh=True solves the rows in difference form (the accurate solution of
-        the same discrete problem; not the reference's elimination)."""
+        the same discrete problem; not the reference's elimination).
+        exact=True evaluates the reference's rows, Thomas, ghosts, der and
+        flux in binary128 on this oracle's FP64 opacity (oo_exact_system): the
+        exact solution of the reference's discrete problem, rounded to FP64."""
         rho3d = np.ascontiguousarray(rho3d, dtype=np.float64)
         T3d = np.ascontiguousarray(T3d, dtype=np.float64)
         nxloc, nyloc, nz = rho3d.shape
@@ -148,7 +152,7 @@ class Oracle:
         self.L.oo_solve_block_ex(nz, nyloc, nxloc, C.byref(self.t), dz, dz1, dz2,
                                  rho3d.reshape(-1), T3d.reshape(-1), fully_ionized_T, int(bool(lgrey)),
                                  sigma_grey, U.reshape(-1), V.reshape(-1), A.reshape(-1), C.byref(d), nthreads,
-                                 int(bool(rh)))
+                                 2 if exact else int(bool(rh)))
         return U, V, A, d
 
 
