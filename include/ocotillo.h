@@ -11,7 +11,8 @@
  * bind(C) / value and the Python tools with ctypes.
  *
  * Threading: a context is used by one host thread at a time.  Every call is
- * synchronous unless its name ends in _async.  No global state.
+ * synchronous unless its name ends in _async.  No global state but the
+ * per-GPU static scratch of oco_solve_block_device_noalloc.
  */
 #ifndef OCOTILLO_H
 #define OCOTILLO_H
@@ -102,6 +103,26 @@ int oco_solve_block(oco_ctx* ctx, int nz, int nyloc, int nxloc, double dz, doubl
 int oco_solve_block_device(oco_ctx* ctx, int nz, int nyloc, int nxloc, double dz, double dz1,
                            double dz2, const double* d_rho3d, const double* d_T3d, double* d_U,
                            double* d_V, double* d_absorp, int mode, oco_diag* diag);
+
+/* oco_solve_block_device with no device allocation at all, for callers
+ * that cannot allocate: code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc.  No counterpart in the reference.  Its
+ * scratch (cell records, queue tickets, failure key, canaries) is one
+ * static device array per GPU of OCO_STATIC_SCRATCH_MIB MiB, held by each
+ * GPU the library runs on until the process ends: blocks of at most about
+ * OCO_STATIC_SCRATCH_MIB * 2^20 / 168 cells (1.6M: 64 x 64 columns of 390
+ * cells; 168 bytes of cell record each), else OCO_E_ARG.  Modes OCO_MODE_MIXED and OCO_MODE_ACCURATE only
+ * (the drop-in's elimination scratch grows with the grid): OCO_E_ARG for
+ * OCO_MODE_FP64.  The wavelength table must already be set
+ * (oco_set_wavelengths, which allocates at set-up).  Every call on one GPU
+ * uses that GPU's one array, whatever the context: calls that overlap on a
+ * GPU (two host threads, two contexts' streams) race and must be
+ * serialised by the caller.  Results equal oco_solve_block_device's bit for
+ * bit. */
+#define OCO_STATIC_SCRATCH_MIB 256
+int oco_solve_block_device_noalloc(oco_ctx* ctx, int nz, int nyloc, int nxloc, double dz, double dz1,
+                                   double dz2, const double* d_rho3d, const double* d_T3d, double* d_U,
+                                   double* d_V, double* d_absorp, int mode, oco_diag* diag);
 
 /* ---- Opacity query ----
  * Absorption coefficient alpha (1/cm), single-scattering albedo and source

@@ -28,7 +28,7 @@ MODE_FP64, MODE_MIXED, MODE_ACCURATE = 0, 1, 2
 
 # every symbol include/ocotillo.h declares
 EXPORTS = ["oco_create", "oco_destroy", "oco_set_wavelengths", "oco_set_physics", "oco_solve_block",
-           "oco_solve_block_device", "oco_strerror", "oco_version", "oco_athena_slab", "oco_athena_slab_device",
+           "oco_solve_block_device", "oco_solve_block_device_noalloc", "oco_strerror", "oco_version", "oco_athena_slab", "oco_athena_slab_device",
            "oco_solve_block_files", "oco_write_record", "oco_opacity_points",
            "oco_batch_create", "oco_batch_destroy", "oco_batch_layout", "oco_batch_buffers", "oco_batch_run",
            "oco_batch_run_host", "oco_batch_tuning",
@@ -96,6 +96,7 @@ def load_library(path=None):
     L.oco_solve_block_device.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double,
                                          C.c_double, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                          C.c_void_p, C.c_int, C.POINTER(Diag)]
+    L.oco_solve_block_device_noalloc.argtypes = L.oco_solve_block_device.argtypes
     _SZ = C.POINTER(C.c_size_t)
     _PP = C.POINTER(C.c_void_p)
     L.oco_solve_block_files.argtypes = ([C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double,
@@ -214,16 +215,18 @@ class Context:
         return U, V, A, d
 
     def solve_block_device(self, nz, nyloc, nxloc, dz, d_rho, d_T, d_U, d_V, d_A, dz1=None, dz2=None,
-                           mode=MODE_FP64, raise_on_pivot=True):
-        """Device pointers (ints, e.g. torch tensor.data_ptr()) in and out."""
+                           mode=MODE_FP64, raise_on_pivot=True, noalloc=False):
+        """Device pointers (ints, e.g. torch tensor.data_ptr()) in and out.
+        noalloc=True: oco_solve_block_device_noalloc (no device allocation;
+        difference-form modes, blocks within the static scratch)."""
         d = Diag()
         dz1 = 1.0 / dz if dz1 is None else dz1
         dz2 = dz * dz if dz2 is None else dz2
-        st = self.L.oco_solve_block_device(self.h, nz, nyloc, nxloc, dz, dz1, dz2, C.c_void_p(d_rho),
-                                           C.c_void_p(d_T), C.c_void_p(d_U), C.c_void_p(d_V), C.c_void_p(d_A),
-                                           mode, C.byref(d))
+        name = "oco_solve_block_device_noalloc" if noalloc else "oco_solve_block_device"
+        st = getattr(self.L, name)(self.h, nz, nyloc, nxloc, dz, dz1, dz2, C.c_void_p(d_rho), C.c_void_p(d_T),
+                                   C.c_void_p(d_U), C.c_void_p(d_V), C.c_void_p(d_A), mode, C.byref(d))
         if st != OK and (raise_on_pivot or st not in (E_PIVOT0_STAGE1, E_PIVOT0_STAGE2)):
-            raise OcoError(st, "oco_solve_block_device: " + strerror(st), d)
+            raise OcoError(st, name + ": " + strerror(st), d)
         return d
 
 

@@ -49,6 +49,9 @@ struct oco_ctx {
     hipEvent_t ev_sweep[2] = {nullptr, nullptr}, ev_copy[2] = {nullptr, nullptr};
     void* pinned = nullptr;
     size_t pinned_n = 0;
+    /* this device's static scratch (oco_solve_block_device_noalloc) */
+    void* sscr = nullptr;
+    size_t sscr_n = 0;
 };
 
 static int hip_status(hipError_t e) { return e == hipSuccess ? OCO_OK : OCO_E_HIP; }
@@ -218,6 +221,12 @@ int oco_create(int device, size_t hbm_budget, oco_ctx** out)
             oco_destroy(c);
             return OCO_E_HIP;
         }
+    }
+    /* the allocation-free solve's scratch on this device, looked up here
+     * with the device current (once per device for the process) */
+    if (oco_static_scratch(device, &c->sscr, &c->sscr_n) != OCO_OK) {
+        oco_destroy(c);
+        return OCO_E_HIP;
     }
     *out = c;
     return OCO_OK;
@@ -449,6 +458,103 @@ int oco_solve_block_device(oco_ctx* c, int nz, int nyloc, int nxloc, double dz, 
         HIPCHK(hipMemcpy(h.data() + nz + 6, d_V, nz * sizeof(double), hipMemcpyDeviceToHost));
         HIPCHK(hipMemcpy(h.data() + 2 * nz + 6, d_A, nz * sizeof(double), hipMemcpyDeviceToHost));
         pillar_minmax(h.data(), h.data() + nz + 6, h.data() + 2 * nz + 6, nz, &d);
+    }
+    if (diag) *diag = d;
+    return d.status;
+}
+
+/* Byte layout of the allocation-free solve in the static scratch: tickets,
+ * failure key, canary sums, first-pillar min/max, then the field-major cell
+ * records (none for grey columns), each 256-byte aligned; the total. */
+struct StaticLayout {
+    size_t tickets, fail, canary, minmax, cells, total;
+};
+
+static size_t align256(size_t n) { return (n + 255) & ~(size_t)255; }
+
+static StaticLayout static_layout(int nz, long ncells, int lgrey)
+{
+    StaticLayout L;
+    L.tickets = 0;
+    L.fail = L.tickets + align256(OCO_TICKET_BYTES);
+    L.canary = L.fail + align256(sizeof(unsigned long long));
+    L.minmax = L.canary + align256((4 + 5 * (size_t)nz) * sizeof(double));
+    L.cells = L.minmax + align256(6 * sizeof(double));
+    L.total = L.cells + (lgrey ? 0 : (size_t)ncells * CR_NFIELD * sizeof(double));
+    return L;
+}
+
+int oco_solve_block_device_noalloc(oco_ctx* c, int nz, int nyloc, int nxloc, double dz, double dz1, double dz2,
+                                   const double* d_rho, const double* d_T, double* d_U, double* d_V, double* d_A,
+                                   int mode, oco_diag* diag)
+{
+    if (!c) return OCO_E_ARG;
+    if (nz < 4 || nyloc < 1 || nxloc < 1 || !d_rho || !d_T || !d_U || !d_V || !d_A) return OCO_E_ARG;
+    /* the difference-form sweep needs no elimination scratch; the drop-in's
+     * lane sweep does (16 B per resident lane and cell) */
+    if (mode != OCO_MODE_MIXED && mode != OCO_MODE_ACCURATE) return OCO_E_ARG;
+    if (c->nw <= 0) return OCO_E_STATE;
+    if (c->lgrey && c->nw != 1) return OCO_E_ARG;   /* source/flux_feautrier.f90:84-87 */
+    if (!c->sscr) return OCO_E_STATE;
+    const long ncol = (long)nxloc * nyloc;
+    const long ncells = ncol * nz;
+    const StaticLayout L = static_layout(nz, ncells, c->lgrey);
+    if (L.total > c->sscr_n) {
+        fprintf(stderr, "ocotillo: oco_solve_block_device_noalloc: %zu bytes of scratch needed, the static scratch "
+                        "holds %zu (at most %ld cells)\n",
+                L.total, c->sscr_n, (long)((c->sscr_n - L.cells) / (CR_NFIELD * sizeof(double))));
+        return OCO_E_ARG;
+    }
+    HIPCHK(hipSetDevice(c->device));
+    const SweepPlan p = sweep_plan(c, nz, ncol, c->nw, mode);
+    if (!p.ok) return OCO_E_ARG;
+    char* base = (char*)c->sscr;
+    unsigned long long* fail = (unsigned long long*)(base + L.fail);
+    double* canary = (double*)(base + L.canary);
+    double* minmax = (double*)(base + L.minmax);
+    double* cells = (double*)(base + L.cells);
+
+    oco_sweep_args a = sweep_args(c, nz, ncol, dz, dz1, dz2, d_rho, d_T);
+    a.U = d_U;
+    a.V = d_V;
+    a.A = d_A;
+    a.cells = cells;
+    a.scratch = nullptr;
+    a.fail = fail;
+    a.canary = canary;
+    a.tickets = (unsigned long long*)(base + L.tickets);
+
+    HIPCHK(hipEventRecord(c->ev[0], c->stream));
+    HIPCHK(oco_launch_fail_reset(fail, 1, c->stream));   /* the last call's key stays otherwise */
+    if (!c->lgrey) HIPCHK(oco_launch_cells(d_rho, d_T, ncells, c->fully_ionized_T, cells, nz, c->stream));
+    HIPCHK(hipEventRecord(c->ev[1], c->stream));
+    HIPCHK(oco_launch_sweep(a, c->lgrey, p.wave, p.lanes, p.pk, p.nblocks, c->stream));   /* resets the tickets */
+    HIPCHK(hipEventRecord(c->ev[2], c->stream));
+    HIPCHK(oco_launch_pillar_minmax(d_U, d_V, d_A, nz, minmax, c->stream));
+
+    unsigned long long key = OCO_FAIL_NONE;
+    double can[4] = {0, 0, 0, 0}, mm[6] = {0, 0, 0, 0, 0, 0};
+    HIPCHK(hipMemcpyAsync(&key, fail, sizeof(key), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(can, canary, sizeof(can), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(mm, minmax, sizeof(mm), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+
+    oco_diag d;
+    memset(&d, 0, sizeof(d));
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, c->ev[0], c->ev[1]) == hipSuccess) d.cells_ms = ms;
+    if (hipEventElapsedTime(&ms, c->ev[1], c->ev[2]) == hipSuccess) d.sweep_ms = ms;
+    if (hipEventElapsedTime(&ms, c->ev[0], c->ev[2]) == hipSuccess) d.total_ms = ms;
+    d.sum_a = can[0];
+    d.sum_b = can[1];
+    d.sum_c = can[2];
+    d.sum_d = can[3];
+    if (key != OCO_FAIL_NONE) {
+        decode_key(key, c->nw, nyloc, &d);
+    } else {
+        d.minU = mm[0]; d.maxU = mm[1];
+        d.minV = mm[2]; d.maxV = mm[3];
+        d.minA = mm[4]; d.maxA = mm[5];
     }
     if (diag) *diag = d;
     return d.status;

@@ -6,6 +6,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "../../include/ocotillo.h"
+
 struct oco_sweep_args {
     int nz;              /* cells per column */
     int nw;              /* wavelengths */
@@ -178,6 +180,14 @@ void oco_build_wtab(const std::vector<double>& waves, std::vector<double>& wt);
 int oco_ctx_device(const oco_ctx* c);
 double oco_ctx_fully_ionized_T(const oco_ctx* c);
 #endif
+
+/* The allocation-free block solve's scratch: this many bytes per GPU in a
+ * file-scope device array; oco_static_scratch gives the calling device's
+ * address (looked up once per device, with that device current). */
+#define OCO_STATIC_SCRATCH_BYTES ((size_t)OCO_STATIC_SCRATCH_MIB << 20)
+int oco_static_scratch(int device, void** ptr, size_t* bytes);
+hipError_t oco_launch_pillar_minmax(const double* U, const double* V, const double* A, int nz, double* out,
+                                    hipStream_t stream);
 
 /* failure key layout: system index << 32 | stage << 28 | row */
 #define OCO_FAIL_NONE 0xffffffffffffffffULL

@@ -23,6 +23,8 @@
  * The FP64 path is compiled with -ffp-contract=off and its elementary
  * functions are oco_math.h's, read from an LDS copy of their tables.
  */
+#include <mutex>
+
 #include "oco_internal.h"
 #include "oco_physics.h"
 #include "oco_mixed.h"
@@ -2298,5 +2300,62 @@ hipError_t oco_launch_sweep(const oco_sweep_args& a, int lgrey, int wave, int la
     } else {
         hipLaunchKernelGGL((sweep_fp64<false, false>), g, bf, 0, stream, a);
     }
+    return hipGetLastError();
+}
+
+/* ---- the allocation-free block solve (oco_solve_block_device_noalloc) ----
+ * Its scratch -- the work-queue tickets, the failure key, the canary sums,
+ * the first pillar's min/max and the field-major cell records -- lives in
+ * one file-scope device array, `scratch`, of OCO_STATIC_SCRATCH_BYTES: every
+ * GPU the library runs on holds its own copy, at its own address, until the
+ * process ends.  Host code never passes the array's name to a kernel (on
+ * the host it is a stand-in, not a device address): oco_static_scratch
+ * looks up each device's address once, with that device current, and the
+ * kernels take that pointer like any other buffer. */
+static __device__ __align__(256) unsigned char scratch[OCO_STATIC_SCRATCH_BYTES];
+
+#define OCO_MAX_DEVICES 64
+static void* scratch_dev[OCO_MAX_DEVICES];
+static std::mutex scratch_mu;
+
+int oco_static_scratch(int device, void** ptr, size_t* bytes)
+{
+    if (device < 0 || device >= OCO_MAX_DEVICES) return OCO_E_ARG;
+    std::lock_guard<std::mutex> lk(scratch_mu);
+    if (!scratch_dev[device]) {
+        int cur = -1;
+        if (hipGetDevice(&cur) != hipSuccess || cur != device) return OCO_E_HIP;   /* the device must be current */
+        void* p = nullptr;
+        if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) {
+            (void)hipGetLastError();
+            return OCO_E_HIP;
+        }
+        scratch_dev[device] = p;
+    }
+    *ptr = scratch_dev[device];
+    *bytes = OCO_STATIC_SCRATCH_BYTES;
+    return OCO_OK;
+}
+
+/* min/max of the first pillar's U (interior), V and alpha
+ * (source/flux_feautrier.f90:222-224) into out[6]: minU, maxU, minV, maxV,
+ * minA, maxA; one thread (nz is at most a few thousand). */
+__global__ void pillar_minmax(const double* U, const double* V, const double* A, int nz, double* out)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    double mu = U[3], xu = U[3], mv = V[0], xv = V[0], ma = A[0], xa = A[0];
+    for (int i = 0; i < nz; i++) {
+        mu = fmin(mu, U[3 + i]); xu = fmax(xu, U[3 + i]);
+        mv = fmin(mv, V[i]); xv = fmax(xv, V[i]);
+        ma = fmin(ma, A[i]); xa = fmax(xa, A[i]);
+    }
+    out[0] = mu; out[1] = xu; out[2] = mv; out[3] = xv; out[4] = ma; out[5] = xa;
+}
+
+hipError_t oco_launch_pillar_minmax(const double* U, const double* V, const double* A, int nz, double* out,
+                                    hipStream_t stream)
+{
+    (void)hipGetLastError();
+    hipLaunchKernelGGL(pillar_minmax, dim3(1), dim3(64), 0, stream, U, V, A, nz, out);
     return hipGetLastError();
 }

@@ -1,0 +1,141 @@
+"""The allocation-free block solve (oco_solve_block_device_noalloc) through
+the C ABI: its results equal oco_solve_block_device's bit for bit, on two
+calls in a row (the second on a smaller block, so it runs on the first's
+leftover scratch), in both difference-form modes; a context whose HBM budget
+holds only its wavelength table cannot run the allocating entry point but
+runs this one; FP64 mode and blocks beyond the static scratch are refused
+with OCO_E_ARG before anything is launched."""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+class DeviceArray:
+    """A device buffer from the HIP runtime the library itself runs on
+    (libamdhip64.so.7 of /opt/rocm, already loaded by the library), so the
+    test needs no other GPU runtime: hipMalloc / hipMemcpy / hipFree by
+    ctypes.  The test allocates its inputs and outputs; the entry point under
+    test allocates nothing."""
+    _hip = None
+
+    def __init__(self, nbytes):
+        import ctypes as C
+        import ocotillo_amd as oco
+        oco.load_library()
+        if DeviceArray._hip is None:
+            h = C.CDLL("libamdhip64.so.7")
+            h.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+            h.hipFree.argtypes = [C.c_void_p]
+            h.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+            h.hipDeviceSynchronize.argtypes = []
+            DeviceArray._hip = h
+        self.n = int(nbytes)
+        p = C.c_void_p()
+        assert DeviceArray._hip.hipMalloc(C.byref(p), self.n) == 0
+        self.ptr = p.value
+
+    def upload(self, a):
+        import ctypes as C
+        a = np.ascontiguousarray(a)
+        assert a.nbytes == self.n
+        assert DeviceArray._hip.hipMemcpy(C.c_void_p(self.ptr), a.ctypes.data_as(C.c_void_p), self.n, 1) == 0
+        return self
+
+    def download(self, shape):
+        import ctypes as C
+        out = np.empty(shape, dtype=np.float64)
+        assert out.nbytes == self.n
+        assert DeviceArray._hip.hipMemcpy(out.ctypes.data_as(C.c_void_p), C.c_void_p(self.ptr), self.n, 2) == 0
+        return out
+
+    def free(self):
+        import ctypes as C
+        if self.ptr:
+            DeviceArray._hip.hipFree(C.c_void_p(self.ptr))
+            self.ptr = None
+
+
+def blocks():
+    from ocotillo_amd.workloads import config4
+    rho, T, dz, waves = config4(nx=64, ny=64, nz=512, nw=48)
+    first = (rho[8:24, 8:24].copy(), T[8:24, 8:24].copy(), dz)      # 16 x 16 columns of 512 cells
+    rho2, T2, dz2, _ = config4(nx=32, ny=32, nz=300, nw=48, seed=7)
+    second = (rho2[3:9, 4:9].copy(), T2[3:9, 4:9].copy(), dz2)     # 6 x 5 columns of 300 cells
+    return [first, second], waves
+
+
+def device_solve(ctx, rho, T, dz, nw, mode, noalloc):
+    nx, ny, nz = rho.shape
+    shapes = {"U": (nw, nx, ny, nz + 6), "V": (nw, nx, ny, nz), "A": (nw, nx, ny, nz)}
+    d_rho = DeviceArray(rho.nbytes).upload(rho)
+    d_T = DeviceArray(T.nbytes).upload(T)
+    out = {k: DeviceArray(8 * int(np.prod(v))).upload(np.full(v, np.nan)) for k, v in shapes.items()}
+    try:
+        d = ctx.solve_block_device(nz, ny, nx, dz, d_rho.ptr, d_T.ptr, out["U"].ptr, out["V"].ptr, out["A"].ptr,
+                                   mode=mode, noalloc=noalloc)
+        return out["U"].download(shapes["U"]), out["V"].download(shapes["V"]), out["A"].download(shapes["A"]), d
+    finally:
+        for x in [d_rho, d_T] + list(out.values()):
+            x.free()
+
+
+@pytest.mark.parametrize("mode_name", ["mixed", "accurate"])
+def test_noalloc_equals_allocating_entry(mode_name):
+    import ocotillo_amd as oco
+    mode = oco.MODE_MIXED if mode_name == "mixed" else oco.MODE_ACCURATE
+    bl, waves = blocks()
+    with oco.Context(0) as a, oco.Context(0) as b:
+        for c in (a, b):
+            c.set_wavelengths(waves)
+            c.set_physics(2e4, False, 0.4)
+        for i, (rho, T, dz) in enumerate(bl):
+            print("case", mode_name, "call", i, rho.shape, flush=True)
+            U0, V0, A0, d0 = device_solve(a, rho, T, dz, waves.size, mode, noalloc=False)
+            U1, V1, A1, d1 = device_solve(b, rho, T, dz, waves.size, mode, noalloc=True)
+            assert d0.status == 0 and d1.status == 0
+            assert np.isfinite(U1).all() and np.isfinite(V1).all() and np.isfinite(A1).all()
+            assert np.array_equal(U1, U0) and np.array_equal(V1, V0) and np.array_equal(A1, A0)
+            for f in ("sum_a", "sum_b", "sum_c", "sum_d", "minU", "maxU", "minV", "maxV", "minA", "maxA"):
+                assert getattr(d1, f) == getattr(d0, f), f
+
+
+def test_noalloc_within_a_budget_that_allows_no_scratch():
+    """A context allowed 1 MiB of device memory (its wavelength table fits,
+    the block's 22 MB of cell records do not)."""
+    import ocotillo_amd as oco
+    bl, waves = blocks()
+    rho, T, dz = bl[0]
+    with oco.Context(0) as ref, oco.Context(0, hbm_budget=1 << 20) as small:
+        for c in (ref, small):
+            c.set_wavelengths(waves)
+            c.set_physics(2e4, False, 0.4)
+        U0, V0, A0, _ = device_solve(ref, rho, T, dz, waves.size, oco.MODE_MIXED, noalloc=False)
+        with pytest.raises(oco.OcoError) as e:
+            device_solve(small, rho, T, dz, waves.size, oco.MODE_MIXED, noalloc=False)
+        assert e.value.status == oco.E_OOM
+        U1, V1, A1, d1 = device_solve(small, rho, T, dz, waves.size, oco.MODE_MIXED, noalloc=True)
+        assert d1.status == 0
+        assert np.array_equal(U1, U0) and np.array_equal(V1, V0) and np.array_equal(A1, A0)
+
+
+def test_noalloc_refusals():
+    """FP64 mode, and a block beyond the static scratch (64 x 64 columns of
+    400 cells: 1.64M cells, 275 MB of records), are OCO_E_ARG; the host
+    checks them before any launch (the small buffers passed are never
+    touched)."""
+    import ocotillo_amd as oco
+    _, waves = blocks()
+    with oco.Context(0) as c:
+        c.set_wavelengths(waves)
+        c.set_physics(2e4, False, 0.4)
+        buf = DeviceArray(16 * 8).upload(np.zeros(16))
+        try:
+            p = buf.ptr
+            for nz, ny, nx, mode in ((512, 4, 4, oco.MODE_FP64), (400, 64, 64, oco.MODE_MIXED)):
+                with pytest.raises(oco.OcoError) as e:
+                    c.solve_block_device(nz, ny, nx, 1e13, p, p, p, p, p, mode=mode, noalloc=True)
+                assert e.value.status == oco.E_ARG
+            assert np.array_equal(buf.download((16,)), np.zeros(16))
+        finally:
+            buf.free()
