@@ -91,11 +91,15 @@ def point_weights(mode, fion):
 
 
 def roofline_of(kernel, points, w_pt, kernel_ms, traffic=None):
-    """VALU roofline of one launch: frozen lane-ops over the launch's device
-    time; TFLOP/s counts each lane-op at the FMA rate of the FP64 peak."""
+    """VALU issue-slot roofline of one launch: the algorithm's frozen
+    lane-operations over the launch's device time, against one FP64 VALU
+    operation per lane per cycle (256 CUs x 64 lanes x 2.4 GHz = 3.93e13
+    lane-op/s; 78.6 TFLOP/s counts each of those slots as an FMA, two FLOPs,
+    so this is not a FLOP rate)."""
     lane_ops = points * w_pt / (kernel_ms * 1e-3)
-    return {"kernel": kernel, "bound": "valu", "achieved": round(2 * lane_ops / 1e12, 3),
-            "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": round(lane_ops / FP64_LANE_OPS, 4),
+    return {"kernel": kernel, "bound": "valu", "achieved": round(lane_ops / 1e12, 3),
+            "peak": round(FP64_LANE_OPS / 1e12, 3), "unit": "Tlane-op/s (FP64 VALU issue slots)",
+            "frac": round(lane_ops / FP64_LANE_OPS, 4),
             "traffic": traffic, "kernel_ms": round(kernel_ms, 4), "ops_per_point": round(w_pt, 1)}
 
 

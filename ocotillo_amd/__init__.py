@@ -183,6 +183,7 @@ class Context:
         w = np.ascontiguousarray(waves_angstrom, dtype=np.float64)
         st = self.L.oco_set_wavelengths(self.h, w.size, _ptr(w))
         if st != OK:
+            self.nw = 0   # the library holds no grid after a failed call
             raise OcoError(st, "oco_set_wavelengths: " + strerror(st))
         self.nw = w.size
 

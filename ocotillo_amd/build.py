@@ -19,10 +19,33 @@ ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 HIPCC = shutil.which("hipcc") or os.path.join(ROCM, "bin", "hipcc")
 CXX = os.path.join(ROCM, "llvm", "bin", "clang++")
 # -ffp-contract=off: every FP64 operation rounds exactly as the reference's;
-# packed FP32 instructions are disabled for all device code.
+# packed FP32 instructions are disabled for all device code (the device pass
+# takes the feature; the host pass warns that it does not know it), and
+# check_no_packed_fp32 verifies that the built code object holds none.
 COMMON = ["-O3", "-fPIC", "-std=c++17", "-ffp-contract=off", "-Wall", "-Wno-unused-function"]
 DEVICE = ["--offload-arch=" + ARCH, "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
           "-munsafe-fp-atomics"]
+PACKED_FP32 = ("v_pk_fma_f32", "v_pk_mul_f32", "v_pk_add_f32")
+
+
+def check_no_packed_fp32(obj):
+    """Disassemble the gfx950 code object this build put into `obj` (its
+    .hip_fatbin section's offload bundle) and fail if it holds any packed FP32
+    arithmetic instruction."""
+    llvm = os.path.join(ROCM, "llvm", "bin")
+    fat, co = obj + ".fatbin", obj + ".co"
+    _run(["objcopy", "-O", "binary", "--only-section=.hip_fatbin", obj, fat])
+    _run([os.path.join(llvm, "clang-offload-bundler"), "--unbundle", "--type=o",
+          "--targets=hipv4-amdgcn-amd-amdhsa--" + ARCH, "--input=" + fat, "--output=" + co])
+    dis = subprocess.run([os.path.join(llvm, "llvm-objdump"), "-d", co], check=True, capture_output=True,
+                         text=True).stdout
+    os.remove(fat)
+    os.remove(co)
+    found = {p: dis.count(p) for p in PACKED_FP32 if p in dis}
+    if found:
+        raise RuntimeError("packed FP32 instructions in %s: %s" % (obj, found))
+    if "v_fma_f64" not in dis:
+        raise RuntimeError("no device code found in %s" % obj)
 
 
 def _run(cmd, cwd=None):
@@ -44,6 +67,8 @@ def build_lib(out=LIB, defines=(), build_dir=BUILD):
             cmd = [CXX] + COMMON + ["-pthread", "-D__HIP_PLATFORM_AMD__", "-I" + os.path.join(ROCM, "include"),
                                     "-c", os.path.join(CSRC, src), "-o", obj]
         _run(cmd)
+        if device:
+            check_no_packed_fp32(obj)
         objs.append(obj)
     tmp = out + ".tmp"
     _run([HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-pthread", "-o", tmp] + objs)

@@ -262,13 +262,22 @@ int oco_set_wavelengths(oco_ctx* c, int nw, const double* waves)
     for (int i = 0; i < nw; i++)
         if (!(waves[i] > 0.0) || !std::isfinite(waves[i])) return OCO_E_ARG;
     HIPCHK(hipSetDevice(c->device));
-    c->waves.assign(waves, waves + nw);
-    std::vector<double> wt;
-    oco_build_wtab(c->waves, wt);
+    std::vector<double> wv(waves, waves + nw), wt;
+    oco_build_wtab(wv, wt);
+    /* until the new table is on the device the context holds no grid, so a
+     * failure below leaves later solves returning OCO_E_STATE (ensure frees
+     * the old table before it allocates the new one) */
+    c->nw = 0;
+    c->waves.clear();
     int st = ensure(c, c->wtab, wt.size() * sizeof(double));
     if (st) return st;
-    HIPCHK(hipMemcpyAsync(c->wtab.p, wt.data(), wt.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
-    HIPCHK(hipStreamSynchronize(c->stream));
+    hipError_t e = hipMemcpyAsync(c->wtab.p, wt.data(), wt.size() * sizeof(double), hipMemcpyHostToDevice, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    if (e != hipSuccess) {
+        fprintf(stderr, "ocotillo: oco_set_wavelengths: %s\n", hipGetErrorString(e));
+        return OCO_E_HIP;
+    }
+    c->waves = std::move(wv);
     c->nw = nw;
     return OCO_OK;
 }

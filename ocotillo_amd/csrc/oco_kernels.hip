@@ -1340,11 +1340,23 @@ __device__ static inline void solve_system_wave(const oco_sys& d, double* lds, b
                                           om[0], om[bs], om[2 * bs], sb[0], sb[1], sb[2], dz, dz2);
         out_store(d.V + bz, flux_v(al[0], ll == 0 ? df : -df, dz1));
     }
-    for (int z = ll + 1; z < nz - 1; z += L) {
-        const double* u = LS + z + 3;
-        out_store(d.V + z, flux_v(LA[z], der7(u[-3], u[-2], u[-1], u[1], u[2], u[3]), dz1));
+    /* a lane's cells z = ll + 1 + k L, k < MM (nz - 2 <= MM L): straight-line
+     * code, every LDS read from an index clamped into the system, so all of
+     * them are issued before the first result is needed */
+#pragma unroll
+    for (int k = 0; k < MM; k++) {
+        const int z = ll + 1 + k * L;
+        const double* u = LS + (z < nz - 1 ? z : nz - 2) + 3;
+        const double v = flux_v(u[-3 - 2 * nz], der7(u[-3], u[-2], u[-1], u[1], u[2], u[3]), dz1);
+        if (z < nz - 1) out_store(d.V + z, v);
     }
-    for (int e = ll; e < nz + 6; e += L) out_store(d.U + e, LS[e]);
+    /* U with ghosts: e = ll + k L, k <= MM (nz + 6 <= (MM + 1) L) */
+#pragma unroll
+    for (int k = 0; k <= MM; k++) {
+        const int e = ll + k * L;
+        const double u = LS[e < nz + 6 ? e : nz + 5];
+        if (e < nz + 6) out_store(d.U + e, u);
+    }
 }
 
 /* The difference-form sweep of the block (OCO_MODE_MIXED, OCO_MODE_ACCURATE):

@@ -101,6 +101,7 @@ struct Job {
     int slot = -1;
     size_t off = 0, bytes = 0;
     const double* d_wtab = nullptr;
+    int table = -1;   /* index of its wavelength table in oco_server::tables while admitted */
     double* d_em = nullptr;
     hipEvent_t ready = nullptr, copied = nullptr;
     JobState state = QUEUED;
@@ -118,7 +119,22 @@ struct Iter {
 struct Table {
     std::vector<double> waves;
     size_t off = 0, bytes = 0;
+    int users = 0;    /* admitted jobs on this grid */
+    bool live = false;
 };
+
+/* Arena bytes of a job's own buffers: rho and T, cell records (mixed:
+ * the serving kernel's chunk-major layout), emergent flux. */
+size_t job_bytes(bool mixed, long ncol, int nz, long nsys)
+{
+    const size_t ncells = (size_t)ncol * nz;
+    const size_t b_in = ncells * sizeof(double);
+    const size_t b_cells = (mixed ? oco_cells_chunked_doubles(ncol, nz) : ncells * OCO_CELL_STRIDE) * sizeof(double);
+    const size_t b_em = (size_t)nsys * 2 * sizeof(double);
+    return Arena::round(b_in) * 2 + Arena::round(b_cells) + Arena::round(b_em);
+}
+
+size_t table_bytes(int nw) { return Arena::round((size_t)nw * OCO_WT_STRIDE * sizeof(double)); }
 
 }  // namespace
 
@@ -160,36 +176,63 @@ struct oco_server {
     clk::time_point t_first, t_last;
     int buf = 0;
 
-    void loop();
+    bool copier_busy = false;                  /* the copy thread holds a task outside the lock */
+    std::condition_variable cv_copier_idle;
+
+    void loop(std::unique_lock<std::mutex>& lk);
     bool admit_one(Job* j);
-    const double* table_for(const double* waves, int nw);
+    int table_for(const double* waves, int nw);
+    void drop_unused_tables();
     bool launch_iteration();
     void retire();
     void finish_copies();
-    void fail_all(int st);
+    void fail_all(int st, std::unique_lock<std::mutex>& lk);
     void copy_loop();
 };
 
-const double* oco_server::table_for(const double* waves, int nw)
+/* index of the resident table of this wavelength grid, built in the arena
+ * if it is not there yet; -1 if it does not fit */
+int oco_server::table_for(const double* waves, int nw)
 {
-    for (auto& t : tables)
-        if ((int)t.waves.size() == nw && std::equal(t.waves.begin(), t.waves.end(), waves))
-            return (const double*)(arena.base + t.off);
+    for (size_t i = 0; i < tables.size(); i++) {
+        const Table& t = tables[i];
+        if (t.live && (int)t.waves.size() == nw && std::equal(t.waves.begin(), t.waves.end(), waves)) return (int)i;
+    }
     Table t;
     t.waves.assign(waves, waves + nw);
     std::vector<double> wt;
     oco_build_wtab(t.waves, wt);
     t.bytes = wt.size() * sizeof(double);
-    if (!arena.alloc(t.bytes, t.off)) return nullptr;
+    if (!arena.alloc(t.bytes, t.off)) return -1;
     if (hipMemcpy(arena.base + t.off, wt.data(), t.bytes, hipMemcpyHostToDevice) != hipSuccess) {
+        (void)hipGetLastError();
         arena.release(t.off, t.bytes);
-        return nullptr;
+        return -1;
     }
+    t.live = true;
+    for (size_t i = 0; i < tables.size(); i++)
+        if (!tables[i].live) {
+            tables[i] = std::move(t);
+            return (int)i;
+        }
     tables.push_back(std::move(t));
-    return (const double*)(arena.base + tables.back().off);
+    return (int)tables.size() - 1;
 }
 
-/* called with mu held; false if the job does not fit yet */
+/* free the resident tables no admitted job uses (their grids are built
+ * again when a later job needs them); mu held */
+void oco_server::drop_unused_tables()
+{
+    for (Table& t : tables)
+        if (t.live && t.users == 0) {
+            arena.release(t.off, t.bytes);
+            t.live = false;
+            t.waves.clear();
+        }
+}
+
+/* called with mu held; false if the job does not fit yet (its table and
+ * buffers: first with the resident tables, then with the unused ones freed) */
 bool oco_server::admit_one(Job* j)
 {
     if (free_slots.empty()) return false;
@@ -198,11 +241,18 @@ bool oco_server::admit_one(Job* j)
     const size_t b_in = (size_t)ncells * sizeof(double);
     const size_t b_cells = (mixed ? oco_cells_chunked_doubles(j->ncol, d.nz) : (size_t)ncells * OCO_CELL_STRIDE) *
                            sizeof(double);
-    const size_t b_em = (size_t)j->nsys * 2 * sizeof(double);
-    j->bytes = Arena::round(b_in) * 2 + Arena::round(b_cells) + Arena::round(b_em);
-    const double* wt = table_for(d.waves, d.nw);
-    if (!wt) return false;
-    if (!arena.alloc(j->bytes, j->off)) return false;
+    j->bytes = job_bytes(mixed, j->ncol, d.nz, j->nsys);
+    int ti = table_for(d.waves, d.nw);
+    if (ti < 0 || !arena.alloc(j->bytes, j->off)) {
+        /* free the tables of grids no admitted job uses (this job's own if it
+         * was just built: it is built again) and try once more */
+        drop_unused_tables();
+        ti = table_for(d.waves, d.nw);
+        if (ti < 0 || !arena.alloc(j->bytes, j->off)) return false;
+    }
+    tables[ti].users++;
+    j->table = ti;
+    const double* wt = (const double*)(arena.base + tables[ti].off);
     j->slot = free_slots.back();
     free_slots.pop_back();
     j->d_wtab = wt;
@@ -348,6 +398,8 @@ void oco_server::finish_copies()
             t_last = j->t_done;
             completed++;
             arena.release(j->off, j->bytes);
+            if (j->table >= 0) tables[j->table].users--;
+            j->table = -1;
             free_slots.push_back(j->slot);
             j->slot = -1;
             j->state = DONE;
@@ -359,13 +411,30 @@ void oco_server::finish_copies()
     }
 }
 
-void oco_server::fail_all(int st)
+/* A HIP error: every unfinished job fails with st.  Before any is marked
+ * done (after which its waiter deletes it), the copy thread lets go of the
+ * task it holds, the tasks not started are dropped, and both streams are
+ * drained, so no launch, copy or thread still reads a job. */
+void oco_server::fail_all(int st, std::unique_lock<std::mutex>& lk)
 {
+    copy_tasks.clear();
+    cv_copier_idle.wait(lk, [&] { return !copier_busy; });
+    (void)hipStreamSynchronize(cs);
+    (void)hipStreamSynchronize(xs);
+    (void)hipGetLastError();
+    for (Iter& it : inflight) (void)hipEventDestroy(it.done);
+    inflight.clear();
     for (auto& kv : all)
         if (kv.second->state != DONE) {
             kv.second->res.status = st;
             kv.second->state = DONE;
         }
+    for (Job* j : active) {
+        if (j->table >= 0) tables[j->table].users--;
+        j->table = -1;
+        if (j->slot >= 0) arena.release(j->off, j->bytes), free_slots.push_back(j->slot);
+        j->slot = -1;
+    }
     queue.clear();
     active.clear();
     cv_done.notify_all();
@@ -382,6 +451,7 @@ void oco_server::copy_loop()
         if (copy_tasks.empty()) return;
         auto task = copy_tasks.front();
         copy_tasks.pop_front();
+        copier_busy = true;
         Job* j = task.first;
         const oco_job d = j->desc;
         const long ncells = j->ncol * d.nz;
@@ -412,26 +482,42 @@ void oco_server::copy_loop()
         if (e != hipSuccess) fatal = OCO_E_HIP;
         if (task.second == 0) j->loaded = true;
         else j->copy_issued = true;
+        copier_busy = false;
+        cv_copier_idle.notify_all();
         cv_work.notify_one();
     }
 }
 
-void oco_server::loop()
+void oco_server::loop(std::unique_lock<std::mutex>& lk)
 {
-    (void)hipSetDevice(device);
-    std::unique_lock<std::mutex> lk(mu);
     for (;;) {
         if (fatal) {
-            fail_all(fatal);
+            fail_all(fatal, lk);
             if (stop) return;
             cv_work.wait(lk);
             continue;
         }
         retire();
         finish_copies();
-        while (!queue.empty() && admit_one(queue.front())) {
-            active.push_back(queue.front());
-            queue.pop_front();
+        while (!queue.empty()) {
+            Job* h = queue.front();
+            if (admit_one(h)) {
+                active.push_back(h);
+                queue.pop_front();
+                continue;
+            }
+            if (active.empty() && inflight.empty() && free_slots.size() == MAX_SLOTS) {
+                /* the arena holds no job and no used table, and the job still
+                 * does not fit: it never will */
+                h->res.status = OCO_E_OOM;
+                h->t_admit = h->t_done = clk::now();
+                h->res.queue_ms = h->res.latency_ms = 1e3 * secs(h->t_submit, h->t_done);
+                h->state = DONE;
+                queue.pop_front();
+                cv_done.notify_all();
+                continue;
+            }
+            break;
         }
         bool launched = false;
         if (inflight.size() < 2) launched = launch_iteration();
@@ -490,7 +576,11 @@ int oco_server_create(oco_ctx* ctx, size_t hbm_budget, int max_nz, int mode, oco
     s->jobs.assign(MAX_SLOTS, oco_serve_job{});
     for (int i = MAX_SLOTS - 1; i >= 0; i--) s->free_slots.push_back(i);
     s->copier = std::thread([s] { s->copy_loop(); });
-    s->th = std::thread([s] { s->loop(); });
+    s->th = std::thread([s] {
+        (void)hipSetDevice(s->device);
+        std::unique_lock<std::mutex> lk(s->mu);
+        s->loop(lk);
+    });
     *out = s;
     return OCO_OK;
 }
@@ -511,10 +601,10 @@ int oco_server_submit(oco_server* s, const oco_job* job, long long* ticket)
     j->nsys = j->ncol * d.nw;
     j->nblk = (d.nw + 63) / 64;
     j->ntiles = j->ncol * j->nblk;
-    const size_t need = Arena::round((size_t)j->ncol * d.nz * sizeof(double)) * 2 +
-                        Arena::round((size_t)j->ncol * d.nz * OCO_CELL_STRIDE * sizeof(double)) +
-                        Arena::round((size_t)j->nsys * 2 * sizeof(double)) +
-                        Arena::round((size_t)d.nw * OCO_WT_STRIDE * sizeof(double));
+    /* what admission needs: the job's buffers and its grid's table (a job
+     * that passes this can still be refused at admission by the first-fit
+     * arena's fragmentation, with OCO_E_OOM through oco_server_wait) */
+    const size_t need = job_bytes(s->mixed, j->ncol, d.nz, j->nsys) + table_bytes(d.nw);
     if (need > s->arena.size) {
         delete j;
         return OCO_E_OOM;

@@ -152,6 +152,12 @@ def run_dump(name, nz, nw, ncol, kind, waves, lgrey=0, full=True, fion=2e4, sigm
     if kind == "c1":
         rho = np.maximum(1e-9 * np.exp(-z ** 2 / (2 * AU ** 2)), 1e-24)[None, :].repeat(ncol, 0)
         T = np.full((ncol, nz), 2000.0)
+    elif kind == "pivot1":
+        # grey: the second column's bottom cell at rho = 1e-170 g/cm^3 has
+        # alpha = 4e-171 /cm, alpha^2 underflows to 0, so b(1) = 0
+        rho = np.maximum(1e-9 * np.exp(-z ** 2 / (2 * AU ** 2)), 1e-24)[None, :].repeat(ncol, 0)
+        rho[1, 0] = 1e-170
+        T = np.full((ncol, nz), 2000.0)
     else:
         rho, T = column_profiles(rng, ncol, z, kind, rho_floor)
     subprocess.check_call([BUILD, "dump", str(nz), str(nw)], stdout=subprocess.DEVNULL)
@@ -207,6 +213,8 @@ def logwaves(nw, w0=100.0, w1=1e5):
 
 
 def main():
+    if len(sys.argv) > 1 and sys.argv[1] == "pivot1":
+        return run_pivot1()
     copy_samples()
     run_samples()
     # C1 exactly (grey isothermal), plus random grey columns
@@ -224,6 +232,13 @@ def main():
     run_dump("pivot0", 256, 32, 2, "c2", logwaves(32), full=False, rho_floor=1e-24)
     run_dump("uneven", 257, 3, 4, "c2", [3000.0, 5000.0, 1e4], full=False)
     run_dump("long", 1024, 4, 2, "c2", logwaves(4, 500.0, 2e4), full=False)
+    run_pivot1()
+
+
+def run_pivot1():
+    """alpha^2 underflowing at the boundary row: the reference's first pivot
+    b(1) is exactly zero and it stops in stage 1 (source/auxiliary.f90:27-30)."""
+    run_dump("pivot1", 16, 1, 3, "pivot1", [3000.0], lgrey=1, full=False)
 
 
 if __name__ == "__main__":

@@ -62,11 +62,14 @@ def test_golden_dump(ctx, name):
     assert flux_err(out["V"], g["V"]).max() <= TOL
 
 
-def test_zero_pivot_reported_like_reference(ctx):
-    g = golden("dump_pivot0.npz")
+@pytest.mark.parametrize("name", ["dump_pivot0.npz", "dump_pivot1.npz"])
+def test_zero_pivot_reported_like_reference(ctx, name):
+    """The reference's stops: stage 2 on the thin floor (dump_pivot0), stage 1
+    where alpha^2 underflows at the boundary row (dump_pivot1, grey)."""
+    g = golden(name)
     ncol, nz = int(g["ncol"]), int(g["nz"])
     ctx.set_wavelengths(g["waves"])
-    ctx.set_physics(float(g["fully_ionized_T"]), False, 0.4)
+    ctx.set_physics(float(g["fully_ionized_T"]), bool(g["lgrey"]), float(g["sigma_grey"]))
     _, _, _, d = ctx.solve_block(g["rho"].reshape(1, ncol, nz), g["T"].reshape(1, ncol, nz), float(g["dz"]),
                                  raise_on_pivot=False)
     assert d.status == int(g["ref_stop_stage"])

@@ -187,3 +187,44 @@ def test_pivot_reported_per_job(ctx):
     assert r_ok.status == 0
     assert r_bad.status == int(g["ref_stop_stage"])
     assert (r_bad.fail_ix, r_bad.fail_iy, r_bad.fail_iw) == (1, int(g["ref_stop_col"]), int(g["ref_stop_iw"]))
+
+
+def test_tight_budget_refused_at_submit(ctx):
+    """A job whose admission needs more than the budget is refused at
+    submit, sized as admission sizes it: 64 columns of 64 cells in mixed mode
+    take 3.4 MB of cell records in the serving layout (5 rows per lane), so a
+    1 MiB budget refuses them (and does not queue a job that could never be
+    admitted)."""
+    import ocotillo_amd as oco
+    with oco.Server(ctx, 1 << 20, max_nz=256, mode=oco.MODE_MIXED) as srv:
+        with pytest.raises(oco.OcoError) as e:
+            srv.submit(*make_job(31, 8, 8, 64, 16))
+        assert e.value.status == oco.E_OOM
+
+
+def test_two_grids_in_a_one_job_budget(ctx):
+    """Two jobs on different wavelength grids in a budget that holds one job
+    and one grid's table (2 x 2 columns, nz 256, nw 2048: 319 KB of buffers
+    and a 377 KB table each, 1 MiB budget): the second waits for the first,
+    whose table is then freed to make room for its own; both complete with
+    the bits they have alone."""
+    import ocotillo_amd as oco
+    from ocotillo_amd.workloads import disk_columns, log_waves
+    jobs = []
+    for seed, w0 in ((41, 100.0), (42, 120.0)):
+        rho, T, dz = disk_columns(2, 2, 256, seed=seed)
+        jobs.append((rho, T, dz, log_waves(2048, w0, 1e5)))
+    alone = []
+    for j in jobs:
+        with oco.Server(ctx, 1 << 30, max_nz=256, mode=oco.MODE_MIXED) as srv:
+            em, r = srv.wait(srv.submit(*j))
+            assert r.status == 0
+            alone.append(em)
+    with oco.Server(ctx, 1 << 20, max_nz=256, mode=oco.MODE_MIXED) as srv:
+        tickets = [srv.submit(*j) for j in jobs]
+        outs = [srv.wait(t) for t in tickets]
+        st = srv.stats()
+    for (em, r), a in zip(outs, alone):
+        assert r.status == 0
+        assert np.array_equal(em, a)
+    assert st.hbm_peak <= st.hbm_budget

@@ -65,13 +65,19 @@ def test_intermediates_bit_exact(name):
             assert np.array_equal([p.S for p in pts], g["S"][col, iw])
 
 
-def test_zero_pivot_matches_reference_stop():
-    """The reference stops on an exactly-zero pivot (source/auxiliary.f90:36-39);
-    the oracle reports the same stage and system."""
-    g = golden("dump_pivot0.npz")
+@pytest.mark.parametrize("name", ["dump_pivot0.npz", "dump_pivot1.npz"])
+def test_zero_pivot_matches_reference_stop(name):
+    """The reference stops on an exactly-zero pivot, in stage 2 on the thin
+    floor of dump_pivot0 (source/auxiliary.f90:36-39) and in stage 1 where
+    alpha^2 underflows at the boundary row of dump_pivot1 (:27-30); the
+    oracle reports the same stage and system."""
+    g = golden(name)
     o = Oracle(g["waves"])
     ncol, nz = int(g["ncol"]), int(g["nz"])
-    _, _, _, d = o.solve_block(g["rho"].reshape(1, ncol, nz), g["T"].reshape(1, ncol, nz), float(g["dz"]))
+    _, _, _, d = o.solve_block(g["rho"].reshape(1, ncol, nz), g["T"].reshape(1, ncol, nz), float(g["dz"]),
+                               lgrey=bool(g["lgrey"]), sigma_grey=float(g["sigma_grey"]))
+    if name == "dump_pivot1.npz":
+        assert d.status == 1 and d.fail_j == 1
     assert d.status == int(g["ref_stop_stage"])
     assert (d.fail_ix, d.fail_iy, d.fail_iw) == (1, int(g["ref_stop_col"]), int(g["ref_stop_iw"]))
     assert "ERROR tridiag stage %d" % d.status in str(g["ref_stdout"])
