@@ -13,8 +13,10 @@ an accuracy pass compares the two modes on every point of the block.
 --config c3: 4096 uneven columns through the batched persistent path.
 
 Prints one JSON line: points/s, the roofline of the dominant kernel (device
-time from HIP events on the library's stream) and a CPU baseline (the FP64
-CPU oracle, OpenMP over the host cores, on a stated sample).
+time from HIP events on the library's stream) and a CPU baseline: the
+reference's own Fortran (compiled -O3 from its sources by
+oracle/ref/build_ref.sh) in one process per host thread, on a stated sample of
+the block.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--config c4|c2|c3]
                     [--mode both|fp64|mixed] [--no-cpu] [--no-accuracy]
@@ -121,8 +123,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-accuracy", action="store_true", help="skip the mixed-vs-FP64 accuracy pass")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving leg of the c4 run")
-    ap.add_argument("--jobs", type=int, default=None, help="serving: jobs after warm-up (c5: 300, c4 leg: 40)")
-    ap.add_argument("--pool", type=int, default=None, help="serving: distinct job shapes (c5: 12, c4 leg: 6)")
+    ap.add_argument("--jobs", type=int, default=None, help="serving: timed jobs after the capacity burst (300)")
     ap.add_argument("--nx", type=int, default=None)
     ap.add_argument("--ny", type=int, default=None)
     ap.add_argument("--nz", type=int, default=None)
@@ -155,9 +156,9 @@ def main():
     if args.config == "files":
         return bench_files(args)
     if args.config == "c5":
-        res = bench_serve(args.jobs or 300, args.pool or 12)
+        res = bench_serve(args.jobs or 300)
         out = {"metric": METRIC, "value": res["p99_ms"], "unit": "ms (p99 latency)", "n_gpus": 1,
-               "steps": res["jobs"], "warmup": args.pool or 12, "ms_per_step": res["mean_ms"],
+               "steps": res["jobs"], "warmup": res["burst_jobs"], "ms_per_step": res["mean_ms"],
                "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "f64+f32",
                "data": "synthetic", "config": {"workload": res["workload"]}, "serving": res,
                "roofline": None, "cpu_baseline": None}
@@ -211,6 +212,7 @@ def bench_block(args):
     ctx.set_physics(2e4, False, 0.4)
     fion = ionised_fraction(rho, T)
     res = {}
+    gpu_sums = None
     codes = {"fp64": oco.MODE_FP64, "mixed": oco.MODE_MIXED, "accurate": oco.MODE_ACCURATE}
     for mode in modes:
         code = codes[mode]
@@ -242,6 +244,10 @@ def bench_block(args):
                     "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4),
                     "bytes_per_point": round(sweep_bytes(nz, ncol, nw) / points, 3)}
         res[mode] = {"value": points / (ms * 1e-3), "ms_per_step": ms, "roofline": roof, "roofline_hbm": roof_hbm}
+        if mode == "fp64":
+            # the CPU baseline's sample columns of the FP64 outputs, for its checksum comparison
+            sx, sy = (min(48, nx), min(48, ny)) if args.config == "c4" else (nx, ny)
+            gpu_sums = [float(x[:, :sx, :sy].sum()) for x in (d_U, d_V, d_A)]
     del d_U, d_V, d_A
     torch.cuda.empty_cache()
     athena = bench_athena(ctx, d_rho, d_T, args.steps)
@@ -252,11 +258,11 @@ def bench_block(args):
 
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline(rho, T, dz, waves, args.config)
+        cpu = cpu_baseline(rho, T, dz, waves, args.config, gpu_sums if "fp64" in modes else None)
     serving = None
     if args.config == "c4" and not args.no_serve:
         del d_rho, d_T
-        serving = bench_serve(args.jobs or 40, args.pool or 6)
+        serving = bench_serve(args.jobs or 300)
 
     head = modes[-1]
     out = {
@@ -520,109 +526,231 @@ def accuracy_pass(ctx, d_rho, d_T, dz, nx, ny, nz, nw, nchunks=4):
     }
 
 
-def cpu_baseline(rho, T, dz, waves, config):
-    """The FP64 CPU oracle (OpenMP) on a bounded sample of the block."""
+def cpu_baseline(rho, T, dz, waves, config, gpu_sums=None):
+    """The reference on the host's cores beside the GPU: its own modules
+    compiled at -O3 (oracle/_ref/time_*/ref_time.x, built by
+    oracle/ref/build_ref.sh), one process per available thread on disjoint
+    column slices of a bounded sample of the block, timed from the first start
+    to the last exit (kind "reference").  Without that build: the FP64 port
+    (the oracle, pinned bit-exact to the reference) with OpenMP over the same
+    threads (kind "port").  gpu_sums: the FP64 GPU outputs' sums of U, V and
+    alpha over the sample's columns, compared with the reference's own."""
+    from oracle import reference as R
     from oracle.oracle import Oracle
-    o = Oracle(waves)
-    nthreads = min(os.cpu_count() or 1, 16)
+    nthreads = R.host_threads()
     if config == "c4":
-        sx, sy = min(64, rho.shape[0]), min(64, rho.shape[1])
+        sx, sy = min(48, rho.shape[0]), min(48, rho.shape[1])
         r, t = rho[:sx, :sy].copy(), T[:sx, :sy].copy()
         sample = "the first %dx%d columns of the block (all nz, all nw), one pass" % (sx, sy)
     else:
         r, t = rho, T
         sample = "the full block, one pass"
     pts = float(r.size) * waves.size
-    tc0 = time.perf_counter()
-    o.solve_block(r, t, dz, nthreads=nthreads)
-    tc = time.perf_counter() - tc0
-    return {"value": round(pts / tc, 1), "unit": "points/s", "cores": nthreads, "kind": "port",
-            "sample": "%s: %.4g points in %.2f s wall" % (sample, pts, tc)}
+    out = {"unit": "points/s", "cores": nthreads, "cpu": R.cpu_model()}
+    if R.binary(r.shape[2], waves.size):
+        wall, lines = R.run(r, t, dz, waves, nthreads)
+        sums = [sum(l[k] for l in lines) for k in (5, 6, 7)]
+        out.update(value=round(pts / wall, 1), kind="reference",
+                   sample="%s: %.4g points, %d processes of the reference (amdflang -O3) on disjoint column "
+                          "slices, %.2f s wall from first start to last exit" % (sample, pts, len(lines), wall))
+        if gpu_sums is not None:
+            out["fp64_gpu_vs_reference_sums"] = {k: float(abs(g - c) / abs(c)) for k, g, c in
+                                                 zip(("U", "V", "alpha"), gpu_sums, sums)}
+    else:
+        o = Oracle(waves)
+        tc0 = time.perf_counter()
+        o.solve_block(r, t, dz, nthreads=nthreads)
+        tc = time.perf_counter() - tc0
+        out.update(value=round(pts / tc, 1), kind="port",
+                   sample="%s: %.4g points in %.2f s wall (OpenMP)" % (sample, pts, tc))
+    return out
 
 
 C5_RATE = 50.0          # jobs/s of the serving workload
 C5_MEAN_POINTS = 5.16e9  # mean points per job of its shape distribution
 
 
-def bench_serve(njobs, pool_size, seed_jobs=True, load=0.7):
+class PinnedRing:
+    """A FIFO arena of pinned host memory for serving inputs: alloc(n) hands
+    out n contiguous doubles after the newest live block (wrapping to the
+    start), blocking while they are in use; free(key) retires a block, and
+    the space behind the oldest live block is reused once every block before
+    it is retired too."""
+
+    def __init__(self, ndoubles):
+        import collections
+        import threading
+        import torch
+        self.buf = torch.empty(ndoubles, dtype=torch.float64, pin_memory=True).numpy()
+        self.cap = ndoubles
+        self.live = collections.deque()     # [start, end, key, retired]
+        self.cv = threading.Condition()
+
+    def alloc(self, n, key):
+        if n > self.cap:
+            raise ValueError("job of %d doubles exceeds the pinned ring (%d)" % (n, self.cap))
+        with self.cv:
+            while True:
+                if not self.live:
+                    at = 0
+                else:
+                    t, h = self.live[0][0], self.live[-1][1]
+                    if h > t:
+                        at = h if self.cap - h >= n else (0 if t >= n else None)
+                    else:
+                        at = h if t - h >= n else None
+                if at is not None:
+                    self.live.append([at, at + n, key, False])
+                    return self.buf[at:at + n]
+                self.cv.wait()
+
+    def free(self, key):
+        with self.cv:
+            for b in self.live:
+                if b[2] == key:
+                    b[3] = True
+                    break
+            while self.live and self.live[0][3]:
+                self.live.popleft()
+            self.cv.notify_all()
+
+
+def serve_inputs(shapes, seed0, noise, ring, workers):
+    """Start writing each job's inputs (serving_job_into, seed seed0 + k) into
+    `ring`, in order, on `workers` threads; returns one future per job whose
+    result is (rho3d, T3d, dz, waves, points).  The ring blocks the
+    producer, so generation runs at most one ring ahead of the consumer."""
+    import concurrent.futures as cf
+    import threading
+    from ocotillo_amd import workloads as W
+    ex = cf.ThreadPoolExecutor(max_workers=workers)
+    futs = [cf.Future() for _ in shapes]
+
+    def fill(k, buf):
+        nx, ny, nz, nw = shapes[k]
+        n = nx * ny * nz
+        rho, T = buf[:n], buf[n:2 * n]
+        dz = W.serving_job_into(rho, T, nx, ny, nz, seed0 + k, noise)
+        return rho.reshape(nx, ny, nz), T.reshape(nx, ny, nz), dz, W.log_waves(nw), float(n * nw)
+
+    def run(k, buf):
+        try:
+            futs[k].set_result(fill(k, buf))
+        except BaseException as e:          # surfaced to the consumer by .result()
+            futs[k].set_exception(e)
+
+    def produce():
+        for k, (nx, ny, nz, nw) in enumerate(shapes):
+            buf = ring.alloc(2 * nx * ny * nz, (seed0, k))
+            ex.submit(run, k, buf)
+        ex.shutdown(wait=False)
+
+    threading.Thread(target=produce, daemon=True).start()
+    return futs
+
+
+def bench_serve(njobs, load=0.7, burst=12, workers=8):
     """Serving (the C5 shapes: nx, ny ~ U{16..256}, nz ~ U{64..2048},
-    nw ~ U{16..512}, mixed precision, emergent-flux outputs): Poisson
-    arrivals into the server.  The stated 50 jobs/s offers ~2.6e11 points/s,
-    above one GPU's capacity, so arrivals run at min(50/s, load x measured
-    capacity / mean job points) and both rates are reported.  Jobs are drawn
-    from a seeded pool of `pool_size` shapes whose inputs sit in pinned host
-    memory (generating ~300 MB of inputs per arrival would bound the host,
-    not the server)."""
+    nw ~ U{16..512}, mixed precision, emergent-flux outputs), every job with
+    its own seeded shape and inputs.  A back-to-back burst of `burst` jobs
+    (inputs ready in pinned memory before the clock starts) measures the
+    server's capacity and warms it; then `njobs` further jobs arrive as a
+    Poisson stream.  The stated 50 jobs/s offers ~2.6e11 points/s, above one
+    GPU's capacity, so arrivals run at min(50/s, load x capacity / mean job
+    points of the timed jobs).  Each job's inputs are generated on host
+    threads just ahead of its arrival, into a pinned ring; the lag between a
+    job's scheduled arrival and its submission (inputs not yet written) is
+    reported, and latency is the server's own, from submission to completion."""
     import queue
     import threading
     import torch
     import ocotillo_amd as oco
     from ocotillo_amd import workloads as W
     torch.cuda.empty_cache()
+    shapes = W.config5_shapes(burst + njobs)
+    b_shapes, t_shapes = shapes[:burst], shapes[burst:]
+    cells = [nx * ny * nz for nx, ny, nz, _ in shapes]
+    t0 = time.perf_counter()
+    noise = W.lognormal_pool(max(cells))
+    ring = PinnedRing(max(1 << 30, 2 * sum(cells[:burst]) + 2 * max(cells)))
+    log("serve: noise pool and %.1f GiB pinned ring in %.1f s" % (ring.cap * 8 / 2 ** 30, time.perf_counter() - t0))
     ctx = oco.Context(0)
     ctx.set_physics(2e4, False, 0.4)
-    shapes = W.config5_shapes(pool_size)
-    pool = []
-    for k, (nx, ny, nz, nw) in enumerate(shapes):
-        rho, T, dz, waves = W.config5_job(nx, ny, nz, nw, seed=W.SEED + 500 + k)
-        pr = torch.empty(rho.shape, dtype=torch.float64, pin_memory=True).numpy()
-        pt = torch.empty(T.shape, dtype=torch.float64, pin_memory=True).numpy()
-        pr[...] = rho
-        pt[...] = T
-        pool.append((pr, pt, dz, waves, float(nx * ny * nz * nw)))
-        log("serve pool job %d: %dx%dx%d, nw=%d (%.3g points)" % (k, nx, ny, nz, nw, pool[-1][4]))
     free, _ = torch.cuda.mem_get_info()
     budget = int(min(200 * 2 ** 30, free - 24 * 2 ** 30))
     with oco.Server(ctx, budget, max_nz=2048, mode=oco.MODE_MIXED) as srv:
-        # capacity (and warm-up): every pool job once, back to back
+        # capacity and warm-up: the burst back to back, inputs generated first
+        bf = serve_inputs(b_shapes, W.SEED + 500, noise, ring, workers)
+        jobs = [f.result() for f in bf]
         t0 = time.perf_counter()
-        tickets = [srv.submit(*j[:4]) for j in pool]
-        for t in tickets:
+        tickets = [srv.submit(*j[:4]) for j in jobs]
+        for k, t in enumerate(tickets):
             srv.wait(t)
-        cap = sum(j[4] for j in pool) / (time.perf_counter() - t0)
-        mean_pts = float(np.mean([j[4] for j in pool]))
+            ring.free((W.SEED + 500, k))
+        cap = sum(j[4] for j in jobs) / (time.perf_counter() - t0)
+        del jobs
+        mean_pts = float(np.mean([nx * ny * nz * nw for nx, ny, nz, nw in t_shapes]))
         rate = min(C5_RATE, load * cap / mean_pts)
-        log("serve capacity %.3g points/s -> arrival rate %.2f jobs/s" % (cap, rate))
+        log("serve capacity %.3g points/s (burst of %d jobs) -> arrival rate %.2f jobs/s" % (cap, burst, rate))
         arrivals = W.poisson_arrivals(njobs, rate)
-        pick = np.random.default_rng(W.SEED + 7).integers(0, pool_size, njobs)
+        seed0 = W.SEED + 1000
+        futs = serve_inputs(t_shapes, seed0, noise, ring, workers)
         q = queue.Queue()
         results = []
 
         def collect():
             while True:
-                t = q.get()
-                if t is None:
+                item = q.get()
+                if item is None:
                     return
+                k, t = item
                 _, r = srv.wait(t)
                 results.append((r.latency_ms, r.queue_ms, r.points, time.perf_counter()))
+                ring.free((seed0, k))
 
         th = threading.Thread(target=collect)
         th.start()
+        lag = []
+        futs[0].result()                   # the stream starts once its first job exists
         t_start = time.perf_counter()
         for k in range(njobs):
             delay = t_start + arrivals[k] - time.perf_counter()
             if delay > 0:
                 time.sleep(delay)
-            q.put(srv.submit(*pool[pick[k]][:4]))
-            if (k + 1) % 20 == 0:
+            j = futs[k].result()
+            lag.append(time.perf_counter() - t_start - arrivals[k])
+            q.put((k, srv.submit(*j[:4])))
+            futs[k] = None
+            if (k + 1) % 50 == 0:
                 log("serve: %d/%d submitted" % (k + 1, njobs))
+        t_sub = time.perf_counter() - t_start
         q.put(None)
         th.join()
         st = srv.stats()
     ctx.close()
     lat = np.array([r[0] for r in results])
+    lag = np.array(lag) * 1e3
     pts = sum(r[2] for r in results)
     wall = max(r[3] for r in results) - t_start
-    return {"workload": "serving: Poisson arrivals, job shapes nx,ny~U{16..256}, nz~U{64..2048}, "
-                        "nw~U{16..512} (pool of %d seeded shapes), mixed precision, emergent-flux outputs"
-                        % pool_size,
-            "jobs": njobs, "arrival_rate_jobs_per_s": round(rate, 3), "stated_rate_jobs_per_s": C5_RATE,
-            "stated_offered_points_per_s": C5_RATE * C5_MEAN_POINTS,
-            "capacity_points_per_s": round(cap, 1),
+    offered_pts = float(sum(nx * ny * nz * nw for nx, ny, nz, nw in t_shapes))
+    return {"workload": "serving: Poisson arrivals of %d jobs, each its own seeded shape nx,ny~U{16..256}, "
+                        "nz~U{64..2048}, nw~U{16..512} and inputs, mixed precision, emergent-flux outputs"
+                        % njobs,
+            "jobs": njobs, "completed": len(results), "burst_jobs": burst,
+            "arrival_rate_jobs_per_s": round(rate, 3),
+            "offered_jobs_per_s": round(njobs / arrivals[-1], 3),
+            "offered_points_per_s": round(offered_pts / arrivals[-1], 1),
+            "submitted_jobs_per_s": round(njobs / t_sub, 3),
+            "achieved_jobs_per_s": round(len(results) / wall, 3),
+            "achieved_points_per_s": round(pts / wall, 1),
+            "stated_rate_jobs_per_s": C5_RATE, "stated_offered_points_per_s": C5_RATE * C5_MEAN_POINTS,
+            "capacity_points_per_s": round(cap, 1), "mean_job_points": round(mean_pts, 1),
             "p50_ms": round(float(np.percentile(lat, 50)), 2), "p99_ms": round(float(np.percentile(lat, 99)), 2),
             "mean_ms": round(float(lat.mean()), 2), "max_ms": round(float(lat.max()), 2),
             "mean_queue_ms": round(float(np.mean([r[1] for r in results])), 2),
-            "points_per_s": round(pts / wall, 1), "launches": int(st.iterations),
+            "submit_lag_p50_ms": round(float(np.percentile(lag, 50)), 2),
+            "submit_lag_max_ms": round(float(lag.max()), 2),
+            "launches": int(st.iterations),
             "hbm_budget_gib": round(budget / 2 ** 30, 1), "hbm_peak_gib": round(st.hbm_peak / 2 ** 30, 2)}
 
 

@@ -147,6 +147,39 @@ def config5_job(nx, ny, nz, nw, seed, sigma_dex=0.5):
     return rho, T, dz, log_waves(nw)
 
 
+def lognormal_pool(n, seed=SEED + 6, sigma_dex=0.5):
+    """n seeded factors 10^(sigma_dex N(0, 1)): the per-cell rho perturbation
+    of the serving jobs, drawn once and shared by slices."""
+    rng = np.random.default_rng(seed)
+    return 10.0 ** (sigma_dex * rng.standard_normal(n))
+
+
+def serving_job_into(rho_out, T_out, nx, ny, nz, seed, pool):
+    """One serving job's inputs written into rho_out, T_out (flat arrays of
+    at least nx ny nz doubles, e.g. pinned host memory), in the solver's
+    layout [ix, iy, z]: the disk profiles of disk_columns (per column T_mid ~
+    U(2e4, 1e5) K, rho0 ~ logU(1e-11, 1e-8), H ~ U(0.5, 2) AU) times a
+    lognormal factor per cell, here a seeded contiguous slice of `pool`
+    (lognormal_pool), then the rho floor.  Returns dz."""
+    rng = np.random.default_rng(seed)
+    z, dz = z_grid(nz)
+    ncol, n = nx * ny, nx * ny * nz
+    Tm = rng.uniform(2e4, 1e5, ncol)[:, None]
+    rho0 = 10.0 ** rng.uniform(-11.0, -8.0, ncol)[:, None]
+    H = rng.uniform(0.5, 2.0, ncol)[:, None] * AU
+    off = int(rng.integers(0, pool.size - n + 1))
+    r = rho_out[:n].reshape(ncol, nz)
+    np.multiply(-0.5 * z * z, 1.0 / (H * H), out=r)
+    np.exp(r, out=r)
+    r *= rho0
+    r *= pool[off:off + n].reshape(ncol, nz)
+    np.maximum(r, RHO_FLOOR, out=r)
+    t = T_out[:n].reshape(ncol, nz)
+    np.multiply(Tm, np.exp(-z ** 2 / (2 * SIGMA_T ** 2))[None, :], out=t)
+    np.maximum(t, T_FLOOR, out=t)
+    return dz
+
+
 def poisson_arrivals(njobs, rate, seed=SEED + 5):
     """Arrival times (s) of a Poisson stream of `rate` jobs per second."""
     rng = np.random.default_rng(seed + 1000)

@@ -11,6 +11,10 @@
 #   build_ref.sh dump <nz> <nw>
 #       builds oracle/_ref/dump_nz<nz>_nw<nw>/ref_dump.x: the reference modules
 #       plus the golden-vector driver oracle/ref/ref_dump.f90.
+#   build_ref.sh time <nz> <nw>
+#       builds oracle/_ref/time_nz<nz>_nw<nw>/ref_time.x: the reference modules
+#       at -O3 (source/Makefile.source:8's optimisation level) plus the timing
+#       driver oracle/ref/ref_time.f90 (the CPU baseline beside the GPU).
 set -e
 REF=${OCO_REFERENCE:-/root/reference}
 HERE=$(cd "$(dirname "$0")" && pwd)
@@ -44,6 +48,20 @@ dump)
   $FC $FFLAGS -I. -c "$HERE/ref_dump.f90" -o ref_dump.o
   $FC common.o continuous_opacity.o gas_state.o auxiliary.o ref_dump.o -o ../ref_dump.x
   ;;
+time)
+  nz=$2; nw=$3
+  dst="$OUT/time_nz${nz}_nw${nw}"
+  mkdir -p "$dst/obj"
+  printf 'integer, parameter :: nw=%s\ninteger, parameter :: nx=1,ny=1,nz=%s\ninteger, parameter :: nprocx=1,nprocy=1,nprocz=1\ninteger, parameter :: nyloc=ny/nprocy,nxloc=nx/nprocx\n' "$nw" "$nz" > "$dst/resolution.in"
+  cd "$dst/obj"
+  TFLAGS="-O3 -fdefault-real-8 -fdefault-double-8"
+  for f in common continuous_opacity gas_state auxiliary; do
+    $FC $TFLAGS -I. -c "$REF/source/$f.f90" -o $f.o
+  done
+  $FC $TFLAGS -I. -c "$HERE/ref_time.f90" -o ref_time.o
+  $FC common.o continuous_opacity.o gas_state.o auxiliary.o ref_time.o -o ../ref_time.x
+  rm -f ./*.o ./*.mod
+  ;;
 *)
-  echo "usage: $0 sample <name> <case_dir> | dump <nz> <nw>" >&2; exit 2;;
+  echo "usage: $0 sample <name> <case_dir> | dump <nz> <nw> | time <nz> <nw>" >&2; exit 2;;
 esac

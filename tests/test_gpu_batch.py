@@ -55,6 +55,55 @@ def test_batch_bit_exact_and_graph_replays():
         b.close()
 
 
+def edge_batch():
+    """Columns across the uneven-batch configuration's whole range (nz 32 ..
+    1024, nw 16 .. 256): the mixed path's column classes' edges 72/73,
+    144/145, 288/289, 512/513, 768/769 and 1024, and the 64-wavelength tile
+    edges 63, 64, 65, 128, 255 and 256, plus the range's ends; disk profiles
+    with the lognormal rho perturbation of the configuration."""
+    from ocotillo_amd.workloads import RHO_FLOOR, disk_columns, log_waves
+    rng = np.random.default_rng(2026)
+    nzs = [32, 72, 73, 144, 145, 288, 289, 512, 513, 768, 769, 1024, 1024, 600]
+    nws = [16, 63, 64, 65, 256, 128, 255, 17, 64, 63, 65, 256, 16, 100]
+    rhos, Ts, dzs, waves = [], [], [], []
+    for n, m in zip(nzs, nws):
+        r, t, d = disk_columns(1, 1, n, rng=rng)
+        rhos.append(np.maximum(r.reshape(-1) * 10.0 ** rng.normal(0.0, 0.5, n), RHO_FLOOR))
+        Ts.append(t.reshape(-1))
+        dzs.append(d)
+        waves.append(log_waves(m))
+    return (np.array(nzs, np.int32), np.array(nws, np.int32), np.concatenate(waves), np.array(dzs),
+            np.concatenate(rhos), np.concatenate(Ts))
+
+
+def test_batch_fp64_whole_range_bit_exact():
+    """FP64 batch over the configuration's whole range of nz and nw: every
+    column bit-exact against the oracle on the autotuned kernel, on three
+    graph replays with eager work between them."""
+    import ocotillo_amd as oco
+    nz, nw, waves, dz, rho, T = edge_batch()
+    print("batch nz", nz.tolist(), "nw", nw.tolist(), flush=True)
+    ref = oracle_columns(nz, nw, waves, dz, rho, T)
+    with oco.Context(0) as ctx:
+        ctx.set_physics(2e4, False, 0.4)
+        b = oco.Batch(ctx, nz, nw, waves, dz)
+        try:
+            for rep in range(3):
+                U, V, A, d = b.run_host(rho, T)
+                assert d.status == 0
+                Us, Vs, As = b.split(U, V, A)
+                for c, (rU, rV, rA, st) in enumerate(ref):
+                    assert st == 0
+                    assert np.array_equal(Us[c], rU) and np.array_equal(Vs[c], rV) and np.array_equal(As[c], rA), \
+                        ("replay", rep, "column", c, "nz", int(nz[c]), "nw", int(nw[c]))
+                ctx.set_wavelengths(waves[:nw[0]])
+                ctx.solve_block(rho[:nz[0]].reshape(1, 1, -1), T[:nz[0]].reshape(1, 1, -1), float(dz[0]))
+            v, nb, tuned = b.tuning()
+            assert tuned == 1 and nb > 0
+        finally:
+            b.close()
+
+
 def test_batch_zero_pivot_first_column():
     import ocotillo_amd as oco
     g = golden("dump_pivot0.npz")
