@@ -208,7 +208,13 @@ __device__ static inline void stage_put(double* buf, int idx, double v)
 }
 
 /* Outputs are written once and not read back by the kernel: nontemporal
- * stores keep them from displacing the scratch and cell records in L2. */
+ * stores keep them from displacing the scratch and cell records in L2.
+ * Tuning knob: OCO_WAVE_NT=false gives the wavefront sweeps ordinary stores
+ * (measured on MI355X, 256 x 256 columns, nz = 512, nw = 128: 39.1 ms
+ * against 37.35 with nontemporal stores). */
+#ifndef OCO_WAVE_NT
+#define OCO_WAVE_NT true
+#endif
 template <bool NT = true>
 __device__ static inline void out_store(double* p, double v)
 {
@@ -1067,7 +1073,7 @@ __device__ static inline void wave_points(const double* cc, const double* cT, in
             L[z] = p.alpha;
             L[nz + z] = p.albedo;
             L[2 * nz + z + 3] = p.S;
-            out_store(A0 + k * astride + z, p.alpha);
+            out_store<OCO_WAVE_NT>(A0 + k * astride + z, p.alpha);
         }
     }
 }
@@ -1338,7 +1344,7 @@ __device__ static inline void solve_system_wave(const oco_sys& d, double* lds, b
         const double* sb = LB + 3 * ll;
         const double df = rh_boundary_der(u[0], u[bs], u[2 * bs], u[3 * bs], al[0], al[bs], al[2 * bs], al[3 * bs],
                                           om[0], om[bs], om[2 * bs], sb[0], sb[1], sb[2], dz, dz2);
-        out_store(d.V + bz, flux_v(al[0], ll == 0 ? df : -df, dz1));
+        out_store<OCO_WAVE_NT>(d.V + bz, flux_v(al[0], ll == 0 ? df : -df, dz1));
     }
     /* a lane's cells z = ll + 1 + k L, k < MM (nz - 2 <= MM L): straight-line
      * code, every LDS read from an index clamped into the system, so all of
@@ -1348,14 +1354,30 @@ __device__ static inline void solve_system_wave(const oco_sys& d, double* lds, b
         const int z = ll + 1 + k * L;
         const double* u = LS + (z < nz - 1 ? z : nz - 2) + 3;
         const double v = flux_v(u[-3 - 2 * nz], der7(u[-3], u[-2], u[-1], u[1], u[2], u[3]), dz1);
-        if (z < nz - 1) out_store(d.V + z, v);
+        if (z < nz - 1) out_store<OCO_WAVE_NT>(d.V + z, v);
     }
-    /* U with ghosts: e = ll + k L, k <= MM (nz + 6 <= (MM + 1) L) */
+    /* U with ghosts, nz + 6 doubles: a system's U starts anywhere in a
+     * 128-byte line (nz + 6 is not a multiple of 16), so with 32 or 64 lanes
+     * the lanes are shifted by the start's offset `off` in the line, e = k L
+     * + ll - off, and every store but the first and last covers whole lines
+     * (measured on MI355X, 256 x 256 columns, nz = 512, nw = 128: 37.35 ->
+     * 36.7 ms).  k <= MM: nz + 6 + 15 <= (MM + 1) L for L >= 32.  With 8 or 16
+     * lanes, e = ll + k L (nz + 6 <= (MM + 1) L). */
+    if constexpr (L >= 32) {
+        const int off = (int)(((unsigned long)d.U >> 3) & 15);
+#pragma unroll
+        for (int k = 0; k <= MM; k++) {
+            const int e = k * L + ll - off;
+            const double u = LS[e < 0 ? 0 : (e < nz + 6 ? e : nz + 5)];
+            if (e >= 0 && e < nz + 6) out_store<OCO_WAVE_NT>(d.U + e, u);
+        }
+        return;
+    }
 #pragma unroll
     for (int k = 0; k <= MM; k++) {
         const int e = ll + k * L;
         const double u = LS[e < nz + 6 ? e : nz + 5];
-        if (e < nz + 6) out_store(d.U + e, u);
+        if (e < nz + 6) out_store<OCO_WAVE_NT>(d.U + e, u);
     }
 }
 
