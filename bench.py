@@ -615,34 +615,48 @@ class PinnedRing:
             self.cv.notify_all()
 
 
-def serve_inputs(shapes, seed0, noise, ring, workers):
-    """Start writing each job's inputs (serving_job_into, seed seed0 + k) into
-    `ring`, in order, on `workers` threads; returns one future per job whose
-    result is (rho3d, T3d, dz, waves, points).  The ring blocks the
-    producer, so generation runs at most one ring ahead of the consumer."""
+def serve_inputs(shapes, seed0, noise, ring, workers, piece=1 << 22):
+    """Start writing each job's inputs (seed seed0 + k) into `ring`, in order,
+    on `workers` threads, each job in pieces of whole columns of about
+    `piece` cells, so a large job is generated by all the threads at once;
+    returns one future per job whose result is (rho3d, T3d, dz, waves,
+    points).  The ring blocks the producer, so generation runs at most one
+    ring ahead of the consumer."""
     import concurrent.futures as cf
     import threading
     from ocotillo_amd import workloads as W
     ex = cf.ThreadPoolExecutor(max_workers=workers)
     futs = [cf.Future() for _ in shapes]
 
-    def fill(k, buf):
-        nx, ny, nz, nw = shapes[k]
-        n = nx * ny * nz
-        rho, T = buf[:n], buf[n:2 * n]
-        dz = W.serving_job_into(rho, T, nx, ny, nz, seed0 + k, noise)
-        return rho.reshape(nx, ny, nz), T.reshape(nx, ny, nz), dz, W.log_waves(nw), float(n * nw)
-
-    def run(k, buf):
-        try:
-            futs[k].set_result(fill(k, buf))
-        except BaseException as e:          # surfaced to the consumer by .result()
-            futs[k].set_exception(e)
-
     def produce():
         for k, (nx, ny, nz, nw) in enumerate(shapes):
-            buf = ring.alloc(2 * nx * ny * nz, (seed0, k))
-            ex.submit(run, k, buf)
+            n, ncol = nx * ny * nz, nx * ny
+            buf = ring.alloc(2 * n, (seed0, k))
+            rho, T = buf[:n], buf[n:2 * n]
+            try:
+                params = W.serving_job_params(nx, ny, nz, seed0 + k, noise.size)
+            except BaseException as e:
+                futs[k].set_exception(e)
+                continue
+            step = max(1, piece // nz)
+            parts = [ex.submit(W.serving_job_fill, rho, T, c, min(c + step, ncol), nz, params, noise)
+                     for c in range(0, ncol, step)]
+            left = [len(parts)]
+            lock = threading.Lock()
+            res = (rho.reshape(nx, ny, nz), T.reshape(nx, ny, nz), params[4], W.log_waves(nw), float(n * nw))
+
+            def done(f, k=k, left=left, lock=lock, res=res):
+                with lock:
+                    left[0] -= 1
+                    last = left[0] == 0
+                if f.exception() is not None:
+                    if not futs[k].done():
+                        futs[k].set_exception(f.exception())
+                elif last and not futs[k].done():
+                    futs[k].set_result(res)
+
+            for f in parts:
+                f.add_done_callback(done)
         ex.shutdown(wait=False)
 
     threading.Thread(target=produce, daemon=True).start()
@@ -673,7 +687,12 @@ def bench_serve(njobs, load=0.7, burst=12, workers=8):
     t0 = time.perf_counter()
     noise = W.lognormal_pool(max(cells))
     ring = PinnedRing(max(1 << 30, 2 * sum(cells[:burst]) + 2 * max(cells)))
-    log("serve: noise pool and %.1f GiB pinned ring in %.1f s" % (ring.cap * 8 / 2 ** 30, time.perf_counter() - t0))
+    # emergent outputs land in pinned memory too: a pageable destination is
+    # downloaded through 1 MB staging copies at a quarter of the rate
+    outs = [nx * ny * nw * 2 for nx, ny, _, nw in shapes]
+    oring = PinnedRing(max(1 << 29, sum(outs[:burst]) + 2 * max(outs)))
+    log("serve: noise pool and %.1f + %.1f GiB pinned rings in %.1f s"
+        % (ring.cap * 8 / 2 ** 30, oring.cap * 8 / 2 ** 30, time.perf_counter() - t0))
     ctx = oco.Context(0)
     ctx.set_physics(2e4, False, 0.4)
     free, _ = torch.cuda.mem_get_info()
@@ -682,11 +701,14 @@ def bench_serve(njobs, load=0.7, burst=12, workers=8):
         # capacity and warm-up: the burst back to back, inputs generated first
         bf = serve_inputs(b_shapes, W.SEED + 500, noise, ring, workers)
         jobs = [f.result() for f in bf]
+        bouts = [oring.alloc(outs[k], ("b", k)).reshape(j[3].size, j[0].shape[0], j[0].shape[1], 2)
+                 for k, j in enumerate(jobs)]
         t0 = time.perf_counter()
-        tickets = [srv.submit(*j[:4]) for j in jobs]
+        tickets = [srv.submit(*j[:4], out=o) for j, o in zip(jobs, bouts)]
         for k, t in enumerate(tickets):
             srv.wait(t)
             ring.free((W.SEED + 500, k))
+            oring.free(("b", k))
         cap = sum(j[4] for j in jobs) / (time.perf_counter() - t0)
         del jobs
         mean_pts = float(np.mean([nx * ny * nz * nw for nx, ny, nz, nw in t_shapes]))
@@ -707,6 +729,7 @@ def bench_serve(njobs, load=0.7, burst=12, workers=8):
                 _, r = srv.wait(t)
                 results.append((r.latency_ms, r.queue_ms, r.points, time.perf_counter()))
                 ring.free((seed0, k))
+                oring.free(("t", k))
 
         th = threading.Thread(target=collect)
         th.start()
@@ -718,8 +741,9 @@ def bench_serve(njobs, load=0.7, burst=12, workers=8):
             if delay > 0:
                 time.sleep(delay)
             j = futs[k].result()
+            o = oring.alloc(outs[burst + k], ("t", k)).reshape(j[3].size, j[0].shape[0], j[0].shape[1], 2)
             lag.append(time.perf_counter() - t_start - arrivals[k])
-            q.put((k, srv.submit(*j[:4])))
+            q.put((k, srv.submit(*j[:4], out=o)))
             futs[k] = None
             if (k + 1) % 50 == 0:
                 log("serve: %d/%d submitted" % (k + 1, njobs))
@@ -801,7 +825,7 @@ def bench_batch(args):
                     % "/".join(str((1, 2, 4)[d]) for d in digits))
         res[mode] = {"value": points / (ms_per_step * 1e-3), "ms_per_step": ms_per_step,
                      "roofline": roofline_of(kern, points, point_weights(mode, fion), float(np.mean(ms))),
-                     "variant": variant, "workgroups": nblocks}
+                     "variant": variant, "workgroups": nblocks, "tune_ms": b.tune_times(code)}
     head = modes[-1]
     out = {
         "metric": METRIC,
@@ -816,6 +840,15 @@ def bench_batch(args):
         "roofline": res[head]["roofline"],
         "cpu_baseline": None,
     }
+    # the autotuner's candidates, median of three launches each: FP64, the lane
+    # kernel's 3 variants x grids of 2, 3, 4 workgroups per SIMD; mixed, per column class
+    # holding columns (up to 72/144/288/512/768/1088 cells), 1, 2, 4 wavefronts
+    # per workgroup
+    edges = [0, 72, 144, 288, 512, 768, 1088]
+    cls_pts = [int((nz[(nz > edges[i]) & (nz <= edges[i + 1])].astype(np.int64)
+                    * nw[(nz > edges[i]) & (nz <= edges[i + 1])]).sum()) for i in range(6)]
+    out["autotune"] = {m: res[m]["tune_ms"] for m in modes}
+    out["class_points"] = cls_pts
     if len(modes) == 2:
         out["fp64"] = {"value": round(res["fp64"]["value"], 1), "ms_per_step": round(res["fp64"]["ms_per_step"], 4),
                        "roofline": res["fp64"]["roofline"]}

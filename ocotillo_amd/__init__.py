@@ -31,7 +31,7 @@ EXPORTS = ["oco_create", "oco_destroy", "oco_set_wavelengths", "oco_set_physics"
            "oco_solve_block_device", "oco_solve_block_device_noalloc", "oco_strerror", "oco_version", "oco_athena_slab", "oco_athena_slab_device",
            "oco_solve_block_files", "oco_write_record", "oco_opacity_points",
            "oco_batch_create", "oco_batch_destroy", "oco_batch_layout", "oco_batch_buffers", "oco_batch_run",
-           "oco_batch_run_host", "oco_batch_tuning",
+           "oco_batch_run_host", "oco_batch_tuning", "oco_batch_tune_times",
            "oco_server_create", "oco_server_submit", "oco_server_wait", "oco_server_poll", "oco_server_stats",
            "oco_server_destroy"]
 
@@ -117,6 +117,7 @@ def load_library(path=None):
     L.oco_batch_run_host.argtypes = [C.c_void_p, _DP, _DP, _DP, _DP, _DP, C.c_int, C.POINTER(Diag)]
     L.oco_batch_tuning.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int),
                                    C.POINTER(C.c_int)]
+    L.oco_batch_tune_times.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_float), C.c_int]
     L.oco_server_create.argtypes = [C.c_void_p, C.c_size_t, C.c_int, C.c_int, C.POINTER(C.c_void_p)]
     L.oco_server_submit.argtypes = [C.c_void_p, C.POINTER(Job), C.POINTER(C.c_longlong)]
     L.oco_server_wait.argtypes = [C.c_void_p, C.c_longlong, C.POINTER(JobResult)]
@@ -348,6 +349,16 @@ class Batch:
         self.L.oco_batch_tuning(self.h, mode, C.byref(v), C.byref(nb), C.byref(t))
         return v.value, nb.value, t.value
 
+    def tune_times(self, mode=MODE_FP64):
+        """The autotuner's median timings (ms) for `mode`, in the order it
+        tried its candidates (oco_batch_tune_times); empty before the first
+        run in the mode."""
+        buf = (C.c_float * 32)()
+        n = self.L.oco_batch_tune_times(self.h, mode, buf, 32)
+        if n < 0:
+            raise OcoError(-n, "oco_batch_tune_times: " + strerror(-n))
+        return [round(buf[i], 4) for i in range(min(n, 32))]
+
     def run(self, mode=MODE_FP64, raise_on_pivot=True):
         d = Diag()
         st = self.L.oco_batch_run(self.h, mode, C.byref(d))
@@ -399,15 +410,22 @@ class Server:
         self.h = h
         self.jobs = {}
 
-    def submit(self, rho3d, T3d, dz, waves):
-        """rho3d, T3d: [ix, iy, z]; returns a ticket."""
+    def submit(self, rho3d, T3d, dz, waves, out=None):
+        """rho3d, T3d: [ix, iy, z]; returns a ticket.  `out`, if given, is the
+        emergent-flux array [iw, ix, iy, 2] (C-ordered float64) the job writes,
+        e.g. in pinned host memory, where the download runs at full rate; by
+        default a new (pageable) array."""
         rho3d = np.ascontiguousarray(rho3d, dtype=np.float64)
         T3d = np.ascontiguousarray(T3d, dtype=np.float64)
         waves = np.ascontiguousarray(waves, dtype=np.float64)
         if rho3d.ndim != 3 or rho3d.shape != T3d.shape:
             raise OcoError(E_ARG, "rho3d and T3d must both be [nxloc, nyloc, nz]")
         nx, ny, nz = rho3d.shape
-        out = np.empty((waves.size, nx, ny, 2))
+        if out is None:
+            out = np.empty((waves.size, nx, ny, 2))
+        elif (out.dtype != np.float64 or out.shape != (waves.size, nx, ny, 2) or not out.flags.c_contiguous
+              or not out.flags.writeable):
+            raise OcoError(E_ARG, "out must be a writeable C-ordered float64 array of shape (nw, nx, ny, 2)")
         job = Job(nx, ny, nz, waves.size, float(dz), rho3d.ctypes.data, T3d.ctypes.data, waves.ctypes.data,
                   out.ctypes.data)
         t = C.c_longlong()

@@ -1015,6 +1015,7 @@ struct oco_batch {
      * the most significant) */
     int nblocks[2] = {0, 0}, variant[2] = {1, 0}, tuned[2] = {0, 0};
     float tune_ms[2][32] = {{0}};
+    int tune_n[2] = {0, 0};
     hipGraph_t graph[2] = {nullptr, nullptr};
     hipGraphExec_t exec[2] = {nullptr, nullptr};
     double graph_fiT[2] = {0.0, 0.0};
@@ -1301,6 +1302,14 @@ int oco_batch_buffers(oco_batch* b, double** rho, double** T, double** U, double
     return OCO_OK;
 }
 
+int oco_batch_tune_times(oco_batch* b, int mode, float* ms, int cap)
+{
+    if (!b || (mode != OCO_MODE_FP64 && mode != OCO_MODE_MIXED) || cap < 0 || (cap > 0 && !ms)) return -OCO_E_ARG;
+    const int n = b->tune_n[mode];
+    for (int i = 0; i < n && i < cap; i++) ms[i] = b->tune_ms[mode][i];
+    return n;
+}
+
 int oco_batch_tuning(oco_batch* b, int mode, int* variant, int* nblocks, int* tuned)
 {
     if (!b || (mode != OCO_MODE_FP64 && mode != OCO_MODE_MIXED)) return OCO_E_ARG;
@@ -1339,6 +1348,7 @@ static int batch_prepare(oco_batch* b, int m)
             std::sort(t3, t3 + 3);
             ms = t3[1];
             if (k < 32) b->tune_ms[m][k++] = ms;
+            b->tune_n[m] = k;
             return OCO_OK;
         };
         if (m == OCO_MODE_MIXED) {

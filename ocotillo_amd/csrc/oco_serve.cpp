@@ -14,7 +14,10 @@
  *   retire   when a launch completes, count its tiles against their jobs; a
  *            job with all tiles done has its emergent flux copied out and its
  *            arena blocks freed, which admits the next queued job.
- * Wavelength tables stay resident in the arena, one per distinct grid.
+ * Wavelength tables stay resident in the arena, one per distinct grid, while
+ * a job uses them (and after, until the room is needed); a new grid's table
+ * is built on the host at admission and uploaded by the copy thread ahead of
+ * its first job's inputs.
  */
 #include <algorithm>
 #include <chrono>
@@ -24,6 +27,7 @@
 #include <cstring>
 #include <deque>
 #include <map>
+#include <memory>
 #include <mutex>
 #include <new>
 #include <thread>
@@ -102,6 +106,7 @@ struct Job {
     size_t off = 0, bytes = 0;
     const double* d_wtab = nullptr;
     int table = -1;   /* index of its wavelength table in oco_server::tables while admitted */
+    std::shared_ptr<std::vector<double>> wt_host;   /* the host table it uploaded, kept until it is done */
     double* d_em = nullptr;
     hipEvent_t ready = nullptr, copied = nullptr;
     JobState state = QUEUED;
@@ -121,6 +126,9 @@ struct Table {
     size_t off = 0, bytes = 0;
     int users = 0;    /* admitted jobs on this grid */
     bool live = false;
+    /* the host table until the copy thread has uploaded it (with the
+     * inputs of the first job on the grid, ahead of that job's records) */
+    std::shared_ptr<std::vector<double>> pending;
 };
 
 /* Arena bytes of a job's own buffers: rho and T, cell records (mixed:
@@ -191,7 +199,9 @@ struct oco_server {
 };
 
 /* index of the resident table of this wavelength grid, built in the arena
- * if it is not there yet; -1 if it does not fit */
+ * if it is not there yet (on the host here; the copy thread uploads it with
+ * the job's inputs, so the scheduler never waits on a transfer); -1 if it
+ * does not fit */
 int oco_server::table_for(const double* waves, int nw)
 {
     for (size_t i = 0; i < tables.size(); i++) {
@@ -200,15 +210,11 @@ int oco_server::table_for(const double* waves, int nw)
     }
     Table t;
     t.waves.assign(waves, waves + nw);
-    std::vector<double> wt;
-    oco_build_wtab(t.waves, wt);
-    t.bytes = wt.size() * sizeof(double);
+    auto wt = std::make_shared<std::vector<double>>();
+    oco_build_wtab(t.waves, *wt);
+    t.bytes = wt->size() * sizeof(double);
     if (!arena.alloc(t.bytes, t.off)) return -1;
-    if (hipMemcpy(arena.base + t.off, wt.data(), t.bytes, hipMemcpyHostToDevice) != hipSuccess) {
-        (void)hipGetLastError();
-        arena.release(t.off, t.bytes);
-        return -1;
-    }
+    t.pending = std::move(wt);
     t.live = true;
     for (size_t i = 0; i < tables.size(); i++)
         if (!tables[i].live) {
@@ -228,6 +234,7 @@ void oco_server::drop_unused_tables()
             arena.release(t.off, t.bytes);
             t.live = false;
             t.waves.clear();
+            t.pending.reset();
         }
 }
 
@@ -458,13 +465,21 @@ void oco_server::copy_loop()
         const size_t b_in = (size_t)ncells * sizeof(double);
         char* p = arena.base + j->off;
         const int slot = j->slot;
+        /* the job's wavelength table, if this is its grid's first upload */
+        std::shared_ptr<std::vector<double>> wt;
+        void* d_wt = nullptr;
+        if (task.second == 0 && j->table >= 0 && tables[j->table].pending) {
+            wt = std::move(tables[j->table].pending);
+            d_wt = arena.base + tables[j->table].off;
+        }
         lk.unlock();
         hipError_t e = hipSuccess;
         if (task.second == 0) {
             double* d_rho = (double*)p;
             double* d_T = (double*)(p + Arena::round(b_in));
             double* d_cells = (double*)(p + 2 * Arena::round(b_in));
-            e = hipMemcpyAsync(d_rho, d.rho3d, b_in, hipMemcpyHostToDevice, xs);
+            if (wt) e = hipMemcpyAsync(d_wt, wt->data(), wt->size() * sizeof(double), hipMemcpyHostToDevice, xs);
+            if (e == hipSuccess) e = hipMemcpyAsync(d_rho, d.rho3d, b_in, hipMemcpyHostToDevice, xs);
             if (e == hipSuccess) e = hipMemcpyAsync(d_T, d.T3d, b_in, hipMemcpyHostToDevice, xs);
             if (e == hipSuccess)
                 e = mixed ? oco_launch_cells_chunked(d_rho, d_T, ncells, d.nz, fully_ionized_T, d_cells, xs)
@@ -480,6 +495,7 @@ void oco_server::copy_loop()
         }
         lk.lock();
         if (e != hipSuccess) fatal = OCO_E_HIP;
+        if (wt) j->wt_host = std::move(wt);
         if (task.second == 0) j->loaded = true;
         else j->copy_issued = true;
         copier_busy = false;

@@ -154,30 +154,48 @@ def lognormal_pool(n, seed=SEED + 6, sigma_dex=0.5):
     return 10.0 ** (sigma_dex * rng.standard_normal(n))
 
 
-def serving_job_into(rho_out, T_out, nx, ny, nz, seed, pool):
-    """One serving job's inputs written into rho_out, T_out (flat arrays of
-    at least nx ny nz doubles, e.g. pinned host memory), in the solver's
-    layout [ix, iy, z]: the disk profiles of disk_columns (per column T_mid ~
-    U(2e4, 1e5) K, rho0 ~ logU(1e-11, 1e-8), H ~ U(0.5, 2) AU) times a
-    lognormal factor per cell, here a seeded contiguous slice of `pool`
-    (lognormal_pool), then the rho floor.  Returns dz."""
+def serving_job_params(nx, ny, nz, seed, pool_size):
+    """The seeded draws of one serving job (see serving_job_fill): per column
+    T_mid ~ U(2e4, 1e5) K, rho0 ~ logU(1e-11, 1e-8), H ~ U(0.5, 2) AU, and
+    the offset of its lognormal factors in the pool.  Returns (Tm, rho0, H,
+    off, dz), Tm, rho0, H as [ncol, 1] arrays."""
     rng = np.random.default_rng(seed)
-    z, dz = z_grid(nz)
-    ncol, n = nx * ny, nx * ny * nz
+    _, dz = z_grid(nz)
+    ncol = nx * ny
     Tm = rng.uniform(2e4, 1e5, ncol)[:, None]
     rho0 = 10.0 ** rng.uniform(-11.0, -8.0, ncol)[:, None]
     H = rng.uniform(0.5, 2.0, ncol)[:, None] * AU
-    off = int(rng.integers(0, pool.size - n + 1))
-    r = rho_out[:n].reshape(ncol, nz)
-    np.multiply(-0.5 * z * z, 1.0 / (H * H), out=r)
+    off = int(rng.integers(0, pool_size - ncol * nz + 1))
+    return Tm, rho0, H, off, dz
+
+
+def serving_job_fill(rho_out, T_out, c0, c1, nz, params, pool):
+    """Columns c0 .. c1 - 1 of one serving job's inputs, written into the
+    flat arrays rho_out, T_out (the whole job's, layout [ix, iy, z], e.g.
+    pinned host memory): the disk profiles of disk_columns times a lognormal
+    factor per cell, a contiguous slice of `pool` (lognormal_pool) at the
+    job's offset, then the rho floor.  Disjoint column ranges may be filled
+    concurrently."""
+    Tm, rho0, H, off, _ = params
+    z, _ = z_grid(nz)
+    n0, n1 = c0 * nz, c1 * nz
+    r = rho_out[n0:n1].reshape(c1 - c0, nz)
+    np.multiply(-0.5 * z * z, 1.0 / (H[c0:c1] * H[c0:c1]), out=r)
     np.exp(r, out=r)
-    r *= rho0
-    r *= pool[off:off + n].reshape(ncol, nz)
+    r *= rho0[c0:c1]
+    r *= pool[off + n0:off + n1].reshape(c1 - c0, nz)
     np.maximum(r, RHO_FLOOR, out=r)
-    t = T_out[:n].reshape(ncol, nz)
-    np.multiply(Tm, np.exp(-z ** 2 / (2 * SIGMA_T ** 2))[None, :], out=t)
+    t = T_out[n0:n1].reshape(c1 - c0, nz)
+    np.multiply(Tm[c0:c1], np.exp(-z ** 2 / (2 * SIGMA_T ** 2))[None, :], out=t)
     np.maximum(t, T_FLOOR, out=t)
-    return dz
+
+
+def serving_job_into(rho_out, T_out, nx, ny, nz, seed, pool):
+    """One serving job's inputs (serving_job_params, serving_job_fill over
+    all its columns).  Returns dz."""
+    params = serving_job_params(nx, ny, nz, seed, pool.size)
+    serving_job_fill(rho_out, T_out, 0, nx * ny, nz, params, pool)
+    return params[4]
 
 
 def poisson_arrivals(njobs, rate, seed=SEED + 5):
