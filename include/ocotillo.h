@@ -44,8 +44,11 @@ enum {
  * optically thin cells, and form the emergent flux V(1), V(nz) from the
  * rows' interface differences instead of differences of U (:64-78, 96-109);
  * both are held to the exact solution of the reference's discrete problem.
- * Their columns take at most 1088 cells (batches and blocks; the server
- * takes any nz up to its max_nz). */
+ * Their columns take 4 to 1088 cells (batches and blocks; the server takes
+ * any nz from 4 up to its max_nz): their emergent flux uses the 4 cells
+ * next to each boundary.  The FP64 drop-in takes any nz >= 2 in a block
+ * solve, as the reference does (nz = 1 leaves the reference's dz and its
+ * ghost update undefined); OCO_E_ARG otherwise. */
 enum {
     OCO_MODE_FP64 = 0,     /* drop-in: the reference's FP64 arithmetic, operation by operation */
     OCO_MODE_MIXED = 1,    /* FP32-core opacity with exponent scaling, FP64 Saha and elimination */

@@ -414,7 +414,7 @@ int oco_solve_block_device(oco_ctx* c, int nz, int nyloc, int nxloc, double dz, 
                            int mode, oco_diag* diag)
 {
     if (!c) return OCO_E_ARG;
-    if (nz < 4 || nyloc < 1 || nxloc < 1 || !d_rho || !d_T || !d_U || !d_V || !d_A) return OCO_E_ARG;
+    if (nz < 2 || nyloc < 1 || nxloc < 1 || !d_rho || !d_T || !d_U || !d_V || !d_A) return OCO_E_ARG;
     if (!mode_ok(mode)) return OCO_E_ARG;
     if (c->nw <= 0) return OCO_E_STATE;
     if (c->lgrey && c->nw != 1) return OCO_E_ARG;   /* source/flux_feautrier.f90:84-87 */
@@ -574,7 +574,7 @@ int oco_solve_block(oco_ctx* c, int nz, int nyloc, int nxloc, double dz, double 
                     oco_diag* diag)
 {
     if (!c || !rho3d || !T3d || !U || !V || !absorp) return OCO_E_ARG;
-    if (nz < 4 || nyloc < 1 || nxloc < 1) return OCO_E_ARG;
+    if (nz < 2 || nyloc < 1 || nxloc < 1) return OCO_E_ARG;
     if (c->nw <= 0) return OCO_E_STATE;
     HIPCHK(hipSetDevice(c->device));
     const size_t ncells = (size_t)nz * nyloc * nxloc;
@@ -711,7 +711,7 @@ int oco_solve_block_files(oco_ctx* c, int nz, int nyloc, int nxloc, double dz, d
 {
     const auto t_start = std::chrono::steady_clock::now();
     if (!c || !rho3d || !T3d || !dir || !snapshot) return OCO_E_ARG;
-    if (nz < 4 || nyloc < 1 || nxloc < 1) return OCO_E_ARG;
+    if (nz < 2 || nyloc < 1 || nxloc < 1) return OCO_E_ARG;
     if (!mode_ok(mode)) return OCO_E_ARG;
     if (c->nw <= 0) return OCO_E_STATE;
     if (c->lgrey && c->nw != 1) return OCO_E_ARG;   /* source/flux_feautrier.f90:84-87 */

@@ -925,6 +925,98 @@ __global__ __launch_bounds__(256) void canary_fp64(const oco_sweep_args a, doubl
     if (tid < 4) buf[tid] = red[tid][0];
 }
 
+/* Columns of 2 or 3 cells (FP64 drop-in): one thread per (column, wavelength)
+ * system, grid-stride, the whole system in registers.  The same point,
+ * row and elimination code as the lane sweep, in the reference's order:
+ * get_tridag_coefficients (source/auxiliary.f90:111-201; nz = 2 is two
+ * boundary rows), tridag (:14-47) with its zero-pivot stops, update_ghosts
+ * (:49-62) in its interleaved order (with fewer than 4 cells its reads
+ * f(n1+i), f(n2-i) reach ghosts the same loop set earlier), der (:64-78)
+ * and calc_flux (:96-109).  Outputs are stored directly: a system is a few
+ * dozen bytes. */
+#define OCO_TINY_NZ 3
+template <bool GREY>
+__global__ __launch_bounds__(256) void sweep_tiny_fp64(const oco_sweep_args a)
+{
+    __shared__ oco_mtab smt;
+    load_mtab(&smt);
+    const int nz = a.nz;
+    const long nsys = a.ncol * a.nw;
+    for (long s = (long)blockIdx.x * blockDim.x + threadIdx.x; s < nsys; s += (long)gridDim.x * blockDim.x) {
+        const long col = s / a.nw;
+        const int iw = (int)(s - col * a.nw);
+        double w[OCO_WT_STRIDE];
+        if (!GREY) {
+#pragma unroll
+            for (int k = 0; k < OCO_WT_STRIDE; k++) w[k] = a.wtab[(long)iw * OCO_WT_STRIDE + k];
+        }
+        oco_pt p[OCO_TINY_NZ];
+#pragma unroll
+        for (int z = 0; z < OCO_TINY_NZ; z++) {
+            if (z < nz) {
+                double rec[CR_NFIELD];
+                if (!GREY) load_record<false>(a, col, z, rec);
+                p[z] = point_rec<GREY, false>(a, col, z, rec, w, &smt);
+            }
+        }
+        double av[OCO_TINY_NZ], bv[OCO_TINY_NZ], cv[OCO_TINY_NZ], dv[OCO_TINY_NZ];
+        boundary_row(p[0], a.dz, a.dz2, bv[0], dv[0]);
+        av[0] = 0.0;
+        cv[0] = p[0].alpha;
+        if (nz == 3) interior_row(p[0].alpha, p[1], p[2].alpha, a.dz2, av[1], bv[1], cv[1], dv[1]);
+        const int n = nz - 1;
+        boundary_row(p[n], a.dz, a.dz2, bv[n], dv[n]);
+        av[n] = p[n].alpha;
+        cv[n] = 0.0;
+        double u[OCO_TINY_NZ], gam[OCO_TINY_NZ];
+        int stage = 0, row = 0;
+        double bet = bv[0];
+        if (bet == 0.0) {
+            stage = 1;
+            row = 1;
+        } else {
+            u[0] = dv[0] / bet;
+            for (int j = 1; j < nz; j++) {
+                gam[j] = cv[j - 1] / bet;
+                bet = bv[j] - av[j] * gam[j];
+                if (bet == 0.0) {
+                    stage = 2;
+                    row = j + 1;
+                    break;
+                }
+                u[j] = (dv[j] - av[j] * u[j - 1]) / bet;
+            }
+        }
+        const long key = col * a.nw + iw;
+        if (stage != 0) {   /* the reference stops at this system: nothing of it is stored */
+            record_fail(a.fail, key, stage, row);
+            continue;
+        }
+        for (int j = nz - 2; j >= 0; j--) u[j] = u[j] - gam[j + 1] * u[j + 1];
+        double f[OCO_TINY_NZ + 6];
+#pragma unroll
+        for (int j = 0; j < OCO_TINY_NZ; j++)
+            if (j < nz) f[3 + j] = u[j];
+        const int n1 = 3, n2 = 3 + nz - 1;
+        for (int i = 1; i <= 3; i++) {
+            f[n1 - i] = 2.0 * f[n1] - f[n1 + i];
+            f[n2 + i] = 2.0 * f[n2] - f[n2 - i];
+        }
+        const long osys = (long)iw * a.ncol + col;
+        double* U = a.U + osys * (nz + 6);
+        double* V = a.V + osys * nz;
+        double* A = a.A + osys * nz;
+        for (int e = 0; e < nz + 6; e++) U[e] = f[e];
+        const double c60 = 1.0 / 60.0;
+        for (int j = 0; j < nz; j++) {
+            const double* g = f + n1 + j;
+            const double df = c60 * (((45.0 * (g[1] - g[-1])) - (9.0 * (g[2] - g[-2]))) + (g[3] - g[-3]));
+            V[j] = ((-1.0 * p[j].alpha1) * df) * a.dz1;
+            A[j] = p[j].alpha;
+        }
+    }
+}
+
 /* ---- mixed precision, one wavefront per system ----
  * The lane-per-system sweep keeps every system's elimination state (gam, u,
  * 1/alpha per row) in an HBM scratch slot: 48 bytes per point of extra
@@ -2323,6 +2415,16 @@ hipError_t oco_launch_sweep(const oco_sweep_args& a, int lgrey, int wave, int la
             hipLaunchKernelGGL((canary_fp64<false>), dim3(1), dim3(256), 0, stream, a, a.canary);
     }
     if (!a.tickets) return hipErrorInvalidValue;
+    if (a.nz <= OCO_TINY_NZ) {   /* 2 or 3 cells: the FP64 drop-in only */
+        if (wave || a.nz < 2) return hipErrorInvalidValue;
+        const long nsys = a.ncol * a.nw;
+        const long nb = (nsys + 255) / 256 < 4096 ? (nsys + 255) / 256 : 4096;
+        if (lgrey)
+            hipLaunchKernelGGL((sweep_tiny_fp64<true>), dim3((unsigned)nb), dim3(256), 0, stream, a);
+        else
+            hipLaunchKernelGGL((sweep_tiny_fp64<false>), dim3((unsigned)nb), dim3(256), 0, stream, a);
+        return hipGetLastError();
+    }
     hipLaunchKernelGGL(tickets_reset, dim3(1), dim3(64), 0, stream, a.tickets);
     const dim3 g(nblocks), bf(OCO_WAVE * OCO_SWEEP_WG);
     if (wave) {

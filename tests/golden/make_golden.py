@@ -215,6 +215,8 @@ def logwaves(nw, w0=100.0, w1=1e5):
 def main():
     if len(sys.argv) > 1 and sys.argv[1] == "pivot1":
         return run_pivot1()
+    if len(sys.argv) > 1 and sys.argv[1] == "tiny":
+        return run_tiny()
     copy_samples()
     run_samples()
     # C1 exactly (grey isothermal), plus random grey columns
@@ -233,6 +235,16 @@ def main():
     run_dump("uneven", 257, 3, 4, "c2", [3000.0, 5000.0, 1e4], full=False)
     run_dump("long", 1024, 4, 2, "c2", logwaves(4, 500.0, 2e4), full=False)
     run_pivot1()
+    run_tiny()
+
+
+def run_tiny():
+    """Columns of 2 and 3 cells: both rows are boundary rows (nz = 2) or one
+    interior row (nz = 3), and update_ghosts (source/auxiliary.f90:49-62)
+    reads ghosts it set earlier in the same loop (f(n1+i) beyond n2)."""
+    run_dump("tiny2", 2, 8, 8, "mix", logwaves(8))
+    run_dump("tiny3", 3, 8, 8, "mix", logwaves(8))
+    run_dump("tiny3_grey", 3, 1, 4, "grey", [3000.0], lgrey=1)
 
 
 def run_pivot1():

@@ -154,3 +154,47 @@ def test_c4_slice_fp64(ctx, where):
         frac = float((relerr(x, ref) <= TOL).mean())
         print("  %s within 1e-12 of the reference: %.6f" % (name, frac), flush=True)
         assert frac >= (0.995 if name == "U" else 0.999)
+
+
+@pytest.mark.parametrize("nz", [2, 3])
+def test_tiny_columns(ctx, nz):
+    """Columns of 2 and 3 cells (the FP64 drop-in's one-thread-per-system
+    kernel): bit-exact against the oracle on disk-profile columns with 128
+    and 37 wavelengths and on grey columns, a stage-1 zero pivot where alpha^2
+    underflows at the bottom cell reported like the oracle's, and OCO_E_ARG
+    from the difference-form modes, whose emergent flux needs 4 cells."""
+    import ocotillo_amd as oco
+    from ocotillo_amd.workloads import disk_columns, log_waves
+    from oracle.oracle import Oracle
+    print("case tiny nz", nz, flush=True)
+    rho, T, _ = disk_columns(5, 7, nz, seed=11 + nz)
+    dz = 1e16 / (nz - 1)
+    for nw in (128, 37):
+        waves = log_waves(nw)
+        ctx.set_wavelengths(waves)
+        ctx.set_physics(2e4, False, 0.4)
+        U, V, A, d = ctx.solve_block(rho, T, dz)
+        assert d.status == 0
+        pU, pV, pA, pd = Oracle(waves, portable=True).solve_block(rho, T, dz)
+        assert pd.status == 0
+        assert np.array_equal(A, pA) and np.array_equal(U, pU) and np.array_equal(V, pV), nw
+    for mode in (oco.MODE_MIXED, oco.MODE_ACCURATE):
+        with pytest.raises(oco.OcoError) as e:
+            ctx.solve_block(rho, T, dz, mode=mode)
+        assert e.value.status == oco.E_ARG
+    # grey, one column's bottom cell at rho = 1e-170: alpha^2 = 0 there, so
+    # b(1) = 0 and the reference stops in stage 1 (source/auxiliary.f90:27-30)
+    ctx.set_wavelengths(np.array([3000.0]))
+    ctx.set_physics(2e4, True, 0.4)
+    rg = np.full((1, 3, nz), 1e-9)
+    Tg = np.full((1, 3, nz), 2000.0)
+    U, V, A, d = ctx.solve_block(rg, Tg, dz)
+    pU, pV, pA, pd = Oracle(np.array([3000.0]), portable=True).solve_block(rg, Tg, dz, lgrey=True)
+    assert d.status == 0 and pd.status == 0
+    assert np.array_equal(U, pU) and np.array_equal(V, pV) and np.array_equal(A, pA)
+    rg[0, 1, 0] = 1e-170
+    _, _, _, d = ctx.solve_block(rg, Tg, dz, raise_on_pivot=False)
+    _, _, _, pd = Oracle(np.array([3000.0]), portable=True).solve_block(rg, Tg, dz, lgrey=True)
+    assert pd.status == oco.E_PIVOT0_STAGE1
+    assert (d.status, d.fail_ix, d.fail_iy, d.fail_iw, d.fail_j) == (pd.status, pd.fail_ix, pd.fail_iy,
+                                                                   pd.fail_iw, pd.fail_j)
