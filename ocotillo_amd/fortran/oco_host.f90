@@ -12,7 +12,7 @@
 !     source/common.f90:9), so one executable serves every shape;
 !   * optional namelist &oco_input: lwave_log (log-spaced wavelengths,
 !     default .false. = the reference's linear grid), oco_mode (0 FP64,
-!     1 mixed), oco_device, hbm_budget_gb;
+!     1 mixed, 2 accurate), oco_device, hbm_budget_gb;
 !   * several snapshot tags or ranges a:b on the command line run each
 !     snapshot as its own run would, into output_snapshot<tag>/ (what
 !     samples/multiInTest/snapshot_looper.sh meant to do), in one process.
