@@ -113,8 +113,8 @@ int oco_solve_block_device(oco_ctx* ctx, int nz, int nyloc, int nxloc, double dz
  * scratch (cell records, queue tickets, failure key, canaries) is one
  * static device array per GPU of OCO_STATIC_SCRATCH_MIB MiB, held by each
  * GPU the library runs on until the process ends: blocks of at most about
- * OCO_STATIC_SCRATCH_MIB * 2^20 / 168 cells (1.6M: 64 x 64 columns of 390
- * cells; 168 bytes of cell record each), else OCO_E_ARG.  Modes OCO_MODE_MIXED and OCO_MODE_ACCURATE only
+ * OCO_STATIC_SCRATCH_MIB * 2^20 / 184 cells (1.45M: 64 x 64 columns of 356
+ * cells; 184 bytes of cell record each), else OCO_E_ARG.  Modes OCO_MODE_MIXED and OCO_MODE_ACCURATE only
  * (the drop-in's elimination scratch grows with the grid): OCO_E_ARG for
  * OCO_MODE_FP64.  The wavelength table must already be set
  * (oco_set_wavelengths, which allocates at set-up).  Every call on one GPU

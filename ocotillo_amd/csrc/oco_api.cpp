@@ -164,6 +164,9 @@ void oco_build_wtab(const std::vector<double>& waves, std::vector<double>& wt)
         w[WT_HBFW] = OCO_AHBF * w[WT_WA3];
         w[WT_HFFW] = OCO_ALPHA0 * w[WT_WA3];
         w[WT_F1M26] = w[WT_F1] - 26.0;
+        w[WT_GBP] = oco_exp(5.960) * std::pow(w[WT_NU9], -(OCO_SQRT3 * OCO_PI1));
+        w[WT_NU3] = oco_exp(-w[WT_L3NU]);
+        w[WT_CHIL] = chi * OCO_LN10_HI;
     }
 }
 

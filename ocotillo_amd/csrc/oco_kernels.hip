@@ -215,11 +215,21 @@ __device__ static inline void stage_put(double* buf, int idx, double v)
 #ifndef OCO_WAVE_NT
 #define OCO_WAVE_NT true
 #endif
+/* Output, scratch and record pointers reach the kernels through structs and
+ * LDS tables, where the compiler cannot see that they point to global
+ * memory, and then emits flat instructions, which count against the LDS
+ * counter as well as the vector memory one: a wait for an LDS result (the
+ * math tables, the stages) would also wait for every store in flight.  These
+ * casts make them global accesses. */
+typedef __attribute__((address_space(1))) double oco_gdouble;
+__device__ static inline oco_gdouble* gptr(double* p) { return (oco_gdouble*)p; }
+__device__ static inline const oco_gdouble* gptr(const double* p) { return (const oco_gdouble*)p; }
+
 template <bool NT = true>
 __device__ static inline void out_store(double* p, double v)
 {
-    if constexpr (NT) __builtin_nontemporal_store(v, p);
-    else *p = v;
+    if constexpr (NT) __builtin_nontemporal_store(v, gptr(p));
+    else *gptr(p) = v;
 }
 
 /* write block b (elements b*R .. b*R+R-1, clipped to [0, len)) of every lane:
@@ -410,7 +420,7 @@ __device__ static inline void alpha_fetch(double* regs, double* const* base, int
 #pragma unroll
     for (int k = 0; k < R; k++) {
         const double* p = base[s0 + k * L];
-        regs[k] = p != nullptr && idx < len ? p[idx] : 1.0;
+        regs[k] = p != nullptr && idx < len ? gptr(p)[idx] : 1.0;
     }
 }
 
@@ -672,7 +682,7 @@ __device__ static inline void solve_backward(const oco_sys& d, bool valid, const
         double df = c60 * (((45.0 * (w4 - w2)) - (9.0 * (w5 - w1))) + (w6 - w0));
         const double vtop = ((-1.0 * a1v(nz - 1, a1q3)) * df) * dz1;
         if constexpr (EM) {
-            if (valid) d.V[1] = vtop;
+            if (valid) *gptr(d.V + 1) = vtop;
         } else {
             stage_put<R>(S->vb, nz - 1, vtop);
             if (((nz - 1) & (R - 1)) == 0) {
@@ -744,7 +754,7 @@ __device__ static inline void solve_backward(const oco_sys& d, bool valid, const
             double df = c60 * (((45.0 * (w4 - w2)) - (9.0 * (w5 - w1))) + (w6 - w0));
             const double v = ((-1.0 * a1v(2 - k, a1q3)) * df) * dz1;
             if constexpr (EM) {
-                if (valid) d.V[0] = v;
+                if (valid) *gptr(d.V) = v;
             } else {
                 stage_put<R>(S->vb, 2 - k, v);
             }
@@ -1733,7 +1743,7 @@ __device__ static inline void batch_tiles(const oco_batch_args& a, const oco_mta
             double w[OCO_WT_STRIDE];
             const double* wr = a.wtab + (woff[col] + iw) * OCO_WT_STRIDE;
 #pragma unroll
-            for (int k = 0; k < OCO_WT_STRIDE; k++) w[k] = wr[k];
+            for (int k = 0; k < OCO_WT_STRIDE; k++) w[k] = gptr(wr)[k];
             solve_system<false, true, 1, BC, MIX, STAGE>(d, valid, w, scr, a.fail, M, S);
         }
     }
@@ -1789,7 +1799,7 @@ __global__ __launch_bounds__(OCO_WAVE) void serve_tiles(const oco_serve_args a)
         double w[OCO_WT_STRIDE];
         const double* wr = jb.wtab + (long)iw * OCO_WT_STRIDE;
 #pragma unroll
-        for (int k = 0; k < OCO_WT_STRIDE; k++) w[k] = wr[k];
+        for (int k = 0; k < OCO_WT_STRIDE; k++) w[k] = gptr(wr)[k];
         solve_system<false, true, 1, BC, MIX, oco_stage_emergent>(d, valid, w, scr, a.fail + slot,
                                                                   MIX ? nullptr : &smt, &none);
     }
@@ -1939,7 +1949,7 @@ __global__ __launch_bounds__(OCO_WAVE * OCO_SERVE_WG) OCO_SERVE_ATTR void serve_
             double rec[CR_NFIELD];
             if (z0 > 0 && z0 < z1) {   /* the cell below the chunk: the previous lane's last */
 #pragma unroll
-                for (int f = 0; f < CR_NFIELD; f++) rec[f] = cc[(long)((m - 1) * CR_NFIELD + f) * OCO_WAVE + lane - 1];
+                for (int f = 0; f < CR_NFIELD; f++) rec[f] = gptr(cc)[(long)((m - 1) * CR_NFIELD + f) * OCO_WAVE + lane - 1];
 #pragma unroll
                 for (int k = 0; k < K; k++) {
                     if (k < nk) {
@@ -1955,7 +1965,7 @@ __global__ __launch_bounds__(OCO_WAVE * OCO_SERVE_WG) OCO_SERVE_ATTR void serve_
                 const int z = z0 + j;
                 if (z < z1) {
 #pragma unroll
-                    for (int f = 0; f < CR_NFIELD; f++) rec[f] = cc[(long)(j * CR_NFIELD + f) * OCO_WAVE + lane];
+                    for (int f = 0; f < CR_NFIELD; f++) rec[f] = gptr(cc)[(long)(j * CR_NFIELD + f) * OCO_WAVE + lane];
 #pragma unroll
                     for (int k = 0; k < K; k++) {
                         if (k < nk) {
@@ -2022,7 +2032,7 @@ __global__ __launch_bounds__(OCO_WAVE * OCO_SERVE_WG) OCO_SERVE_ATTR void serve_
                         const double* q3 = wslot(k, bz + 3 * bs);
                         const double df = rh_boundary_der(q0[0], q1[0], q2[0], q3[0], q0[3], q1[3], q2[3], q3[3],
                                                           q0[4], q1[4], q2[4], q0[5], q1[5], q2[5], dz, dz2);
-                        jb.em[((long)(iw0 + k) * jb.ncol + col) * 2 + lane] = flux_v(q0[3], lane == 0 ? df : -df, dz1);
+                        *gptr(jb.em + ((long)(iw0 + k) * jb.ncol + col) * 2 + lane) = flux_v(q0[3], lane == 0 ? df : -df, dz1);
                     }
                 }
             }

@@ -128,7 +128,7 @@ __device__ static inline oco_pt oco_point_mixed(const double* c, const double* w
      * stimulated-emission terms, A_Hbf wa^3 and alpha0 wa^3 come from the
      * table, and the neutral albedo is e_scatter over the sum before the
      * factor 1/mp (which cancels). */
-    const double sf = mx_omexp_near((w[WT_CHI * WS] * theta) * OCO_LN10, damp, E, r1) * fn;
+    const double sf = mx_omexp_near(w[WT_CHIL * WS] * theta, damp, E, r1) * fn;
     /* Unsold tail + sum_n g(n) n^-3 exp(-chi_n/kT), from the cell's term */
     double sm = c[CR_UNSOLD] + w[WT_G1 * WS];
     sm = sm + w[WT_G2 * WS] * c[CR_E2];
@@ -152,10 +152,13 @@ __device__ static inline oco_pt oco_point_mixed(const double* c, const double* w
         alpha = (s1 + s2) * (rho * OCO_MP1);
         kappa = s2;
     } else {
-        double inner = w[WT_NU9 * WS] * c[CR_U0];
-        double gB = mx_log(mx_exp(5.960 - (OCO_SQRT3 * OCO_PI1) * mx_log(inner)) + OCO_EXP1);
-        double lnQ = (c[CR_U1] - w[WT_L3NU * WS]) + mx_log(mx_omexp_near(w[WT_HNK * WS] * c[CR_U2], damp, E, r1));
-        double kRho = (OCO_C_BREMS * mx_exp(lnQ)) * gB;
+        /* bremsstrahlung: exp(5.960 - sqrt3/pi log(inner)) = WT_GBP CR_MQ and
+         * exp(lnQ) = CR_MR WT_NU3 (1 - exp(-h nu / kT)), so one logarithm is
+         * left where the reference's form has three logarithms and two
+         * exponentials */
+        const double gB = mx_log(w[WT_GBP * WS] * c[CR_MQ] + OCO_EXP1);
+        const double Q = (c[CR_MR] * w[WT_NU3 * WS]) * mx_omexp_near(w[WT_HNK * WS] * c[CR_U2], damp, E, r1);
+        const double kRho = (OCO_C_BREMS * Q) * gB;
         alpha = s1 * (rho * OCO_MP1) + kRho;
         kappa = s1 + alpha * (c[CR_U3] * (1.0 / OCO_MP1));   /* kappa mp */
     }

@@ -62,6 +62,8 @@ enum {
                                     log(ne n) + 2 log 1 - 0.5 log T, T1, rho1 */
     CR_ESM,         /* e_scatter * mp1 */
     CR_DT1,         /* damping_factor_const * T1 */
+    CR_MQ, CR_MR,   /* mixed mode only; ionised: U0^-(sqrt3/pi) = (T*1e-4)^(1.5 sqrt3/pi), exp(U1) =
+                       ne n / sqrt(T); neutral: 0 */
     CR_NFIELD
 };
 #define OCO_CELL_STRIDE 24   /* doubles per record (padded to 192 B) */
@@ -79,9 +81,12 @@ enum {
     WT_WCM1, WT_SFW5,        /* 1/wcm, source_function_const*w1cm5 */
     WT_HBFW, WT_HFFW,        /* mixed mode only: A_Hbf*wa^3, alpha0*wa^3, */
     WT_F1M26,                /* f_coeff(1) - 26 */
+    WT_GBP,                  /* mixed mode only: exp(5.960) (nu*1e-9)^-(sqrt3/pi) */
+    WT_NU3,                  /* mixed mode only: nu^-3 = exp(-3 log nu) */
+    WT_CHIL,                 /* mixed mode only: chi ln10 */
     WT_NFIELD
 };
-#define OCO_WT_STRIDE 23
+#define OCO_WT_STRIDE 26
 
 /* wavelength_independent_pillars (source/gas_state.f90:30-110 and
  * source/continuous_opacity.f90:143-163, 233-244, 262-296): fills rec. */
@@ -144,6 +149,12 @@ OCO_HD static inline void oco_cell_record(double rho, double T, double fully_ion
     rec[CR_ESM] = es * OCO_MP1;
     /* damping_factor_const = (h*c)*k1 (:73) */
     rec[CR_DT1] = ((OCO_H_PLANCK * OCO_C_LIGHT) * OCO_K1_CGS) * T1;
+    /* the cell factors of the mixed-precision bremsstrahlung term: the
+     * reference's exp(5.960 - sqrt3/pi log(nu 1e-9 U0)) and exp(U1 - 3 log nu)
+     * (source/continuous_opacity.f90:281-300) split into a cell factor here
+     * and a wavelength factor in the table (WT_GBP, WT_NU3) */
+    rec[CR_MQ] = neutral ? 0.0 : oco_exp_t(-(OCO_SQRT3 * OCO_PI1) * oco_log_t(rec[CR_U0], M), M);
+    rec[CR_MR] = neutral ? 0.0 : oco_exp_t(rec[CR_U1], M);
 }
 
 typedef struct {

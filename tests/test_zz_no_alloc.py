@@ -121,7 +121,7 @@ def test_noalloc_within_a_budget_that_allows_no_scratch():
 
 def test_noalloc_refusals():
     """FP64 mode, and a block beyond the static scratch (64 x 64 columns of
-    400 cells: 1.64M cells, 275 MB of records), are OCO_E_ARG; the host
+    400 cells: 1.64M cells, 301 MB of records), are OCO_E_ARG; the host
     checks them before any launch (the small buffers passed are never
     touched)."""
     import ocotillo_amd as oco
