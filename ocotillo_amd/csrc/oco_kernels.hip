@@ -1832,8 +1832,14 @@ __global__ __launch_bounds__(OCO_WAVE) void serve_tiles(const oco_serve_args a)
 #ifndef OCO_SERVE_WG
 #define OCO_SERVE_WG 4
 #endif
+/* Wavelengths each record load serves (K) and wavefronts per SIMD: measured
+ * on MI355X, serving workload (capacity burst of 12 jobs): K = 4 at 2 per
+ * SIMD (256 VGPRs, 128 bytes of spills per lane) 1.05e11 points/s; K = 2 at
+ * 2 per SIMD 9.3e10, at 3 per SIMD 8.9-9.0e10; K = 1 at 2 per SIMD 5.0e10;
+ * K = 6 at 2 per SIMD 5.7e10 (spills); K = 8 at 1 per SIMD 6.7e10; K = 4 with
+ * 2-wavefront workgroups 1.0e11. */
 #ifndef OCO_SERVE_K
-#define OCO_SERVE_K 2
+#define OCO_SERVE_K 4
 #endif
 /* Doubles per row of the serving kernel's boundary window: D (then x), W,
  * Z, alpha, albedo, S */
@@ -1892,12 +1898,10 @@ hipError_t oco_launch_cells_chunked(const double* rho, const double* T, long nce
     return hipGetLastError();
 }
 
-/* Waves of the serving kernel per SIMD the compiler is asked to fit: 3
- * (168 VGPRs, some spilled) serves 7.7e10 points/s on the serving workload
- * where 2 (211 VGPRs) serves 7.2e10; its chunk loop waits on each row's
- * record loads, and a third wavefront hides more of that. */
+/* Waves of the serving kernel per SIMD the compiler is asked to fit (with
+ * OCO_SERVE_K, above). */
 #ifndef OCO_SERVE_WAVES_PER_EU
-#define OCO_SERVE_WAVES_PER_EU 3
+#define OCO_SERVE_WAVES_PER_EU 2
 #endif
 #if OCO_SERVE_WAVES_PER_EU > 0
 #define OCO_SERVE_ATTR __attribute__((amdgpu_waves_per_eu(OCO_SERVE_WAVES_PER_EU)))
