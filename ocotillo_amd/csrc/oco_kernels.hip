@@ -1,5 +1,5 @@
 /*
- * oco_kernels.hip -- gfx950 kernels of the FP64 drop-in path.
+ * oco_kernels.hip -- the gfx950 kernels of the path.
  *
  *   cells_fp64  one lane per cell: Saha state, number densities and every
  *               temperature-only factor of the opacity (the reference's
@@ -16,9 +16,15 @@
  *               back-substitution fused with the ghost extrapolation, the
  *               6th-order derivative (7-point register window) and V.
  *
+ *   sweep_tiny_fp64  the drop-in for columns of 2 or 3 cells, one thread
+ *               per system.
  *   sweep_mixed_wave  the difference-form sweep of OCO_MODE_MIXED (opacity
  *               of oco_mixed.h) and OCO_MODE_ACCURATE (the drop-in's FP64
  *               opacity): an aligned group of L lanes per system.
+ *   batch_fp64, batch_mixed_wave  the same per column of a batch of uneven
+ *               columns; serve_tiles, serve_wave  the same for the server's
+ *               jobs, emergent flux only; athena_slab, opacity_points  input
+ *               construction and the opacity query.
  *
  * The FP64 path is compiled with -ffp-contract=off and its elementary
  * functions are oco_math.h's, read from an LDS copy of their tables.
@@ -139,9 +145,6 @@ __device__ static inline void record_fail(unsigned long long* fail, long sys, in
 #ifndef OCO_STAGE_ROWS
 #define OCO_STAGE_ROWS 16
 #endif
-#ifndef OCO_STAGE_ROWS_MIXED
-#define OCO_STAGE_ROWS_MIXED 16
-#endif
 /* How an FP64 back pass gets 1/alpha: OCO_A1_SCRATCH from the scratch row;
  * OCO_A1_DIRECT by reading alpha from the lane's own output; OCO_A1_STAGED
  * by loading alpha blocks of R rows into the V stage, coalesced (the reverse
@@ -169,7 +172,6 @@ struct oco_stage_t {
 #define OCO_BLOCK_A1 OCO_A1_STAGED
 #endif
 typedef oco_stage_t<OCO_STAGE_ROWS, true, OCO_BLOCK_A1> oco_stage;
-typedef oco_stage_t<OCO_STAGE_ROWS_MIXED> oco_stage_mixed;
 /* The batched kernels keep ordinary stores (their columns' ragged nz and nw
  * leave many partial lines, which the L2 combines) and the FP64 one 8 rows.
  * A batched column's alpha is one contiguous (nz, nw) array, so a lane's
@@ -180,7 +182,6 @@ typedef oco_stage_t<OCO_STAGE_ROWS_MIXED> oco_stage_mixed;
 #define OCO_BATCH_A1 OCO_A1_DIRECT
 #endif
 typedef oco_stage_t<8, false, OCO_BATCH_A1> oco_stage_batch;
-typedef oco_stage_t<OCO_STAGE_ROWS_MIXED, false> oco_stage_batch_mixed;
 
 /* Output policy of the serving kernels: only the emergent flux V(1), V(nz)
  * of each system is stored (d.V[0], d.V[1]); U and alpha are not written. */
@@ -310,13 +311,12 @@ __device__ static inline oco_pt point_rec(const oco_sweep_args& a, long col, int
     return oco_point_s<WS>(rec, w, M);
 }
 
-template <bool GREY, int WS = 1, bool MIX = false>
+template <bool GREY, int WS = 1>
 __device__ static inline oco_pt point_sys(const oco_sys& d, int z, const double* rec, const double* w,
                                           const oco_mtab* M)
 {
     if (GREY) return oco_grey_point(d.rho[z], d.T[z], d.sigma_grey, M);
-    if constexpr (MIX) return oco_point_mixed<WS>(rec, w);
-    else return oco_point_s<WS>(rec, w, M);
+    return oco_point_s<WS>(rec, w, M);
 }
 
 /* Tuning knob: OCO_SWEEP_WAVES_PER_EU=n asks the compiler to fit n waves per
@@ -328,29 +328,13 @@ __device__ static inline oco_pt point_sys(const oco_sys& d, int z, const double*
 #endif
 
 /* Elimination scratch of the lane-per-system sweeps, one row per call:
- *   FP64 drop-in: [gam, u, 1/alpha] as doubles, 24 bytes;
- *   mixed: [Z double][D as a float pair hi + lo][1/alpha float], 20 bytes,
- *   each of the three a coalesced line of 64 lanes.
- * D = hi + lo keeps ~2^-48 relative, so long runs of D -> 1 (optically thin
- * rows) add no error along the back-substitution; 1/alpha in FP32 costs
- * 6e-8 on V.  `scr` is the lane's slot pointer (wave slot + lane). */
-constexpr long SCR_ROW_MIXED = 160;   /* doubles per row of a wave's mixed slot: 64 + 64 + 32 */
-
-/* AR (OCO_A1_*, FP64): with a reload, the scratch row is [gam, u], 16 bytes. */
-template <bool MIX, int AR = OCO_A1_SCRATCH>
+ * [gam, u, 1/alpha] as doubles, each a coalesced line of 64 lanes; with
+ * 1/alpha recomputed from a reloaded alpha (AR, OCO_A1_*), [gam, u], 16
+ * bytes.  `scr` is the lane's slot pointer (wave slot + lane). */
+template <int AR = OCO_A1_SCRATCH>
 __device__ static inline void scr_put(double* scr, int j, double gam, double u, double a1)
 {
-    if constexpr (MIX) {
-        const int lane = lane_id();
-        double* sp = scr + (long)j * SCR_ROW_MIXED;
-        sp[0] = u;
-        const double D = -gam;
-        float2 pk;
-        pk.x = (float)D;
-        pk.y = (float)(D - (double)pk.x);
-        *(float2*)(sp + OCO_WAVE) = pk;
-        ((float*)(sp - lane + 2 * OCO_WAVE))[lane] = (float)a1;
-    } else if constexpr (AR != OCO_A1_SCRATCH) {
+    if constexpr (AR != OCO_A1_SCRATCH) {
         (void)a1;
         double* sp = scr + (long)j * 2 * OCO_WAVE;
         sp[0] = gam;
@@ -365,18 +349,11 @@ __device__ static inline void scr_put(double* scr, int j, double gam, double u, 
 
 /* a1: 1/alpha of row j (OCO_A1_DIRECT: alpha of row j, read from the lane's
  * output A, the caller takes a1_of; OCO_A1_STAGED: not set) */
-template <bool MIX, int AR = OCO_A1_SCRATCH>
+template <int AR = OCO_A1_SCRATCH>
 __device__ static inline void scr_get(const double* scr, int j, double& gam, double& u, double& a1,
                                       const double* A = nullptr)
 {
-    if constexpr (MIX) {
-        const int lane = lane_id();
-        const double* sp = scr + (long)j * SCR_ROW_MIXED;
-        u = sp[0];
-        const float2 pk = *(const float2*)(sp + OCO_WAVE);
-        gam = -((double)pk.x + (double)pk.y);
-        a1 = (double)((const float*)(sp - lane + 2 * OCO_WAVE))[lane];
-    } else if constexpr (AR == OCO_A1_DIRECT) {
+    if constexpr (AR == OCO_A1_DIRECT) {
         const double* sp = scr + (long)j * 2 * OCO_WAVE;
         gam = sp[0];
         u = sp[OCO_WAVE];
@@ -405,8 +382,8 @@ __device__ static inline double a1_of(double v)
 
 /* How a lane sweep with this mode and output policy gets 1/alpha in its back
  * pass (OCO_A1_*; alpha is written by the forward pass). */
-template <bool MIX, class STAGE>
-constexpr int alpha_reload() { return MIX || STAGE::emergent ? OCO_A1_SCRATCH : STAGE::reload; }
+template <class STAGE>
+constexpr int alpha_reload() { return STAGE::emergent ? OCO_A1_SCRATCH : STAGE::reload; }
 
 /* OCO_A1_STAGED: alpha rows [b*R, b*R + R) of every lane into registers
  * (thread t holds element t mod R of lanes t/R + k*64/R, the output flush's
@@ -435,14 +412,13 @@ __device__ static inline void alpha_park(double* buf, const double* regs)
 
 /* One (column, wavelength) system per lane, in two halves.  `w` is the
  * lane's wavelength-table row with stride WS; `scr` its scratch column
- * (stride 64 doubles per entry).  MIX selects the mixed-precision opacity and
- * the difference-form (FP64) elimination.
+ * (stride 64 doubles per entry).  The FP64 drop-in's arithmetic.
  *
  * Forward half: points (the opacity of the cells recomputed in registers),
  * rows and the elimination, alpha staged to its output; every row's gam, u
  * and 1/alpha go to scratch.  Returns true when the system hit the
  * reference's zero-pivot stop (FP64; recorded in `fail`). */
-template <bool GREY, bool UNIFORM, int WS, bool MIX, class STAGE>
+template <bool GREY, bool UNIFORM, int WS, class STAGE>
 __device__ static inline bool solve_forward(const oco_sys& d, bool valid, const double* w, double* scr,
                                             unsigned long long* fail, const oco_mtab* M, STAGE* S)
 {
@@ -462,62 +438,39 @@ __device__ static inline bool solve_forward(const oco_sys& d, bool valid, const 
      * whose work is independent of the elimination chain. */
     double rec[CR_NFIELD];
     if (!GREY) load_rec<UNIFORM>(d.cells, 0, rec);
-    oco_pt pB = point_sys<GREY, WS, MIX>(d, 0, rec, w, M);
+    oco_pt pB = point_sys<GREY, WS>(d, 0, rec, w, M);
     if (!GREY) load_rec<UNIFORM>(d.cells, 1, rec);
-    oco_pt pC = point_sys<GREY, WS, MIX>(d, 1, rec, w, M);
+    oco_pt pC = point_sys<GREY, WS>(d, 1, rec, w, M);
     if (!GREY) load_rec<UNIFORM>(d.cells, 2, rec);
 
     double bv, dv, av, cv;
     int fail_stage = 0;
     int fail_row = 1;
     double bet = 0.0, u;
-    /* mixed: difference form, x[j] = D[j] x[j+1] + Z[j] with om = 1 - D[j-1]
-     * carried separately; u holds Z, and the gam slot of row j holds
-     * -D[j-1], so the back pass U[j] = u[j] - gam[j+1] U[j+1] is shared */
-    double om = 1.0, Dprev = 0.0;
-    if constexpr (MIX) {
-        double H, R;
-        rh_boundary_row(pB, dz, dz2, H, R);  /* row 0: A = 0, C = alpha */
-        const double r = mx_rcp(pB.alpha + H);
-        Dprev = pB.alpha * r;
-        u = R * r;
-        om = H * r;
-    } else {
-        boundary_row(pB, dz, dz2, bv, dv);  /* row 0: a = 0, c = alpha */
-        cv = pB.alpha;
-        fail_stage = (bv == 0.0) ? 1 : 0;
-        bet = bv;
-        u = dv / bet;
-    }
-    scr_put<MIX, alpha_reload<MIX, STAGE>()>(scr, 0, 0.0, u, pB.alpha1);
+    boundary_row(pB, dz, dz2, bv, dv);  /* row 0: a = 0, c = alpha */
+    cv = pB.alpha;
+    fail_stage = (bv == 0.0) ? 1 : 0;
+    bet = bv;
+    u = dv / bet;
+    scr_put<alpha_reload<STAGE>()>(scr, 0, 0.0, u, pB.alpha1);
     if constexpr (!EM) stage_put<R>(S->ub, 0, pB.alpha);
     double aA = pB.alpha;   /* alpha of row j-1 */
     pB = pC;                /* row j = 1 */
-    pC = point_sys<GREY, WS, MIX>(d, 2, rec, w, M);
+    pC = point_sys<GREY, WS>(d, 2, rec, w, M);
 
     for (int j = 1; j <= nz - 3; j++) {
         if (!GREY) load_rec<UNIFORM>(d.cells, j + 2, rec);
         /* row j and its elimination step */
         double gam;
-        if constexpr (MIX) {
-            double A, C, H, R;
-            rh_interior_row(aA, pB, pC.alpha, dz2, A, C, H, R);
-            const double r = mx_rcp((A * om + C) + H);
-            gam = -Dprev;
-            Dprev = C * r;
-            u = (R + A * u) * r;
-            om = (A * om + H) * r;
-        } else {
-            double cprev = cv;
-            interior_row(aA, pB, pC.alpha, dz2, av, bv, cv, dv);
-            gam = cprev / bet;
-            bet = bv - av * gam;
-            int first = (bet == 0.0) && (fail_stage == 0);
-            fail_row = first ? j + 1 : fail_row;
-            fail_stage = first ? 2 : fail_stage;
-            u = (dv - av * u) / bet;
-        }
-        scr_put<MIX, alpha_reload<MIX, STAGE>()>(scr, j, gam, u, pB.alpha1);
+        double cprev = cv;
+        interior_row(aA, pB, pC.alpha, dz2, av, bv, cv, dv);
+        gam = cprev / bet;
+        bet = bv - av * gam;
+        int first = (bet == 0.0) && (fail_stage == 0);
+        fail_row = first ? j + 1 : fail_row;
+        fail_stage = first ? 2 : fail_stage;
+        u = (dv - av * u) / bet;
+        scr_put<alpha_reload<STAGE>()>(scr, j, gam, u, pB.alpha1);
         if constexpr (!EM) {
             stage_put<R>(S->ub, j, pB.alpha);
             if ((j & (R - 1)) == R - 1) {
@@ -527,7 +480,7 @@ __device__ static inline bool solve_forward(const oco_sys& d, bool valid, const 
             }
         }
         /* point j+2 (independent of the chain above) */
-        oco_pt pD = point_sys<GREY, WS, MIX>(d, j + 2, rec, w, M);
+        oco_pt pD = point_sys<GREY, WS>(d, j + 2, rec, w, M);
         aA = pB.alpha;
         pB = pC;
         pC = pD;
@@ -535,25 +488,15 @@ __device__ static inline bool solve_forward(const oco_sys& d, bool valid, const 
     {   /* row nz-2 (interior, no new point) and row nz-1 (boundary) */
         int j = nz - 2;
         double gam;
-        if constexpr (MIX) {
-            double A, C, H, R;
-            rh_interior_row(aA, pB, pC.alpha, dz2, A, C, H, R);
-            const double r = mx_rcp((A * om + C) + H);
-            gam = -Dprev;
-            Dprev = C * r;
-            u = (R + A * u) * r;
-            om = (A * om + H) * r;
-        } else {
-            double cprev = cv;
-            interior_row(aA, pB, pC.alpha, dz2, av, bv, cv, dv);
-            gam = cprev / bet;
-            bet = bv - av * gam;
-            int first = (bet == 0.0) && (fail_stage == 0);
-            fail_row = first ? j + 1 : fail_row;
-            fail_stage = first ? 2 : fail_stage;
-            u = (dv - av * u) / bet;
-        }
-        scr_put<MIX, alpha_reload<MIX, STAGE>()>(scr, j, gam, u, pB.alpha1);
+        double cprev = cv;
+        interior_row(aA, pB, pC.alpha, dz2, av, bv, cv, dv);
+        gam = cprev / bet;
+        bet = bv - av * gam;
+        int first = (bet == 0.0) && (fail_stage == 0);
+        fail_row = first ? j + 1 : fail_row;
+        fail_stage = first ? 2 : fail_stage;
+        u = (dv - av * u) / bet;
+        scr_put<alpha_reload<STAGE>()>(scr, j, gam, u, pB.alpha1);
         if constexpr (!EM) {
             stage_put<R>(S->ub, j, pB.alpha);
             if ((j & (R - 1)) == R - 1) {
@@ -563,13 +506,7 @@ __device__ static inline bool solve_forward(const oco_sys& d, bool valid, const 
             }
         }
         j = nz - 1;
-        if constexpr (MIX) {
-            double H, R;
-            rh_boundary_row(pC, dz, dz2, H, R);   /* A = alpha, C = 0 */
-            const double r = mx_rcp(pC.alpha * om + H);
-            gam = -Dprev;
-            u = (R + pC.alpha * u) * r;
-        } else {
+        {
             double cprev = cv;
             boundary_row(pC, dz, dz2, bv, dv);
             av = pC.alpha;                       /* a(nz) = alpha, c(nz) = 0 */
@@ -580,18 +517,16 @@ __device__ static inline bool solve_forward(const oco_sys& d, bool valid, const 
             fail_stage = first ? 2 : fail_stage;
             u = (dv - av * u) / bet;
         }
-        scr_put<MIX, alpha_reload<MIX, STAGE>()>(scr, j, gam, u, pC.alpha1);
+        scr_put<alpha_reload<STAGE>()>(scr, j, gam, u, pC.alpha1);
         if constexpr (!EM) {
             stage_put<R>(S->ub, j, pC.alpha);
             wave_sync();
             stage_flush<R, STAGE::nt>(S->ub, S->base[2], j / R, nz);   /* last (possibly partial) block */
         }
     }
-    (void)om;
-    (void)Dprev;
     (void)u;
     /* the reference stops at this system (source/auxiliary.f90:27-39): its
-     * U and V are not written (the difference form has no zero pivot) */
+     * U and V are not written */
     if (fail_stage != 0 && valid) record_fail(fail, d.key, fail_stage, fail_row);
     return fail_stage != 0;
 }
@@ -600,7 +535,7 @@ __device__ static inline bool solve_forward(const oco_sys& d, bool valid, const 
  * back-substitution fused with the ghosts, the derivative and V; U and V
  * staged to their outputs.  `valid` false: nothing is stored (a lane past
  * the end, or a system that stopped). */
-template <int BC, bool MIX, class STAGE>
+template <int BC, class STAGE>
 __device__ static inline void solve_backward(const oco_sys& d, bool valid, const double* scr, STAGE* S)
 {
     const int nz = d.nz;
@@ -619,7 +554,7 @@ __device__ static inline void solve_backward(const oco_sys& d, bool valid, const
     const double c60 = 1.0 / 60.0;
     /* window w0..w6 = extended U at e = j .. j+6 (e = z + 3) */
     double gnext, a1q0, Un;              /* gam of the row above; 1/alpha of the V row pending */
-    constexpr int AR = alpha_reload<MIX, STAGE>();
+    constexpr int AR = alpha_reload<STAGE>();
     constexpr bool ST = AR == OCO_A1_STAGED && !EM;
     /* OCO_A1_STAGED: the V stage's slot of row e holds alpha(e) until V(e)
      * replaces it; apf holds the next block's alpha, in flight */
@@ -646,7 +581,7 @@ __device__ static inline void solve_backward(const oco_sys& d, bool valid, const
             }
         }
     };
-    scr_get<MIX, AR>(scr, nz - 1, gnext, Un, a1q0, d.A);   /* U(nz) = u(nz) (:43) */
+    scr_get<AR>(scr, nz - 1, gnext, Un, a1q0, d.A);   /* U(nz) = u(nz) (:43) */
     a1q0 = a1_of<AR>(a1q0);
     double a1q1 = 0.0, a1q2 = 0.0, a1q3 = 0.0;
     double t[4];
@@ -654,7 +589,7 @@ __device__ static inline void solve_backward(const oco_sys& d, bool valid, const
     for (int k = 1; k < 4; k++) {
         int j = nz - 1 - k;
         double gj, uj, a1;
-        scr_get<MIX, AR>(scr, j, gj, uj, a1, d.A);
+        scr_get<AR>(scr, j, gj, uj, a1, d.A);
         double Uj = uj - gnext * t[k - 1];
         gnext = gj;
         a1q3 = a1q2; a1q2 = a1q1; a1q1 = a1q0; a1q0 = a1_of<AR>(a1);
@@ -703,13 +638,13 @@ __device__ static inline void solve_backward(const oco_sys& d, bool valid, const
 #pragma unroll
         for (int k = 0; k < BC; k++) {
             int j = jc - k;
-            scr_get<MIX, AR>(scr, j < 0 ? 0 : j, cg[k], cu[k], ca[k], d.A);
+            scr_get<AR>(scr, j < 0 ? 0 : j, cg[k], cu[k], ca[k], d.A);
         }
         for (; jc >= 0; jc -= BC) {
 #pragma unroll
             for (int k = 0; k < BC; k++) {      /* next chunk, in flight */
                 int j = jc - BC - k;
-                scr_get<MIX, AR>(scr, j < 0 ? 0 : j, ng[k], nu[k], na[k], d.A);
+                scr_get<AR>(scr, j < 0 ? 0 : j, ng[k], nu[k], na[k], d.A);
             }
 #pragma unroll
             for (int k = 0; k < BC; k++) {
@@ -768,16 +703,16 @@ __device__ static inline void solve_backward(const oco_sys& d, bool valid, const
 }
 
 /* One (column, wavelength) system, one lane: the two halves in turn. */
-template <bool GREY, bool UNIFORM, int WS, int BC, bool MIX, class STAGE>
+template <bool GREY, bool UNIFORM, int WS, int BC, class STAGE>
 __device__ static inline void solve_system(const oco_sys& d, bool valid, const double* w, double* scr,
                                            unsigned long long* fail, const oco_mtab* M, STAGE* S)
 {
-    const bool stopped = solve_forward<GREY, UNIFORM, WS, MIX, STAGE>(d, valid, w, scr, fail, M, S);
+    const bool stopped = solve_forward<GREY, UNIFORM, WS, STAGE>(d, valid, w, scr, fail, M, S);
     /* the back pass reads alpha where the forward pass stored it, through
      * other lanes of this wavefront (the staged flushes): a workgroup-scope
      * fence makes those stores visible to this workgroup's loads */
-    if constexpr (alpha_reload<MIX, STAGE>()) __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
-    solve_backward<BC, MIX, STAGE>(d, valid && !stopped, scr, S);
+    if constexpr (alpha_reload<STAGE>()) __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+    solve_backward<BC, STAGE>(d, valid && !stopped, scr, S);
 }
 
 /* One wavefront per workgroup, grid-stride over tiles of 64 systems; the
@@ -841,7 +776,7 @@ __device__ __forceinline__ long wave_take(unsigned long long* ctr, unsigned long
     return (long)(((unsigned long long)hi << 32) | lo);
 }
 
-template <bool GREY, bool UNIFORM, bool MIX, class STAGE>
+template <bool GREY, bool UNIFORM, class STAGE>
 __device__ static inline void sweep_tiles(const oco_sweep_args& a, const oco_mtab* M, STAGE* S)
 {
     const int lane = lane_id();
@@ -859,7 +794,7 @@ __device__ static inline void sweep_tiles(const oco_sweep_args& a, const oco_mta
 #pragma unroll
             for (int k = 0; k < OCO_WT_STRIDE; k++) w[k] = a.wtab[iw * OCO_WT_STRIDE + k];
         }
-        solve_system<GREY, UNIFORM, 1, OCO_BACK_CHUNK, MIX, STAGE>(d, valid, w, scr, a.fail, M, S);
+        solve_system<GREY, UNIFORM, 1, OCO_BACK_CHUNK, STAGE>(d, valid, w, scr, a.fail, M, S);
     }
 }
 
@@ -877,7 +812,7 @@ __global__ __launch_bounds__(OCO_WAVE * OCO_SWEEP_WG) OCO_SWEEP_ATTR void sweep_
     __shared__ oco_mtab smt;
     __shared__ oco_stage stg[OCO_SWEEP_WG];
     load_mtab(&smt);
-    sweep_tiles<GREY, UNIFORM, false>(a, &smt, &stg[threadIdx.x / OCO_WAVE]);
+    sweep_tiles<GREY, UNIFORM>(a, &smt, &stg[threadIdx.x / OCO_WAVE]);
 }
 
 /* First-pillar canaries (source/flux_feautrier.f90:198-201): sum(a), sum(b),
@@ -1701,7 +1636,7 @@ typedef const __attribute__((address_space(4))) int* oco_cptr_i;
 typedef const __attribute__((address_space(4))) long* oco_cptr_l;
 typedef const __attribute__((address_space(4))) double* oco_cptr_d;
 
-template <int BC, bool MIX, class STAGE>
+template <int BC, class STAGE>
 __device__ static inline void batch_tiles(const oco_batch_args& a, const oco_mtab* M, int* tile_s, STAGE* S)
 {
     const int lane = lane_id();
@@ -1744,7 +1679,7 @@ __device__ static inline void batch_tiles(const oco_batch_args& a, const oco_mta
             const double* wr = a.wtab + (woff[col] + iw) * OCO_WT_STRIDE;
 #pragma unroll
             for (int k = 0; k < OCO_WT_STRIDE; k++) w[k] = gptr(wr)[k];
-            solve_system<false, true, 1, BC, MIX, STAGE>(d, valid, w, scr, a.fail, M, S);
+            solve_system<false, true, 1, BC, STAGE>(d, valid, w, scr, a.fail, M, S);
         }
     }
 }
@@ -1756,7 +1691,7 @@ __global__ __launch_bounds__(OCO_WAVE) void batch_fp64(const oco_batch_args a)
     __shared__ int tile_s;
     __shared__ oco_stage_batch stg;
     load_mtab(&smt);
-    batch_tiles<BC, false>(a, &smt, &tile_s, &stg);
+    batch_tiles<BC>(a, &smt, &tile_s, &stg);
 }
 
 
@@ -1766,13 +1701,13 @@ __global__ __launch_bounds__(OCO_WAVE) void batch_fp64(const oco_batch_args a)
  * output.  A launch mixes tiles of every job admitted so far, so jobs that
  * arrive while others run join at the next launch (continuous batching);
  * each system is solved by one lane exactly as in a launch of its own. */
-template <int BC, bool MIX>
+template <int BC>
 __global__ __launch_bounds__(OCO_WAVE) void serve_tiles(const oco_serve_args a)
 {
     __shared__ oco_mtab smt;
     __shared__ int tile_s;
     __shared__ oco_stage_emergent none;
-    if (!MIX) load_mtab(&smt);
+    load_mtab(&smt);
     const int lane = lane_id();
     double* scr = a.scratch + (long)blockIdx.x * a.scr_stride + lane;
     oco_cptr_i tiles = (oco_cptr_i)a.tiles;
@@ -1800,8 +1735,7 @@ __global__ __launch_bounds__(OCO_WAVE) void serve_tiles(const oco_serve_args a)
         const double* wr = jb.wtab + (long)iw * OCO_WT_STRIDE;
 #pragma unroll
         for (int k = 0; k < OCO_WT_STRIDE; k++) w[k] = gptr(wr)[k];
-        solve_system<false, true, 1, BC, MIX, oco_stage_emergent>(d, valid, w, scr, a.fail + slot,
-                                                                  MIX ? nullptr : &smt, &none);
+        solve_system<false, true, 1, BC, oco_stage_emergent>(d, valid, w, scr, a.fail + slot, &smt, &none);
     }
 }
 
@@ -2058,14 +1992,14 @@ hipError_t oco_launch_serve(const oco_serve_args& a, int mixed, int nblocks, hip
         hipLaunchKernelGGL((serve_wave<OCO_SERVE_K>), dim3(nblocks), dim3(OCO_WAVE * OCO_SERVE_WG),
                            serve_wave_lds(a.win_rows), stream, a);
     else
-        hipLaunchKernelGGL((serve_tiles<2, false>), dim3(nblocks), dim3(OCO_WAVE), 0, stream, a);
+        hipLaunchKernelGGL((serve_tiles<2>), dim3(nblocks), dim3(OCO_WAVE), 0, stream, a);
     return hipGetLastError();
 }
 
 int oco_serve_grid(int mixed, int max_nz, int device)
 {
     int per_cu = 0, cus = 0;
-    const void* k = mixed ? (const void*)serve_wave<OCO_SERVE_K> : (const void*)serve_tiles<2, false>;
+    const void* k = mixed ? (const void*)serve_wave<OCO_SERVE_K> : (const void*)serve_tiles<2>;
     const size_t sh = mixed ? serve_wave_lds(oco_serve_win_rows(max_nz)) : 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, mixed ? OCO_WAVE * OCO_SERVE_WG : OCO_WAVE, sh) !=
             hipSuccess || per_cu < 1)
