@@ -57,10 +57,30 @@ WM_IONISED = 143 + 7 * (WM_DIV - 1) + WM_OMEXP + WM_EXP10 + WM_EXPM1 + 3 * WM_LO
 # rocprofv3 FETCH_SIZE / WRITE_SIZE passes over one launch, scaled by the
 # copy-kernel calibration (x2.0 / x1.0); workloads not listed report None
 SWEEP_TRAFFIC_BYTES = {
-    (256, 256, 512, 128, "sweep_mixed_wave"): 1.13610e11,
-    (256, 256, 512, 128, "sweep_mixed"): 3.40102e11,
-    (256, 256, 512, 128, "sweep_fp64"): 3.04761e11,
+    (256, 256, 512, 128, "sweep_mixed_wave"): 1.13004e11,
+    (256, 256, 512, 128, "sweep_fp64"): 3.04432e11,
 }
+
+
+def copy_rate_gbs(reps=5, nbytes=1 << 31):
+    """HBM rate of a plain device-to-device copy on this GPU (torch copy_ of
+    2 GiB, read + write counted, best of `reps`, HIP events): what this box's
+    memory sustains for a simple streaming pattern, next to the 8 TB/s peak."""
+    import torch
+    src = torch.ones(nbytes // 8, dtype=torch.float64, device="cuda:0")
+    dst = torch.empty_like(src)
+    best = None
+    for _ in range(reps + 1):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        dst.copy_(src)
+        e1.record()
+        e1.synchronize()
+        ms = e0.elapsed_time(e1)
+        best = ms if best is None or ms < best else best
+    del src, dst
+    torch.cuda.empty_cache()
+    return 2.0 * nbytes / (best * 1e-3) / 1e9
 
 
 def ionised_fraction(rho, T, fully_ionized_T=2e4):
@@ -207,6 +227,8 @@ def bench_block(args):
     d_A = torch.empty((nw, nx, ny, nz), dtype=torch.float64, device=dev)
     torch.cuda.synchronize()
 
+    copy_gbs = copy_rate_gbs()
+    log("plain device copy: %.0f GB/s" % copy_gbs)
     ctx = oco.Context(0)
     ctx.set_wavelengths(waves)
     ctx.set_physics(2e4, False, 0.4)
@@ -238,6 +260,14 @@ def bench_block(args):
         kern = "sweep_fp64" if mode == "fp64" else mixed_kernel_name(nz)
         traffic = SWEEP_TRAFFIC_BYTES.get((nx, ny, nz, nw, kern))
         roof = roofline_of(kern, points, point_weights(mode, fion), sw, traffic)
+        if traffic is not None:
+            # the launch's measured HBM traffic over its time, against the peak
+            # and against a plain copy's rate on this box
+            tr = traffic / (sw * 1e-3) / 1e9
+            roof["traffic_GBps"] = round(tr, 1)
+            roof["traffic_frac_of_peak"] = round(tr / HBM_PEAK_GBS, 4)
+            roof["copy_GBps"] = round(copy_gbs, 1)
+            roof["traffic_frac_of_copy"] = round(tr / copy_gbs, 4)
         roof["cells_kernel_ms"] = round(float(np.mean(cells_ms)), 4)
         gbs = sweep_bytes(nz, ncol, nw) / (sw * 1e-3) / 1e9
         roof_hbm = {"kernel": kern, "bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS,
