@@ -234,7 +234,7 @@ def bench_block(args):
     ctx.set_physics(2e4, False, 0.4)
     fion = ionised_fraction(rho, T)
     res = {}
-    gpu_sums = None
+    gpu_sums = gpu_cols = None
     codes = {"fp64": oco.MODE_FP64, "mixed": oco.MODE_MIXED, "accurate": oco.MODE_ACCURATE}
     for mode in modes:
         code = codes[mode]
@@ -275,9 +275,12 @@ def bench_block(args):
                     "bytes_per_point": round(sweep_bytes(nz, ncol, nw) / points, 3)}
         res[mode] = {"value": points / (ms * 1e-3), "ms_per_step": ms, "roofline": roof, "roofline_hbm": roof_hbm}
         if mode == "fp64":
-            # the CPU baseline's sample columns of the FP64 outputs, for its checksum comparison
+            # the CPU baseline's sample columns of the FP64 outputs, for its checksum comparison,
+            # and the sample's first columns (ix = 0, iy < 16) point by point
             sx, sy = (min(48, nx), min(48, ny)) if args.config == "c4" else (nx, ny)
             gpu_sums = [float(x[:, :sx, :sy].sum()) for x in (d_U, d_V, d_A)]
+            nd = min(16, sy)
+            gpu_cols = tuple(x[:, 0, :nd].cpu().numpy() for x in (d_U, d_V, d_A))
     del d_U, d_V, d_A
     torch.cuda.empty_cache()
     athena = bench_athena(ctx, d_rho, d_T, args.steps)
@@ -288,7 +291,8 @@ def bench_block(args):
 
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline(rho, T, dz, waves, args.config, gpu_sums if "fp64" in modes else None)
+        cpu = cpu_baseline(rho, T, dz, waves, args.config, gpu_sums if "fp64" in modes else None,
+                           gpu_cols if "fp64" in modes else None)
     serving = None
     if args.config == "c4" and not args.no_serve:
         del d_rho, d_T
@@ -556,7 +560,27 @@ def accuracy_pass(ctx, d_rho, d_T, dz, nx, ny, nz, nw, nchunks=4):
     }
 
 
-def cpu_baseline(rho, T, dz, waves, config, gpu_sums=None):
+def compare_points(gpu, ref):
+    """The FP64 drop-in's U, V, alpha against the reference's on the same
+    columns ([nw, ncol, nz(+6)] each): U and alpha relative to each value;
+    the emergent V(1), V(nz) relative to each value; interior V relative to
+    its system's largest |V| (V changes sign inside the column)."""
+    gU, gV, gA = gpu
+    rU, rV, rA = ref
+    out = {"columns": int(rU.shape[1]), "points": int(rA.size)}
+    for name, g, r in (("U", gU, rU), ("alpha", gA, rA)):
+        rel = np.abs(g - r) / np.abs(r)
+        out[name + "_frac_le_1e-12"] = round(float((rel <= 1e-12).mean()), 6)
+        out[name + "_max_rel"] = float(rel.max())
+    em = np.abs(gV[:, :, [0, -1]] - rV[:, :, [0, -1]]) / np.abs(rV[:, :, [0, -1]])
+    out["V_emergent_frac_le_1e-12"] = round(float((em <= 1e-12).mean()), 6)
+    out["V_emergent_max_rel"] = float(em.max())
+    scale = np.abs(rV).max(axis=2, keepdims=True)
+    out["V_max_rel_to_system_max"] = float((np.abs(gV - rV) / scale).max())
+    return out
+
+
+def cpu_baseline(rho, T, dz, waves, config, gpu_sums=None, gpu_cols=None):
     """The reference on the host's cores beside the GPU: its own modules
     compiled at -O3 (oracle/_ref/time_*/ref_time.x, built by
     oracle/ref/build_ref.sh), one process per available thread on disjoint
@@ -564,7 +588,10 @@ def cpu_baseline(rho, T, dz, waves, config, gpu_sums=None):
     to the last exit (kind "reference").  Without that build: the FP64 port
     (the oracle, pinned bit-exact to the reference) with OpenMP over the same
     threads (kind "port").  gpu_sums: the FP64 GPU outputs' sums of U, V and
-    alpha over the sample's columns, compared with the reference's own."""
+    alpha over the sample's columns, compared with the reference's own;
+    gpu_cols: the FP64 GPU outputs of the sample's first columns (ix = 0,
+    iy < ncol), compared point by point with the reference's (written by its
+    first process after its timed loop)."""
     from oracle import reference as R
     from oracle.oracle import Oracle
     nthreads = R.host_threads()
@@ -578,7 +605,11 @@ def cpu_baseline(rho, T, dz, waves, config, gpu_sums=None):
     pts = float(r.size) * waves.size
     out = {"unit": "points/s", "cores": nthreads, "cpu": R.cpu_model()}
     if R.binary(r.shape[2], waves.size):
-        wall, lines = R.run(r, t, dz, waves, nthreads)
+        nd = gpu_cols[0].shape[1] if gpu_cols is not None else 0
+        got = R.run(r, t, dz, waves, nthreads, dump_cols=nd)
+        wall, lines = got[0], got[1]
+        if nd:
+            out["fp64_gpu_vs_reference_points"] = compare_points(gpu_cols, got[2])
         sums = [sum(l[k] for l in lines) for k in (5, 6, 7)]
         out.update(value=round(pts / wall, 1), kind="reference",
                    sample="%s: %.4g points, %d processes of the reference (amdflang -O3) on disjoint column "

@@ -12,11 +12,14 @@
 ! nw)).  nz and nw come from the resolution.in the recipe writes next to the
 ! objects (compile-time shapes, source/common.f90:9).
 !
-! Usage: ref_time.x <input.bin> <gas_state namelist file>
+! Usage: ref_time.x <input.bin> <gas_state namelist file> [<dump.bin> <ndump>]
 !   input.bin (stream, little endian):
 !     int32 ncol, int32 lgrey, float64 dz, float64 sigma_grey,
 !     float64 waves(nw), float64 rho(nz,ncol), float64 T(nz,ncol)
 ! Prints one line: ncol nz nw wall_s cpu_s sum(U) sum(V) sum(absorp).
+! With a dump file, after the timed loop, writes U(:,1:ndump,:),
+! V(:,1:ndump,:) and absorp(:,1:ndump,:) to it (stream, in that order) for
+! a point-by-point comparison.
 program ref_time
   use Common
   use GasState
@@ -31,8 +34,9 @@ program ref_time
   integer(4) :: ncol, lgrey_i
   integer :: icol, iw, iu
   integer(8) :: t0, t1, rate
-  character(len=256) :: fin, fnml
+  character(len=256) :: fin, fnml, fdump, sdump
   character(len=90) :: nml90
+  integer :: ndump
 
   call get_command_argument(1, fin)
   call get_command_argument(2, fnml)
@@ -80,4 +84,15 @@ program ref_time
   call cpu_time(c1)
   call system_clock(t1)
   print '(3I8,2ES16.8,3ES24.16)', ncol, nz, nw, real(t1 - t0) / real(rate), c1 - c0, sum(U), sum(V), sum(absorp)
+  if (command_argument_count() >= 4) then
+    call get_command_argument(3, fdump)
+    call get_command_argument(4, sdump)
+    read(sdump, *) ndump
+    ndump = min(ndump, int(ncol))
+    open(newunit=iu, file=trim(fdump), form='unformatted', access='stream', status='replace')
+    write(iu) U(:,1:ndump,:)
+    write(iu) V(:,1:ndump,:)
+    write(iu) absorp(:,1:ndump,:)
+    close(iu)
+  endif
 endprogram ref_time

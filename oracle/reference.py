@@ -44,11 +44,15 @@ def write_input(path, rho_cols, T_cols, dz, waves, lgrey=False, sigma_grey=0.4):
         f.write(T_cols.tobytes())
 
 
-def run(rho3d, T3d, dz, waves, nprocs, fully_ionized_T=2e4):
+def run(rho3d, T3d, dz, waves, nprocs, fully_ionized_T=2e4, dump_cols=0):
     """The block's columns (rho3d, T3d [nx, ny, nz]) split over nprocs
     processes of the reference started together.  Returns (wall seconds from
     the first start to the last exit, list of per-process result lines:
-    ncol, nz, nw, loop wall s, loop cpu s, sum U, sum V, sum absorp)."""
+    ncol, nz, nw, loop wall s, loop cpu s, sum U, sum V, sum absorp).  With
+    dump_cols > 0, the first process also writes its first dump_cols
+    columns' outputs (after its timed loop), and a third item is returned:
+    (U [nw, dump_cols, nz+6], V [nw, dump_cols, nz], absorp [nw, dump_cols,
+    nz]), the block's columns in (ix, iy) order, iy fastest."""
     nx, ny, nz = rho3d.shape
     exe = binary(nz, len(waves))
     if exe is None:
@@ -65,16 +69,28 @@ def run(rho3d, T3d, dz, waves, nprocs, fully_ionized_T=2e4):
             fi = os.path.join(td, "in%d.bin" % i)
             write_input(fi, cols_rho[s], cols_T[s], dz, waves)
             files.append(fi)
+        nd = min(int(dump_cols), len(slices[0]))
+        dump = os.path.join(td, "dump.bin")
+        cmds = [[exe, fi, nml] + ([dump, str(nd)] if i == 0 and nd > 0 else []) for i, fi in enumerate(files)]
         t0 = time.perf_counter()
-        procs = [subprocess.Popen([exe, fi, nml], stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True)
-                 for fi in files]
+        procs = [subprocess.Popen(c, stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True) for c in cmds]
         outs = [p.communicate() for p in procs]
         wall = time.perf_counter() - t0
-    for p, (o, e) in zip(procs, outs):
-        if p.returncode != 0:
-            raise RuntimeError("reference run failed: %s %s" % (o[-500:], e[-500:]))
-    lines = [[float(x) for x in o.split()] for o, _ in outs]
-    return wall, lines
+        for p, (o, e) in zip(procs, outs):
+            if p.returncode != 0:
+                raise RuntimeError("reference run failed: %s %s" % (o[-500:], e[-500:]))
+        lines = [[float(x) for x in o.split()] for o, _ in outs]
+        if nd <= 0:
+            return wall, lines
+        nw = len(waves)
+        raw = np.fromfile(dump, dtype="<f8")
+        nu, nv = nw * nd * (nz + 6), nw * nd * nz
+        if raw.size != nu + 2 * nv:
+            raise RuntimeError("reference dump: %d doubles, expected %d" % (raw.size, nu + 2 * nv))
+        U = raw[:nu].reshape(nw, nd, nz + 6)
+        V = raw[nu:nu + nv].reshape(nw, nd, nz)
+        A = raw[nu + nv:].reshape(nw, nd, nz)
+    return wall, lines, (U, V, A)
 
 
 def cpu_model():
