@@ -53,27 +53,68 @@ def _run(cmd, cwd=None):
     subprocess.check_call(cmd, cwd=cwd)
 
 
-def build_lib(out=LIB, defines=(), build_dir=BUILD):
+# Host sanitizers for the C-ABI layer (the library's host code: the C++
+# files and the host side of the HIP file; device code is never
+# instrumented).  On a hipcc line each flag goes after -Xarch_host.
+SANITIZE = ("address", "undefined")
+SAN_LIB = os.path.join(PKG, "libocotillo_san.so")
+SAN_BUILD = os.path.join(PKG, "_build_san")
+
+
+def _san_host_flags():
+    return ["-fsanitize=" + s for s in SANITIZE] + ["-fno-omit-frame-pointer", "-fno-sanitize-recover=all"]
+
+
+def _san_hip_flags():
+    out = []
+    for f in _san_host_flags():
+        out += ["-Xarch_host", f]
+    return out
+
+
+def build_lib(out=LIB, defines=(), build_dir=BUILD, sanitize=False):
     """Compile and link the library.  `defines` are extra -D tuning macros
-    (e.g. OCO_SWEEP_WAVES_PER_EU=4) for the kernels."""
+    (e.g. OCO_SWEEP_WAVES_PER_EU=4) for the kernels; `sanitize` builds the
+    host code with AddressSanitizer and UndefinedBehaviorSanitizer (a program
+    that loads such a library must itself be built with them)."""
     os.makedirs(build_dir, exist_ok=True)
     objs = []
     for src, device in (("oco_kernels.hip", True), ("oco_api.cpp", False), ("oco_serve.cpp", False),
                         ("oco_output.cpp", False)):
         obj = os.path.join(build_dir, os.path.splitext(src)[0] + ".o")
         if device:
-            cmd = [HIPCC] + COMMON + DEVICE + ["-D" + d for d in defines] + ["-c", os.path.join(CSRC, src), "-o", obj]
+            cmd = ([HIPCC] + COMMON + DEVICE + (_san_hip_flags() if sanitize else []) + ["-D" + d for d in defines] +
+                   ["-c", os.path.join(CSRC, src), "-o", obj])
         else:   # host-only C++ (the C ABI): no device pass
-            cmd = [CXX] + COMMON + ["-pthread", "-D__HIP_PLATFORM_AMD__", "-I" + os.path.join(ROCM, "include"),
-                                    "-c", os.path.join(CSRC, src), "-o", obj]
+            cmd = ([CXX] + COMMON + (_san_host_flags() if sanitize else []) +
+                   ["-pthread", "-D__HIP_PLATFORM_AMD__", "-I" + os.path.join(ROCM, "include"),
+                    "-c", os.path.join(CSRC, src), "-o", obj])
         _run(cmd)
         if device:
             check_no_packed_fp32(obj)
         objs.append(obj)
     tmp = out + ".tmp"
-    _run([HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-pthread", "-o", tmp] + objs)
+    _run([HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-pthread"] + (_san_hip_flags() if sanitize else []) +
+         ["-o", tmp] + objs)
     os.replace(tmp, out)
     return out
+
+
+SAN_DRIVER_SRC = os.path.join(os.path.dirname(PKG), "tests", "sanitize", "abi_check.c")
+SAN_DRIVER = os.path.join(os.path.dirname(PKG), "tests", "sanitize", "abi_check")
+
+
+def build_sanitized():
+    """The sanitized library (libocotillo_san.so) and the C program that
+    drives its entry points under the sanitizers (tests/sanitize/abi_check,
+    test infrastructure)."""
+    build_lib(out=SAN_LIB, build_dir=SAN_BUILD, sanitize=True)
+    if os.path.exists(SAN_DRIVER_SRC):
+        _run([os.path.join(ROCM, "llvm", "bin", "clang"), "-O1", "-g", "-std=gnu11", "-Wall"] + _san_host_flags() +
+             ["-D__HIP_PLATFORM_AMD__", "-I" + os.path.join(ROCM, "include"), SAN_DRIVER_SRC, "-o", SAN_DRIVER,
+              "-L" + PKG, "-locotillo_san", "-Wl,-rpath," + PKG, "-Wl,-rpath,$ORIGIN/../../ocotillo_amd",
+              "-L" + os.path.join(ROCM, "lib"), "-lamdhip64", "-Wl,-rpath," + os.path.join(ROCM, "lib"), "-lm"])
+    return SAN_LIB
 
 
 def build_fortran():
@@ -87,6 +128,7 @@ def build_fortran():
 
 def build():
     build_lib()
+    build_sanitized()
     build_fortran()
 
 
