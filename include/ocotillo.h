@@ -35,7 +35,8 @@ enum {
     OCO_E_STATE = 6,
     OCO_E_NODEVICE = 7,
     OCO_E_BUSY = 8,
-    OCO_E_IO = 9
+    OCO_E_IO = 9,
+    OCO_E_NONFINITE = 10   /* an output is NaN or infinite (OCO_CHECK_FINITE) */
 };
 
 /* Precision modes of oco_solve_block*.  MIXED and ACCURATE solve the same
@@ -80,6 +81,22 @@ int oco_destroy(oco_ctx* ctx);
  * (source/continuous_opacity.f90:64-141): builds the per-wavelength tables
  * and keeps them resident on the device. */
 int oco_set_wavelengths(oco_ctx* ctx, int nw, const double* waves_angstrom);
+
+/* Numeric checks of the block solves (oco_solve_block, _device, _noalloc,
+ * _files), off by default; also set by the environment variable
+ * OCO_CHECK_FINITE=1 at oco_create.  OCO_CHECK_FINITE: after the solve, every
+ * U, V and absorp value is checked, and a NaN or infinity anywhere turns the
+ * result into OCO_E_NONFINITE with fail_ix, fail_iy, fail_iw of the first
+ * such (column, wavelength) system in the reference's loop order and fail_j
+ * its first such cell (1-based; its absorp first, then its U, whose ghosts
+ * count as -2..0 and nz+1..nz+3, then its V); oco_solve_block_files then
+ * writes no file.  The device counterpart of the reference's debug build
+ * flags -ffpe-trap=invalid,zero,overflow (source/Makefile.source:8), which
+ * stop the run at the first invalid, divide-by-zero or overflow operation;
+ * it costs one read of the outputs (about 20 ms on a 256 x 256 x 512 block
+ * of 128 wavelengths). */
+#define OCO_CHECK_FINITE 1
+int oco_set_checks(oco_ctx* ctx, int flags);
 
 /* Replaces the &gas_state_input namelist value fully_ionized_T
  * (source/gas_state.f90:14-28) and the &input values lgrey, sigma_grey

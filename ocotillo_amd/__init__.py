@@ -23,11 +23,12 @@ __all__ = ["OcoError", "Context", "Batch", "Server", "Diag", "AthenaUnits", "loa
 PKG = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(PKG, "libocotillo.so")
 
-OK, E_PIVOT0_STAGE1, E_PIVOT0_STAGE2, E_ARG, E_OOM, E_HIP, E_STATE, E_NODEVICE, E_BUSY, E_IO = range(10)
+OK, E_PIVOT0_STAGE1, E_PIVOT0_STAGE2, E_ARG, E_OOM, E_HIP, E_STATE, E_NODEVICE, E_BUSY, E_IO, E_NONFINITE = range(11)
 MODE_FP64, MODE_MIXED, MODE_ACCURATE = 0, 1, 2
+CHECK_FINITE = 1
 
 # every symbol include/ocotillo.h declares
-EXPORTS = ["oco_create", "oco_destroy", "oco_set_wavelengths", "oco_set_physics", "oco_solve_block",
+EXPORTS = ["oco_create", "oco_destroy", "oco_set_wavelengths", "oco_set_physics", "oco_set_checks", "oco_solve_block",
            "oco_solve_block_device", "oco_solve_block_device_noalloc", "oco_strerror", "oco_version", "oco_athena_slab", "oco_athena_slab_device",
            "oco_solve_block_files", "oco_write_record", "oco_opacity_points",
            "oco_batch_create", "oco_batch_destroy", "oco_batch_layout", "oco_batch_buffers", "oco_batch_run",
@@ -91,6 +92,7 @@ def load_library(path=None):
     L.oco_destroy.argtypes = [C.c_void_p]
     L.oco_set_wavelengths.argtypes = [C.c_void_p, C.c_int, _DP]
     L.oco_set_physics.argtypes = [C.c_void_p, C.c_double, C.c_int, C.c_double]
+    L.oco_set_checks.argtypes = [C.c_void_p, C.c_int]
     L.oco_solve_block.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double,
                                   _DP, _DP, _DP, _DP, _DP, C.c_int, C.POINTER(Diag)]
     L.oco_solve_block_device.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double,
@@ -192,6 +194,13 @@ class Context:
         st = self.L.oco_set_physics(self.h, fully_ionized_T, int(bool(lgrey)), sigma_grey)
         if st != OK:
             raise OcoError(st, "oco_set_physics: " + strerror(st))
+
+    def set_checks(self, finite=False):
+        """finite=True: every block solve checks its outputs and returns
+        E_NONFINITE (OcoError) at the first NaN or infinity (OCO_CHECK_FINITE)."""
+        st = self.L.oco_set_checks(self.h, CHECK_FINITE if finite else 0)
+        if st != OK:
+            raise OcoError(st, "oco_set_checks: " + strerror(st))
 
     def solve_block(self, rho3d, T3d, dz, dz1=None, dz2=None, mode=MODE_FP64, out=None, raise_on_pivot=True):
         """Host arrays in, host arrays out.  Returns (U, V, absorp, Diag)."""

@@ -43,7 +43,8 @@ struct oco_ctx {
     double sigma_grey = 0.4;
     int wave_lanes = 0;   /* lanes per system of the difference-form sweep: 8, 16, 32, 64, 0 = by nz */
     int wave_group = 0;   /* wavefronts per workgroup of the difference-form sweep: 1, 2, 4, 0 = default */
-    DevBuf wtab, rho, T, cells, scratch, U, V, A, fail, canary, raw, tg, tickets;
+    int checks = 0;       /* OCO_CHECK_* flags of the block solves */
+    DevBuf wtab, rho, T, cells, scratch, U, V, A, fail, canary, raw, tg, tickets, fkey;
     /* block outputs to files: copy stream, per-slot events, pinned staging */
     hipStream_t copy_stream = nullptr;
     hipEvent_t ev_sweep[2] = {nullptr, nullptr}, ev_copy[2] = {nullptr, nullptr};
@@ -190,6 +191,7 @@ const char* oco_strerror(int s)
     case OCO_E_NODEVICE: return "no HIP device";
     case OCO_E_BUSY: return "context busy";
     case OCO_E_IO: return "output file could not be written";
+    case OCO_E_NONFINITE: return "an output is NaN or infinite (OCO_CHECK_FINITE)";
     default: return "unknown status";
     }
 }
@@ -215,6 +217,8 @@ int oco_create(int device, size_t hbm_budget, oco_ctx** out)
     /* OCO_WAVE_GROUP=1|2|4: wavefronts per workgroup of the difference-form
      * sweep (the same bits; default: the library's measured choice) */
     if (const char* e = getenv("OCO_WAVE_GROUP")) c->wave_group = atoi(e);
+    /* OCO_CHECK_FINITE=1: oco_set_checks(ctx, OCO_CHECK_FINITE) from the start */
+    if (const char* e = getenv("OCO_CHECK_FINITE")) c->checks = atoi(e) ? OCO_CHECK_FINITE : 0;
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
         delete c;
         return OCO_E_HIP;
@@ -241,7 +245,7 @@ int oco_destroy(oco_ctx* c)
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     for (DevBuf* b : {&c->wtab, &c->rho, &c->T, &c->cells, &c->scratch, &c->U, &c->V, &c->A, &c->fail,
-                      &c->canary, &c->raw, &c->tg, &c->tickets})
+                      &c->canary, &c->raw, &c->tg, &c->tickets, &c->fkey})
         release(c, *b);
     for (auto& e : c->ev)
         if (e) (void)hipEventDestroy(e);
@@ -282,6 +286,13 @@ int oco_set_wavelengths(oco_ctx* c, int nw, const double* waves)
     }
     c->waves = std::move(wv);
     c->nw = nw;
+    return OCO_OK;
+}
+
+int oco_set_checks(oco_ctx* c, int flags)
+{
+    if (!c || (flags & ~OCO_CHECK_FINITE)) return OCO_E_ARG;
+    c->checks = flags;
     return OCO_OK;
 }
 
@@ -410,6 +421,34 @@ static void pillar_minmax(const double* U, const double* V, const double* A, int
     }
 }
 
+/* OCO_CHECK_FINITE: enqueue the non-finite scan of one chunk of a block's
+ * outputs (wavelengths iw0 .. iw0 + nw_run - 1 of nw) into *key. */
+static hipError_t finite_scan(int nz, int nyloc, int nxloc, int iw0, int nw_run, int nw, const double* U,
+                              const double* V, const double* A, unsigned long long* key, hipStream_t s)
+{
+    const long ncol = (long)nxloc * nyloc;
+    hipError_t e = oco_launch_finite_scan(A, ncol * nz * nw_run, nz, 0, nyloc, nxloc, iw0, nw, nz, key, s);
+    if (e == hipSuccess)
+        e = oco_launch_finite_scan(U, ncol * (nz + 6) * nw_run, nz + 6, 1, nyloc, nxloc, iw0, nw, nz, key, s);
+    if (e == hipSuccess) e = oco_launch_finite_scan(V, ncol * nz * nw_run, nz, 2, nyloc, nxloc, iw0, nw, nz, key, s);
+    return e;
+}
+
+/* the first non-finite output (finite_scan's key) as OCO_E_NONFINITE and its
+ * system and cell */
+static void decode_finite(unsigned long long key, int nz, int nw, int nyloc, oco_diag* d)
+{
+    const unsigned long long mz = (unsigned long long)(nz + 6);
+    const long sys = (long)(key / (3 * mz));
+    const int which = (int)((key / mz) % 3), e = (int)(key % mz);
+    const long col = sys / nw;
+    d->status = OCO_E_NONFINITE;
+    d->fail_iw = (int)(sys % nw) + 1;
+    d->fail_ix = (int)(col / nyloc) + 1;
+    d->fail_iy = (int)(col % nyloc) + 1;
+    d->fail_j = which == 1 ? e - 2 : e + 1;
+}
+
 extern "C" {
 
 int oco_solve_block_device(oco_ctx* c, int nz, int nyloc, int nxloc, double dz, double dz1, double dz2,
@@ -428,9 +467,12 @@ int oco_solve_block_device(oco_ctx* c, int nz, int nyloc, int nxloc, double dz, 
     if (!p.ok) return OCO_E_ARG;
     int st;
     if ((st = sweep_buffers(c, p, nz, ncells))) return st;
+    const bool check = (c->checks & OCO_CHECK_FINITE) != 0;
+    if (check && (st = ensure(c, c->fkey, sizeof(unsigned long long)))) return st;
 
     const unsigned long long none = OCO_FAIL_NONE;
     HIPCHK(hipMemcpyAsync(c->fail.p, &none, sizeof(none), hipMemcpyHostToDevice, c->stream));
+    if (check) HIPCHK(oco_launch_fail_reset((unsigned long long*)c->fkey.p, 1, c->stream));
 
     oco_sweep_args a = sweep_args(c, nz, ncol, dz, dz1, dz2, d_rho, d_T);
     a.U = d_U;
@@ -444,6 +486,12 @@ int oco_solve_block_device(oco_ctx* c, int nz, int nyloc, int nxloc, double dz, 
     HIPCHK(hipEventRecord(c->ev[1], c->stream));
     HIPCHK(oco_launch_sweep(a, c->lgrey, p.wave, p.lanes, p.pk, p.nblocks, c->stream));
     HIPCHK(hipEventRecord(c->ev[2], c->stream));
+    unsigned long long fkey = none;
+    if (check) {
+        HIPCHK(finite_scan(nz, nyloc, nxloc, 0, c->nw, c->nw, d_U, d_V, d_A, (unsigned long long*)c->fkey.p,
+                           c->stream));
+        HIPCHK(hipMemcpyAsync(&fkey, c->fkey.p, sizeof(fkey), hipMemcpyDeviceToHost, c->stream));
+    }
 
     unsigned long long key = none;
     double can[4] = {0, 0, 0, 0};
@@ -470,6 +518,7 @@ int oco_solve_block_device(oco_ctx* c, int nz, int nyloc, int nxloc, double dz, 
         HIPCHK(hipMemcpy(h.data() + nz + 6, d_V, nz * sizeof(double), hipMemcpyDeviceToHost));
         HIPCHK(hipMemcpy(h.data() + 2 * nz + 6, d_A, nz * sizeof(double), hipMemcpyDeviceToHost));
         pillar_minmax(h.data(), h.data() + nz + 6, h.data() + 2 * nz + 6, nz, &d);
+        if (fkey != none) decode_finite(fkey, nz, c->nw, nyloc, &d);
     }
     if (diag) *diag = d;
     return d.status;
@@ -479,7 +528,7 @@ int oco_solve_block_device(oco_ctx* c, int nz, int nyloc, int nxloc, double dz, 
  * failure key, canary sums, first-pillar min/max, then the field-major cell
  * records (none for grey columns), each 256-byte aligned; the total. */
 struct StaticLayout {
-    size_t tickets, fail, canary, minmax, cells, total;
+    size_t tickets, fail, fkey, canary, minmax, cells, total;
 };
 
 static size_t align256(size_t n) { return (n + 255) & ~(size_t)255; }
@@ -489,7 +538,8 @@ static StaticLayout static_layout(int nz, long ncells, int lgrey)
     StaticLayout L;
     L.tickets = 0;
     L.fail = L.tickets + align256(OCO_TICKET_BYTES);
-    L.canary = L.fail + align256(sizeof(unsigned long long));
+    L.fkey = L.fail + align256(sizeof(unsigned long long));
+    L.canary = L.fkey + align256(sizeof(unsigned long long));
     L.minmax = L.canary + align256((4 + 5 * (size_t)nz) * sizeof(double));
     L.cells = L.minmax + align256(6 * sizeof(double));
     L.total = L.cells + (lgrey ? 0 : (size_t)ncells * CR_NFIELD * sizeof(double));
@@ -543,6 +593,13 @@ int oco_solve_block_device_noalloc(oco_ctx* c, int nz, int nyloc, int nxloc, dou
     HIPCHK(oco_launch_sweep(a, c->lgrey, p.wave, p.lanes, p.pk, p.nblocks, c->stream));   /* resets the tickets */
     HIPCHK(hipEventRecord(c->ev[2], c->stream));
     HIPCHK(oco_launch_pillar_minmax(d_U, d_V, d_A, nz, minmax, c->stream));
+    unsigned long long fkey = OCO_FAIL_NONE;
+    if (c->checks & OCO_CHECK_FINITE) {
+        unsigned long long* dk = (unsigned long long*)(base + L.fkey);
+        HIPCHK(oco_launch_fail_reset(dk, 1, c->stream));
+        HIPCHK(finite_scan(nz, nyloc, nxloc, 0, c->nw, c->nw, d_U, d_V, d_A, dk, c->stream));
+        HIPCHK(hipMemcpyAsync(&fkey, dk, sizeof(fkey), hipMemcpyDeviceToHost, c->stream));
+    }
 
     unsigned long long key = OCO_FAIL_NONE;
     double can[4] = {0, 0, 0, 0}, mm[6] = {0, 0, 0, 0, 0, 0};
@@ -567,6 +624,7 @@ int oco_solve_block_device_noalloc(oco_ctx* c, int nz, int nyloc, int nxloc, dou
         d.minU = mm[0]; d.maxU = mm[1];
         d.minV = mm[2]; d.maxV = mm[3];
         d.minA = mm[4]; d.maxA = mm[5];
+        if (fkey != OCO_FAIL_NONE) decode_finite(fkey, nz, c->nw, nyloc, &d);
     }
     if (diag) *diag = d;
     return d.status;
@@ -736,20 +794,24 @@ int oco_solve_block_files(oco_ctx* c, int nz, int nyloc, int nxloc, double dz, d
     const size_t slot_bytes = (size_t)nwc * per_w;
     int st;
     if ((st = sweep_buffers(c, p_max, nz, (long)ncells))) return st;
+    const bool check = (c->checks & OCO_CHECK_FINITE) != 0;
+    if (check && (st = ensure(c, c->fkey, sizeof(unsigned long long)))) return st;
     if ((st = ensure(c, c->rho, ncells * sizeof(double)))) return st;
     if ((st = ensure(c, c->T, ncells * sizeof(double)))) return st;
     if ((st = ensure(c, c->U, 2 * (size_t)nwc * bU))) return st;
     if ((st = ensure(c, c->V, 2 * (size_t)nwc * bV))) return st;
     if ((st = ensure(c, c->A, 2 * (size_t)nwc * bV))) return st;
     /* pinned: two chunk slots [U][absorp][V], then per-chunk pivot keys, the
-     * canary sums and the "no pivot" value */
-    if ((st = ensure_pinned(c, 2 * slot_bytes + ((size_t)nch + 5) * sizeof(double)))) return st;
+     * canary sums, the "no pivot" value and the non-finite key */
+    if ((st = ensure_pinned(c, 2 * slot_bytes + ((size_t)nch + 6) * sizeof(double)))) return st;
     if ((st = ensure_copy_stream(c))) return st;
     char* pin = (char*)c->pinned;
     unsigned long long* keys = (unsigned long long*)(pin + 2 * slot_bytes);
     double* can = (double*)(keys + nch);
     unsigned long long* none = (unsigned long long*)(can + 4);
+    unsigned long long* fkey = none + 1;
     *none = OCO_FAIL_NONE;
+    *fkey = OCO_FAIL_NONE;
 
     oco_record_file* F[3] = {nullptr, nullptr, nullptr};
     auto close_all = [&](int keep) {
@@ -781,6 +843,7 @@ int oco_solve_block_files(oco_ctx* c, int nz, int nyloc, int nxloc, double dz, d
         he = oco_launch_cells((const double*)c->rho.p, (const double*)c->T.p, (long)ncells, c->fully_ionized_T,
                               (double*)c->cells.p, p_full.wave ? nz : 0, c->stream);
     if (he == hipSuccess) he = hipEventRecord(c->ev[1], c->stream);
+    if (he == hipSuccess && check) he = oco_launch_fail_reset((unsigned long long*)c->fkey.p, 1, c->stream);
     if (he != hipSuccess) {
         fprintf(stderr, "ocotillo: block outputs: %s\n", hipGetErrorString(he));
         close_all(0);
@@ -857,6 +920,10 @@ int oco_solve_block_files(oco_ctx* c, int nz, int nyloc, int nxloc, double dz, d
         char* sb = pin + slot * slot_bytes;
         he = hipMemcpyAsync(c->fail.p, none, sizeof(*none), hipMemcpyHostToDevice, c->stream);
         if (he == hipSuccess) he = oco_launch_sweep(a, c->lgrey, pk.wave, pk.lanes, pk.pk, pk.nblocks, c->stream);
+        if (he == hipSuccess && check)
+            he = finite_scan(nz, nyloc, nxloc, iw0, n, nw, a.U, a.V, a.A, (unsigned long long*)c->fkey.p, c->stream);
+        if (he == hipSuccess && check && k == nch - 1)
+            he = hipMemcpyAsync(fkey, c->fkey.p, sizeof(*fkey), hipMemcpyDeviceToHost, c->stream);
         if (he == hipSuccess)
             he = hipMemcpyAsync(keys + k, c->fail.p, sizeof(*keys), hipMemcpyDeviceToHost, c->stream);
         if (he == hipSuccess && k == 0)
@@ -898,6 +965,10 @@ int oco_solve_block_files(oco_ctx* c, int nz, int nyloc, int nxloc, double dz, d
         if (best != OCO_FAIL_NONE) {
             decode_key(best, nw, nyloc, &d);
             d.minU = d.maxU = d.minV = d.maxV = d.minA = d.maxA = 0.0;
+        } else if (*fkey != OCO_FAIL_NONE) {
+            /* the reference's trapping build stops before output_binary */
+            decode_finite(*fkey, nz, nw, nyloc, &d);
+            best = *fkey;
         }
     }
     const int keep = st == OCO_OK && best == OCO_FAIL_NONE;

@@ -135,6 +135,13 @@ size_t oco_cells_chunked_doubles(long ncol, int nz);
 int oco_serve_win_rows(int max_nz);
 int oco_serve_grid(int mixed, int max_nz, int device);
 hipError_t oco_launch_fail_reset(unsigned long long* fail, int n, hipStream_t stream);
+/* Non-finite scan of one output array of a block solve, laid out (inner,
+ * nyloc, nxloc, nw_run) with its first wavelength iw0 of nw: the first NaN
+ * or infinity in the reference's loop order as the key
+ * ((system * 3 + which) * (nz + 6) + e), system = (ix * nyloc + iy) * nw + iw,
+ * atomicMin into *key (OCO_FAIL_NONE when none); which: 0 absorp, 1 U, 2 V. */
+hipError_t oco_launch_finite_scan(const double* p, long n, int inner, int which, int nyloc, int nxloc, int iw0, int nw,
+                                  int nz, unsigned long long* key, hipStream_t stream);
 /* mixed != 0 selects the mixed-precision kernels (non-grey only). */
 /* the FP64 lane-per-system batch kernel, variant 0..2 */
 hipError_t oco_launch_batch(const oco_batch_args& a, int variant, int nblocks, hipStream_t stream);

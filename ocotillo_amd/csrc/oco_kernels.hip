@@ -1638,6 +1638,39 @@ __global__ void fail_reset(unsigned long long* fail, int n)
     if (i < n) fail[i] = OCO_FAIL_NONE;
 }
 
+/* The first non-finite value of an output array (oco_launch_finite_scan):
+ * grid-stride over the array; a lane that finds one forms its key in the
+ * reference's loop order and takes the minimum. */
+__global__ __launch_bounds__(256) void finite_scan(const double* __restrict__ p, long n, int inner, int which,
+                                                   int nyloc, int nxloc, int iw0, int nw, int nz,
+                                                   unsigned long long* key)
+{
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+        const double v = __builtin_nontemporal_load(gptr(p + i));
+        if (__builtin_isfinite(v)) continue;
+        const long e = i % inner;
+        long t = i / inner;
+        const long iy = t % nyloc;
+        t /= nyloc;
+        const long ix = t % nxloc;
+        const long iw = iw0 + t / nxloc;
+        const unsigned long long sys = (unsigned long long)((ix * nyloc + iy) * nw + iw);
+        atomicMin(key, (sys * 3ull + (unsigned long long)which) * (unsigned long long)(nz + 6) + (unsigned long long)e);
+    }
+}
+
+hipError_t oco_launch_finite_scan(const double* p, long n, int inner, int which, int nyloc, int nxloc, int iw0, int nw,
+                                  int nz, unsigned long long* key, hipStream_t stream)
+{
+    (void)hipGetLastError();
+    if (n <= 0) return hipSuccess;
+    long blocks = (n + 255) / 256;
+    if (blocks > 8192) blocks = 8192;
+    hipLaunchKernelGGL(finite_scan, dim3((unsigned)blocks), dim3(256), 0, stream, p, n, inner, which, nyloc, nxloc,
+                       iw0, nw, nz, key);
+    return hipGetLastError();
+}
+
 hipError_t oco_launch_fail_reset(unsigned long long* fail, int n, hipStream_t stream)
 {
     (void)hipGetLastError();   /* a pending not-ready status from an event query is not this launch's */

@@ -8,8 +8,9 @@ module oco_capi
 
   integer(c_int), parameter, public :: OCO_OK = 0, OCO_E_PIVOT0_STAGE1 = 1, OCO_E_PIVOT0_STAGE2 = 2, &
        OCO_E_ARG = 3, OCO_E_OOM = 4, OCO_E_HIP = 5, OCO_E_STATE = 6, OCO_E_NODEVICE = 7, OCO_E_BUSY = 8, &
-       OCO_E_IO = 9
+       OCO_E_IO = 9, OCO_E_NONFINITE = 10
   integer(c_int), parameter, public :: OCO_MODE_FP64 = 0, OCO_MODE_MIXED = 1
+  integer(c_int), parameter, public :: OCO_CHECK_FINITE = 1
 
   type, bind(C), public :: oco_diag
     integer(c_int) :: status, fail_ix, fail_iy, fail_iw, fail_j
@@ -24,7 +25,8 @@ module oco_capi
     integer(c_int) :: avoid_momentum_underflow
   end type oco_athena_units
 
-  public :: oco_create, oco_destroy, oco_set_wavelengths, oco_set_physics, oco_solve_block, oco_message
+  public :: oco_create, oco_destroy, oco_set_wavelengths, oco_set_physics, oco_set_checks, oco_solve_block, &
+            oco_message
   public :: oco_athena_slab, oco_solve_block_files, oco_write_record
 
   interface
@@ -54,6 +56,12 @@ module oco_capi
       real(c_double), value :: fully_ionized_T, sigma_grey
       integer(c_int), value :: lgrey
     end function oco_set_physics
+
+    integer(c_int) function oco_set_checks(ctx, flags) bind(C, name='oco_set_checks')
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: flags
+    end function oco_set_checks
 
     integer(c_int) function oco_solve_block(ctx, nz, nyloc, nxloc, dz, dz1, dz2, rho3d, t3d, u, v, absorp, &
          mode, diag) bind(C, name='oco_solve_block')

@@ -12,7 +12,8 @@
 !     source/common.f90:9), so one executable serves every shape;
 !   * optional namelist &oco_input: lwave_log (log-spaced wavelengths,
 !     default .false. = the reference's linear grid), oco_mode (0 FP64,
-!     1 mixed, 2 accurate), oco_device, hbm_budget_gb;
+!     1 mixed, 2 accurate), oco_device, hbm_budget_gb, check_finite (stop at
+!     a NaN or infinity in the outputs, default .false.);
 !   * several snapshot tags or ranges a:b on the command line run each
 !     snapshot as its own run would, into output_snapshot<tag>/ (what
 !     samples/multiInTest/snapshot_looper.sh meant to do), in one process.
@@ -545,7 +546,7 @@ program flux_feautrier_gpu
   real(8) :: fully_ionized_T, hbm_budget_gb = 0
   real :: start, finish, start_loop = 0, finish_loop = 0
   integer(8) :: c0, c1, crate
-  logical :: lgrey = .false., lread_athena = .true., lwave_log = .false.
+  logical :: lgrey = .false., lread_athena = .true., lwave_log = .false., check_finite = .false.
   integer :: oco_mode = 0, oco_device = 0, iprocx, iprocy, ix, iy, i, err, iu
   integer(c_int) :: st
   character(len=90) :: snapshot, inputfile = './input.in', arg
@@ -557,7 +558,7 @@ program flux_feautrier_gpu
   real(8) :: loop_wall = 0, dev_ms = 0
   namelist /input/ z0, z1, w0, w1, sigma_grey, lgrey, lread_athena
   namelist /gas_state_input/ fully_ionized_T
-  namelist /oco_input/ lwave_log, oco_mode, oco_device, hbm_budget_gb
+  namelist /oco_input/ lwave_log, oco_mode, oco_device, hbm_budget_gb, check_finite
 
   call cpu_time(start)
   call get_command_argument(1, snapshot)
@@ -625,6 +626,7 @@ program flux_feautrier_gpu
   endif
   st = oco_set_wavelengths(ctx, int(nw, c_int), waves)
   if (st == OCO_OK) st = oco_set_physics(ctx, fully_ionized_T, merge(1_c_int, 0_c_int, lgrey), sigma_grey)
+  if (st == OCO_OK .and. check_finite) st = oco_set_checks(ctx, OCO_CHECK_FINITE)
   if (st /= OCO_OK) then
     print*, "ocotillo GPU library: ", oco_message(st)
     stop 2
@@ -679,6 +681,11 @@ program flux_feautrier_gpu
         print*, "ERROR tridiag stage 2: bet = b(j)-a(j)*gam(j) =", 0d0
         print*, "(column ix,iy =", diag%fail_ix, diag%fail_iy, " wavelength", diag%fail_iw, " row", diag%fail_j, ")"
         stop
+      else if (st == OCO_E_NONFINITE) then
+        ! the device counterpart of the reference's -ffpe-trap debug build
+        print*, "ERROR: NaN or infinity in the outputs (check_finite)"
+        print*, "(column ix,iy =", diag%fail_ix, diag%fail_iy, " wavelength", diag%fail_iw, " cell", diag%fail_j, ")"
+        stop 2
       else if (st /= OCO_OK) then
         print*, "ocotillo GPU library: ", oco_message(st)
         stop 2

@@ -81,6 +81,7 @@ static void cpu_checks(void)
     EXPECT(oco_destroy(NULL) == OCO_OK || oco_destroy(NULL) == OCO_E_ARG, "destroy(NULL)");
     EXPECT(oco_set_wavelengths(NULL, 1, &x) == OCO_E_ARG, "set_wavelengths(NULL)");
     EXPECT(oco_set_physics(NULL, 2e4, 0, 0.4) == OCO_E_ARG, "set_physics(NULL)");
+    EXPECT(oco_set_checks(NULL, OCO_CHECK_FINITE) == OCO_E_ARG, "set_checks(NULL)");
     EXPECT(oco_solve_block(NULL, 4, 1, 1, 1.0, 1.0, 1.0, &x, &x, &x, &x, &x, OCO_MODE_FP64, &d) == OCO_E_ARG,
            "solve_block(NULL)");
     EXPECT(oco_solve_block_device(NULL, 4, 1, 1, 1.0, 1.0, 1.0, &x, &x, &x, &x, &x, OCO_MODE_FP64, &d) ==
@@ -147,6 +148,21 @@ static void gpu_checks(void)
         EXPECT(all_finite(U, sizeof(U) / 8) && all_finite(V, sizeof(V) / 8) && all_finite(A, sizeof(A) / 8),
                "finite outputs");
     }
+
+    STEP("finite check");
+    EXPECT(oco_set_checks(c, 2) == OCO_E_ARG, "unknown check flag");
+    EXPECT(oco_set_checks(c, OCO_CHECK_FINITE) == OCO_OK, "set_checks");
+    EXPECT(oco_solve_block(c, NZ, NY, NX, dz, dz1, dz2, rho, T, U, V, A, OCO_MODE_FP64, &d) == OCO_OK,
+           "clean block under the check");
+    {
+        const double keep = rho[NZ + 3];
+        rho[NZ + 3] = NAN;   /* column (ix, iy) = (1, 2), cell 4 */
+        st = oco_solve_block(c, NZ, NY, NX, dz, dz1, dz2, rho, T, U, V, A, OCO_MODE_MIXED, &d);
+        EXPECT(st == OCO_E_NONFINITE && d.fail_ix == 1 && d.fail_iy == 2 && d.fail_iw == 1 && d.fail_j == 4,
+               "NaN located");
+        rho[NZ + 3] = keep;
+    }
+    EXPECT(oco_set_checks(c, 0) == OCO_OK, "checks off");
 
     STEP("device buffers and the allocation-free solve");
     double *d_rho = NULL, *d_T = NULL, *d_U = NULL, *d_V = NULL, *d_A = NULL;

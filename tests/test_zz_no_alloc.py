@@ -8,52 +8,9 @@ with OCO_E_ARG before anything is launched."""
 import numpy as np
 import pytest
 
+from conftest import DeviceArray
+
 pytestmark = pytest.mark.gpu
-
-
-class DeviceArray:
-    """A device buffer from the HIP runtime the library itself runs on
-    (libamdhip64.so.7 of /opt/rocm, already loaded by the library), so the
-    test needs no other GPU runtime: hipMalloc / hipMemcpy / hipFree by
-    ctypes.  The test allocates its inputs and outputs; the entry point under
-    test allocates nothing."""
-    _hip = None
-
-    def __init__(self, nbytes):
-        import ctypes as C
-        import ocotillo_amd as oco
-        oco.load_library()
-        if DeviceArray._hip is None:
-            h = C.CDLL("libamdhip64.so.7")
-            h.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
-            h.hipFree.argtypes = [C.c_void_p]
-            h.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
-            h.hipDeviceSynchronize.argtypes = []
-            DeviceArray._hip = h
-        self.n = int(nbytes)
-        p = C.c_void_p()
-        assert DeviceArray._hip.hipMalloc(C.byref(p), self.n) == 0
-        self.ptr = p.value
-
-    def upload(self, a):
-        import ctypes as C
-        a = np.ascontiguousarray(a)
-        assert a.nbytes == self.n
-        assert DeviceArray._hip.hipMemcpy(C.c_void_p(self.ptr), a.ctypes.data_as(C.c_void_p), self.n, 1) == 0
-        return self
-
-    def download(self, shape):
-        import ctypes as C
-        out = np.empty(shape, dtype=np.float64)
-        assert out.nbytes == self.n
-        assert DeviceArray._hip.hipMemcpy(out.ctypes.data_as(C.c_void_p), C.c_void_p(self.ptr), self.n, 2) == 0
-        return out
-
-    def free(self):
-        import ctypes as C
-        if self.ptr:
-            DeviceArray._hip.hipFree(C.c_void_p(self.ptr))
-            self.ptr = None
 
 
 def blocks():
