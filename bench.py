@@ -58,7 +58,7 @@ WM_IONISED = 143 + 7 * (WM_DIV - 1) + WM_OMEXP + WM_EXP10 + WM_EXPM1 + 3 * WM_LO
 # copy-kernel calibration (x2.0 / x1.0); workloads not listed report None
 SWEEP_TRAFFIC_BYTES = {
     (256, 256, 512, 128, "sweep_mixed_wave"): 1.13004e11,
-    (256, 256, 512, 128, "sweep_fp64"): 3.04432e11,
+    (256, 256, 512, 128, "sweep_fp64"): 2.99987e11,
 }
 
 
