@@ -31,7 +31,7 @@ CHECK_FINITE = 1
 EXPORTS = ["oco_create", "oco_destroy", "oco_set_wavelengths", "oco_set_physics", "oco_set_checks", "oco_solve_block",
            "oco_solve_block_device", "oco_solve_block_device_noalloc", "oco_strerror", "oco_version", "oco_athena_slab", "oco_athena_slab_device",
            "oco_solve_block_files", "oco_write_record", "oco_opacity_points",
-           "oco_batch_create", "oco_batch_destroy", "oco_batch_layout", "oco_batch_buffers", "oco_batch_run",
+           "oco_batch_create", "oco_batch_destroy", "oco_batch_layout", "oco_batch_buffers", "oco_batch_column", "oco_batch_run",
            "oco_batch_run_host", "oco_batch_tuning", "oco_batch_tune_times",
            "oco_server_create", "oco_server_submit", "oco_server_wait", "oco_server_poll", "oco_server_stats",
            "oco_server_destroy"]
@@ -115,6 +115,7 @@ def load_library(path=None):
     L.oco_batch_destroy.argtypes = [C.c_void_p]
     L.oco_batch_layout.argtypes = [C.c_void_p, _SZ, _SZ, _SZ, _SZ]
     L.oco_batch_buffers.argtypes = [C.c_void_p, _PP, _PP, _PP, _PP, _PP]
+    L.oco_batch_column.argtypes = [C.c_void_p, C.c_int, _SZ, C.POINTER(C.c_int), _SZ, C.POINTER(C.c_int)]
     L.oco_batch_run.argtypes = [C.c_void_p, C.c_int, C.POINTER(Diag)]
     L.oco_batch_run_host.argtypes = [C.c_void_p, _DP, _DP, _DP, _DP, _DP, C.c_int, C.POINTER(Diag)]
     L.oco_batch_tuning.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int),
@@ -347,10 +348,21 @@ class Batch:
         self.close()
 
     def buffers(self):
-        """Device pointers (rho, T, U, V, absorp) of the plan's resident arrays."""
+        """Device pointers (rho, T, U, V, absorp) of the plan's resident arrays
+        (outputs with padded rows: `column`)."""
         p = [C.c_void_p() for _ in range(5)]
         self.L.oco_batch_buffers(self.h, *[C.byref(x) for x in p])
         return [x.value for x in p]
+
+    def column(self, col):
+        """(u_off, ldu, va_off, ldva): where column `col` sits in the device
+        outputs, offsets in doubles and leading dimensions (rows padded to 16
+        doubles)."""
+        uo, vo, lu, lv = C.c_size_t(), C.c_size_t(), C.c_int(), C.c_int()
+        st = self.L.oco_batch_column(self.h, int(col), C.byref(uo), C.byref(lu), C.byref(vo), C.byref(lv))
+        if st != OK:
+            raise OcoError(st, "oco_batch_column: " + strerror(st))
+        return uo.value, lu.value, vo.value, lv.value
 
     def tuning(self, mode=MODE_FP64):
         """(kernel variant, workgroups, tuned?) of the autotuner for `mode`."""

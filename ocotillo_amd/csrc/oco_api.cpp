@@ -1067,9 +1067,10 @@ int oco_athena_slab(oco_ctx* c, int nz, int nyloc, int nxloc, int nzl, int iz0, 
 struct oco_batch {
     oco_ctx* ctx = nullptr;
     int ncol = 0, max_nz = 0, ntiles = 0;
-    long ncells = 0, nsys = 0, nU = 0, nVA = 0;
+    long ncells = 0, nsys = 0, nU = 0, nVA = 0;   /* nU, nVA: the packed (host) layout's totals */
+    long dU = 0, dVA = 0;                         /* the device buffers' totals (padded rows) */
     std::vector<int> nz, nw;
-    std::vector<long> woff;
+    std::vector<long> woff, uoff, voff;           /* device offsets of each column's U and V, absorp */
     DevBuf ints, longs, dz, wtab, rho, T, cells, U, V, A, scratch, counter, fail;
     /* mixed mode, wavefront per system (when every column fits it): item
      * lists for workgroups of 1, 2 and 4 wavefronts, their queue tickets and
@@ -1205,7 +1206,7 @@ int oco_batch_create(oco_ctx* c, int ncol, const int* nz, const int* nw, const d
     b->nw.assign(nw, nw + ncol);
     std::vector<long> coff(ncol), woff(ncol), uoff(ncol), voff(ncol);
     std::vector<double> dz3(3 * (size_t)ncol);
-    long nc = 0, ns = 0, nU = 0, nVA = 0;
+    long nc = 0, ns = 0, nU = 0, nVA = 0, pU = 0, pVA = 0;
     for (int i = 0; i < ncol; i++) {
         if (nz[i] < 4 || nw[i] < 1 || !(dz[i] > 0.0)) {
             delete b;
@@ -1217,8 +1218,10 @@ int oco_batch_create(oco_ctx* c, int ncol, const int* nz, const int* nw, const d
         voff[i] = nVA;
         nc += nz[i];
         ns += nw[i];
-        nU += (long)(nz[i] + 6) * nw[i];
-        nVA += (long)nz[i] * nw[i];
+        nU += (long)oco_batch_ldu(nz[i]) * nw[i];
+        nVA += (long)oco_batch_ldv(nz[i]) * nw[i];
+        pU += (long)(nz[i] + 6) * nw[i];
+        pVA += (long)nz[i] * nw[i];
         b->max_nz = std::max(b->max_nz, nz[i]);
         dz3[3 * i] = dz[i];
         dz3[3 * i + 1] = 1.0 / dz[i];
@@ -1231,9 +1234,13 @@ int oco_batch_create(oco_ctx* c, int ncol, const int* nz, const int* nw, const d
         }
     b->ncells = nc;
     b->nsys = ns;
-    b->nU = nU;
-    b->nVA = nVA;
+    b->nU = pU;
+    b->nVA = pVA;
+    b->dU = nU;
+    b->dVA = nVA;
     b->woff = woff;
+    b->uoff = uoff;
+    b->voff = voff;
     /* tiles of 64 wavelengths, longest columns first (LPT order) */
     std::vector<int> order(ncol);
     std::iota(order.begin(), order.end(), 0);
@@ -1362,6 +1369,16 @@ int oco_batch_layout(oco_batch* b, size_t* ncells, size_t* nsys, size_t* nU, siz
     if (nsys) *nsys = (size_t)b->nsys;
     if (nU) *nU = (size_t)b->nU;
     if (nVA) *nVA = (size_t)b->nVA;
+    return OCO_OK;
+}
+
+int oco_batch_column(oco_batch* b, int col, size_t* u_off, int* ldu, size_t* va_off, int* ldva)
+{
+    if (!b || col < 0 || col >= b->ncol) return OCO_E_ARG;
+    if (u_off) *u_off = (size_t)b->uoff[col];
+    if (ldu) *ldu = oco_batch_ldu(b->nz[col]);
+    if (va_off) *va_off = (size_t)b->voff[col];
+    if (ldva) *ldva = oco_batch_ldv(b->nz[col]);
     return OCO_OK;
 }
 
@@ -1531,9 +1548,23 @@ int oco_batch_run_host(oco_batch* b, const double* rho, const double* T, double*
     HIPCHK(hipMemcpyAsync(b->T.p, T, b->ncells * sizeof(double), hipMemcpyHostToDevice, c->stream));
     int st = oco_batch_run(b, mode, diag);
     if (st == OCO_E_ARG || st == OCO_E_HIP || st == OCO_E_OOM) return st;
-    HIPCHK(hipMemcpyAsync(U, b->U.p, b->nU * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(hipMemcpyAsync(V, b->V.p, b->nVA * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(hipMemcpyAsync(absorp, b->A.p, b->nVA * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    /* the device rows are padded (oco_batch_ldu, oco_batch_ldv); the host
+     * arrays get the packed layout, one pitched copy per column and array */
+    long pu = 0, pv = 0;
+    for (int i = 0; i < b->ncol; i++) {
+        const size_t nz = (size_t)b->nz[i], nw = (size_t)b->nw[i];
+        HIPCHK(hipMemcpy2DAsync(U + pu, (nz + 6) * sizeof(double), (const double*)b->U.p + b->uoff[i],
+                                (size_t)oco_batch_ldu((int)nz) * sizeof(double), (nz + 6) * sizeof(double), nw,
+                                hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipMemcpy2DAsync(V + pv, nz * sizeof(double), (const double*)b->V.p + b->voff[i],
+                                (size_t)oco_batch_ldv((int)nz) * sizeof(double), nz * sizeof(double), nw,
+                                hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipMemcpy2DAsync(absorp + pv, nz * sizeof(double), (const double*)b->A.p + b->voff[i],
+                                (size_t)oco_batch_ldv((int)nz) * sizeof(double), nz * sizeof(double), nw,
+                                hipMemcpyDeviceToHost, c->stream));
+        pu += (long)(nz + 6) * nw;
+        pv += (long)nz * nw;
+    }
     HIPCHK(hipStreamSynchronize(c->stream));
     return st;
 }

@@ -34,6 +34,13 @@ struct oco_sweep_args {
 #define OCO_TICKET_STRIDE 32
 #define OCO_TICKET_BYTES (OCO_TICKET_QUEUES * OCO_TICKET_STRIDE * 8)
 
+/* Leading dimensions of a batched column's device outputs: U rows of nz + 6
+ * doubles and V, absorp rows of nz, each rounded up to 16 doubles, so every
+ * system's row starts on a 128-byte line and the lane sweep's output runs
+ * are whole lines */
+__host__ __device__ static inline int oco_batch_ldu(int nz) { return (nz + 6 + 15) & ~15; }
+__host__ __device__ static inline int oco_batch_ldv(int nz) { return (nz + 15) & ~15; }
+
 /* Batched uneven columns (per-column shapes and wavelength grids). */
 struct oco_batch_args {
     int ntiles;

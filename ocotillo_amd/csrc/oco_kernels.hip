@@ -1743,9 +1743,9 @@ __device__ static inline void batch_tiles(const oco_batch_args& a, const oco_mta
             d.cells = a.cells + coff[col] * OCO_CELL_STRIDE;
             d.rho = a.rho + coff[col];
             d.T = a.T + coff[col];
-            d.U = a.U + uoff[col] + (long)iw * (nz + 6);
-            d.V = a.V + voff[col] + (long)iw * nz;
-            d.A = a.A + voff[col] + (long)iw * nz;
+            d.U = a.U + uoff[col] + (long)iw * oco_batch_ldu(nz);
+            d.V = a.V + voff[col] + (long)iw * oco_batch_ldv(nz);
+            d.A = a.A + voff[col] + (long)iw * oco_batch_ldv(nz);
             d.key = woff[col] + iw;
             double w[OCO_WT_STRIDE];
             const double* wr = a.wtab + (woff[col] + iw) * OCO_WT_STRIDE;
@@ -2157,7 +2157,8 @@ __global__ __launch_bounds__(OCO_WAVE * K) void batch_mixed_wave(const oco_batch
             const int nk = nw - iw0 < S ? nw - iw0 : S;
             const long c0 = coff[col], v0 = voff[col];
             wave_points<K, OCO_PK_MIXED>(a.cells_soa + c0 * CR_NFIELD, nullptr, nz, a.wtab + woff[col] * OCO_WT_STRIDE,
-                                         iw0, nk, a.A + v0 + (long)iw0 * nz, nz, lds, lstr, 0.0, nullptr);
+                                         iw0, nk, a.A + v0 + (long)iw0 * oco_batch_ldv(nz), oco_batch_ldv(nz), lds, lstr,
+                                         0.0, nullptr);
             __syncthreads();
             const bool valid = g < nk;
             const int iw = iw0 + (valid ? g : 0);
@@ -2165,9 +2166,9 @@ __global__ __launch_bounds__(OCO_WAVE * K) void batch_mixed_wave(const oco_batch
             d.nz = nz;
             d.dz = dzs[3 * col]; d.dz1 = dzs[3 * col + 1]; d.dz2 = dzs[3 * col + 2]; d.sigma_grey = 0.0;
             d.cells = nullptr; d.rho = nullptr; d.T = nullptr;
-            d.U = a.U + uoff[col] + (long)iw * (nz + 6);
-            d.V = a.V + v0 + (long)iw * nz;
-            d.A = a.A + v0 + (long)iw * nz;
+            d.U = a.U + uoff[col] + (long)iw * oco_batch_ldu(nz);
+            d.V = a.V + v0 + (long)iw * oco_batch_ldv(nz);
+            d.A = a.A + v0 + (long)iw * oco_batch_ldv(nz);
             d.key = woff[col] + iw;
             solve_system_wave<MM, L>(d, lds + g * lstr, valid);
         }

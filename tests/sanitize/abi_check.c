@@ -251,6 +251,12 @@ static void gpu_checks(void)
                     EXPECT(oco_batch_run_host(b, br, bT, bU, bV, bA, m ? OCO_MODE_MIXED : OCO_MODE_FP64, &d) == OCO_OK,
                            "batch_run_host");
             EXPECT(all_finite(bU, nU) && all_finite(bV, nVA), "batch finite");
+            size_t uo = 0, vo = 0;
+            int lu = 0, lv = 0;
+            EXPECT(oco_batch_column(b, 2, &uo, &lu, &vo, &lv) == OCO_OK && lu == 96 && lv == 96 &&
+                       uo == (size_t)(48 * 5 + 32 * 20) && vo == (size_t)(48 * 5 + 32 * 20),
+                   "batch_column offsets and padded rows");
+            EXPECT(oco_batch_column(b, 3, &uo, &lu, &vo, &lv) == OCO_E_ARG, "batch_column past the end");
             float ms[32];
             EXPECT(oco_batch_tune_times(b, OCO_MODE_MIXED, ms, 32) > 0, "tune times");
             free(bU);

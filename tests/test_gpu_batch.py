@@ -127,3 +127,42 @@ def test_batch_zero_pivot_first_column():
         for c in (0, 2):
             assert np.array_equal(Us[c], ref[c][0]) and np.array_equal(Vs[c], ref[c][1])
         b.close()
+
+
+@pytest.mark.parametrize("mode_name", ["fp64", "mixed"])
+def test_device_rows_padded(mode_name):
+    """The device outputs hold each system's row padded to a multiple of 16
+    doubles (one 128-byte line) at the column's offset (oco_batch_column):
+    the same values as run_host's packed arrays, columns back to back."""
+    import ctypes as C
+    import ocotillo_amd as oco
+    mode = oco.MODE_FP64 if mode_name == "fp64" else oco.MODE_MIXED
+    nz, nw, waves, dz, rho, T = small_batch()
+    with oco.Context(0) as ctx:
+        ctx.set_physics(2e4, False, 0.4)
+        b = oco.Batch(ctx, nz, nw, waves, dz)
+        U, V, A, d = b.run_host(rho, T, mode=mode)
+        assert d.status == 0
+        Us, Vs, As = b.split(U, V, A)
+        hip = C.CDLL("libamdhip64.so.7")   # the runtime the library runs on
+        hip.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+        _, _, dU, dV, dA = b.buffers()
+
+        def fetch(base, off, n):
+            out = np.empty(n, dtype=np.float64)
+            assert hip.hipMemcpy(out.ctypes.data_as(C.c_void_p), C.c_void_p(base + 8 * off), 8 * n, 2) == 0
+            return out
+
+        next_u = next_v = 0
+        for c in range(nz.size):
+            uo, lu, vo, lv = b.column(c)
+            n, m = int(nz[c]), int(nw[c])
+            assert (uo, vo) == (next_u, next_v)
+            assert lu % 16 == 0 and n + 6 <= lu < n + 22 and lv % 16 == 0 and n <= lv < n + 16
+            for base, off, ld, rows, ref in ((dU, uo, lu, n + 6, Us[c]), (dV, vo, lv, n, Vs[c]), (dA, vo, lv, n, As[c])):
+                got = fetch(base, off, ld * m).reshape(m, ld)[:, :rows]
+                assert np.array_equal(got, ref), (mode_name, c)   # split gives (nw_c, rows)
+            next_u, next_v = uo + lu * m, vo + lv * m
+        with pytest.raises(oco.OcoError):
+            b.column(nz.size)
+        b.close()
