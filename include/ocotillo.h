@@ -251,7 +251,7 @@ int oco_batch_run_host(oco_batch* plan, const double* rho, const double* T, doub
 int oco_batch_tuning(oco_batch* plan, int mode, int* variant, int* nblocks, int* tuned);
 /* The autotuner's timings for a mode, in the order it tried the candidates
  * (median of three launches each, ms): FP64, the lane kernel's 3 variants x
- * 3 grids (variant-major; 2, 3, 4 one-wavefront workgroups per SIMD); mixed, per column
+ * 3 grids (variant-major; 2, 3, 4 wavefronts per SIMD); mixed, per column
  * class that holds columns, 1, 2, 4 wavefronts per workgroup (fewer where a
  * size does not fit).  Writes at most cap values and returns how many there
  * are (0 before the first run in the mode), or a negative OCO_E_ARG. */

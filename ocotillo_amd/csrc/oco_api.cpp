@@ -1321,7 +1321,10 @@ int oco_batch_create(oco_ctx* c, int ncol, const int* nz, const int* nw, const d
     if (!st) st = ensure(c, b->U, nU * sizeof(double));
     if (!st) st = ensure(c, b->V, nVA * sizeof(double));
     if (!st) st = ensure(c, b->A, nVA * sizeof(double));
-    if (!st) st = ensure(c, b->scratch, (size_t)std::min(b->ntiles, max_blocks) * b->max_nz * 3 * 64 * sizeof(double));
+    /* a scratch slot per wavefront: the FP64 kernel's grid rounds the tuned
+     * count up to whole workgroups of OCO_BATCH_WG_MAX at most */
+    if (!st) st = ensure(c, b->scratch, (size_t)(std::min(b->ntiles, max_blocks) + OCO_BATCH_WG_MAX) * b->max_nz * 3 * 64 *
+                                            sizeof(double));
     if (!st) st = ensure(c, b->counter, sizeof(unsigned int));
     if (!st) st = ensure(c, b->fail, sizeof(unsigned long long));
     if (!st && b->wave_ok) {

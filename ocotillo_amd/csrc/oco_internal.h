@@ -40,6 +40,9 @@ struct oco_sweep_args {
  * are whole lines */
 __host__ __device__ static inline int oco_batch_ldu(int nz) { return (nz + 6 + 15) & ~15; }
 __host__ __device__ static inline int oco_batch_ldv(int nz) { return (nz + 15) & ~15; }
+/* wavefronts per workgroup of the FP64 batch kernel, at most (its grid is
+ * the tuned wavefront count rounded up to whole workgroups) */
+#define OCO_BATCH_WG_MAX 4
 
 /* Batched uneven columns (per-column shapes and wavelength grids). */
 struct oco_batch_args {
@@ -150,7 +153,8 @@ hipError_t oco_launch_fail_reset(unsigned long long* fail, int n, hipStream_t st
 hipError_t oco_launch_finite_scan(const double* p, long n, int inner, int which, int nyloc, int nxloc, int iw0, int nw,
                                   int nz, unsigned long long* key, hipStream_t stream);
 /* mixed != 0 selects the mixed-precision kernels (non-grey only). */
-/* the FP64 lane-per-system batch kernel, variant 0..2 */
+/* the FP64 lane-per-system batch kernel, variant 0..2, on nblocks
+ * wavefronts (rounded up to whole workgroups of OCO_BATCH_WG_MAX at most) */
 hipError_t oco_launch_batch(const oco_batch_args& a, int variant, int nblocks, hipStream_t stream);
 /* batched mixed mode, difference form: the per-column field-major records
  * (and the failure key), then the sweep of one class of columns (its items,

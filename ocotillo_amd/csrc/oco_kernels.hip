@@ -172,16 +172,23 @@ struct oco_stage_t {
 #define OCO_BLOCK_A1 OCO_A1_STAGED
 #endif
 typedef oco_stage_t<OCO_STAGE_ROWS, true, OCO_BLOCK_A1> oco_stage;
-/* The batched kernels keep ordinary stores (their columns' ragged nz and nw
- * leave many partial lines, which the L2 combines) and the FP64 one 8 rows.
- * A batched column's alpha is one contiguous (nz, nw) array, so a lane's
- * reads of its own alpha stay within a few pages of its neighbours': the
- * FP64 batch kernel reads it directly (measured on MI355X, 4096 uneven
- * columns: 9.98 ms with 1/alpha in scratch, 9.39 ms). */
+/* The FP64 batch kernel stages 16-row runs like the block sweep and stores
+ * them nontemporally: a batch's device rows are padded to whole lines
+ * (oco_batch_ldu, oco_batch_ldv), so every run is one line (measured on
+ * MI355X, 4096 uneven columns: 8-row runs with ordinary stores 7.22-7.28 ms,
+ * 16-row 6.93-6.96, 16-row nontemporal 6.67-6.78).  A batched column's
+ * alpha is one contiguous (ld, nw) array, so a lane's reads of its own alpha
+ * stay within a few pages of its neighbours': the kernel reads it directly
+ * (9.98 ms with 1/alpha in scratch, 9.39 ms, before the padding). */
 #ifndef OCO_BATCH_A1
 #define OCO_BATCH_A1 OCO_A1_DIRECT
 #endif
-typedef oco_stage_t<8, false, OCO_BATCH_A1> oco_stage_batch;
+typedef oco_stage_t<16, true, OCO_BATCH_A1> oco_stage_batch;
+/* wavefronts per workgroup of the FP64 batch kernel: each takes its own
+ * tiles, and they share one LDS copy of the math tables, so two workgroups'
+ * stages fit a CU (8 wavefronts, 2 per SIMD) */
+#define OCO_BATCH_WG 4
+static_assert(OCO_BATCH_WG <= OCO_BATCH_WG_MAX, "scratch slots per launch");
 
 /* Output policy of the serving kernels: only the emergent flux V(1), V(nz)
  * of each system is stored (d.V[0], d.V[1]); U and alpha are not written. */
@@ -1709,10 +1716,12 @@ typedef const __attribute__((address_space(4))) long* oco_cptr_l;
 typedef const __attribute__((address_space(4))) double* oco_cptr_d;
 
 template <int BC, class STAGE>
-__device__ static inline void batch_tiles(const oco_batch_args& a, const oco_mtab* M, int* tile_s, STAGE* S)
+__device__ static inline void batch_tiles(const oco_batch_args& a, const oco_mtab* M, STAGE* S)
 {
     const int lane = lane_id();
-    double* scr = a.scratch + (long)blockIdx.x * a.scr_stride + lane;
+    /* one scratch slot per wavefront of the grid */
+    const long slot = (long)blockIdx.x * (blockDim.x / OCO_WAVE) + threadIdx.x / OCO_WAVE;
+    double* scr = a.scratch + slot * a.scr_stride + lane;
     oco_cptr_i tiles = (oco_cptr_i)a.tiles;
     oco_cptr_i nzs = (oco_cptr_i)a.nz;
     oco_cptr_i nws = (oco_cptr_i)a.nw;
@@ -1722,10 +1731,9 @@ __device__ static inline void batch_tiles(const oco_batch_args& a, const oco_mta
     oco_cptr_l voff = (oco_cptr_l)a.v_off;
     oco_cptr_d dzs = (oco_cptr_d)a.dz;
     for (;;) {
-        if (lane == 0) *tile_s = (int)atomicAdd(a.counter, 1u);
-        __syncthreads();
-        int t = __builtin_amdgcn_readfirstlane(*tile_s);
-        __syncthreads();
+        unsigned tv = 0u;   /* this wavefront's next tile */
+        if (lane == 0) tv = atomicAdd(a.counter, 1u);
+        const int t = (int)__builtin_amdgcn_readfirstlane(tv);
         if (t >= a.ntiles) break;
         const int col = tiles[2 * t];
         const int nz = nzs[col];
@@ -1757,13 +1765,12 @@ __device__ static inline void batch_tiles(const oco_batch_args& a, const oco_mta
 }
 
 template <int BC>
-__global__ __launch_bounds__(OCO_WAVE) void batch_fp64(const oco_batch_args a)
+__global__ __launch_bounds__(OCO_WAVE * OCO_BATCH_WG) void batch_fp64(const oco_batch_args a)
 {
     __shared__ oco_mtab smt;
-    __shared__ int tile_s;
-    __shared__ oco_stage_batch stg;
+    __shared__ oco_stage_batch stg[OCO_BATCH_WG];
     load_mtab(&smt);
-    batch_tiles<BC>(a, &smt, &tile_s, &stg);
+    batch_tiles<BC>(a, &smt, &stg[threadIdx.x / OCO_WAVE]);
 }
 
 
@@ -2084,7 +2091,7 @@ hipError_t oco_launch_batch(const oco_batch_args& a, int variant, int nblocks, h
 {
     (void)hipGetLastError();   /* a pending not-ready status from an event query is not this launch's */
     hipLaunchKernelGGL(batch_reset, dim3(1), dim3(64), 0, stream, a.counter, a.fail);
-    const dim3 g(nblocks), b(OCO_WAVE);
+    const dim3 g((nblocks + OCO_BATCH_WG - 1) / OCO_BATCH_WG), b(OCO_WAVE * OCO_BATCH_WG);
     switch (variant) {
     case 0: hipLaunchKernelGGL((batch_fp64<1>), g, b, 0, stream, a); break;
     case 1: hipLaunchKernelGGL((batch_fp64<2>), g, b, 0, stream, a); break;
