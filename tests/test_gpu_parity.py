@@ -156,6 +156,28 @@ def test_c4_slice_fp64(ctx, where):
         assert frac >= (0.995 if name == "U" else 0.999)
 
 
+@pytest.mark.parametrize("nz", [37, 101])
+def test_u_runs_shifted_to_lines(ctx, nz):
+    """The drop-in stages U shifted by the offset of its rows in a 128-byte
+    line when a wavefront's 64 systems share that offset: with 4 x 4 columns
+    and nw = 64, a column's rows are 16 (nz + 6) doubles apart, so every
+    column's wavefront shifts, and with nz + 6 odd the 16 columns take all 16
+    shifts (including 14 and 15, whose first block ends in the bottom
+    ghosts).  Bit-exact against the oracle."""
+    from ocotillo_amd.workloads import log_waves
+    from oracle.oracle import Oracle
+    print("case u shifts nz", nz, flush=True)
+    rho, T, dz = c2_block(4, 4, nz=nz, seed=5)
+    waves = log_waves(64)
+    ctx.set_wavelengths(waves)
+    ctx.set_physics(2e4, False, 0.4)
+    U, V, A, d = ctx.solve_block(rho, T, dz)
+    assert d.status == 0
+    pU, pV, pA, pd = Oracle(waves, portable=True).solve_block(rho, T, dz)
+    assert pd.status == 0
+    assert np.array_equal(A, pA) and np.array_equal(V, pV) and np.array_equal(U, pU)
+
+
 @pytest.mark.parametrize("nz", [2, 3])
 def test_tiny_columns(ctx, nz):
     """Columns of 2 and 3 cells (the FP64 drop-in's one-thread-per-system
