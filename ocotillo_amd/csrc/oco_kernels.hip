@@ -435,15 +435,27 @@ constexpr int alpha_reload() { return STAGE::emergent ? OCO_A1_SCRATCH : STAGE::
  * pattern: R consecutive threads read one lane's 8R-byte run), then into
  * the V stage's slots; a lane without an alpha output gets 1. */
 template <int R>
-__device__ static inline void alpha_fetch(double* regs, double* const* base, int b, int len)
+__device__ static inline void alpha_fetch(double* regs, double* const* base, int b, int len, int sh = 0)
 {
     constexpr int L = OCO_WAVE / R;
-    const int r = lane_id() & (R - 1), s0 = lane_id() / R, idx = b * R + r;
+    const int r = lane_id() & (R - 1), s0 = lane_id() / R, idx = b * R + r - sh;
 #pragma unroll
     for (int k = 0; k < R; k++) {
         const double* p = base[s0 + k * L];
-        regs[k] = p != nullptr && idx < len ? gptr(p)[idx] : 1.0;
+        regs[k] = p != nullptr && idx >= 0 && idx < len ? gptr(p)[idx] : 1.0;
     }
+}
+
+/* The offset, in doubles within a run of R, of every valid lane's row p
+ * when all of them share it (so staged runs can start on whole lines), else
+ * 0; q, when given, must share the same offset. */
+template <int R>
+__device__ static inline int line_shift(const double* p, bool valid, const double* q = nullptr)
+{
+    const int sl = (int)(((unsigned long)p >> 3) & (R - 1));
+    const int ql = q ? (int)(((unsigned long)q >> 3) & (R - 1)) : sl;
+    const int s0 = __builtin_amdgcn_readfirstlane(sl);
+    return __builtin_amdgcn_ballot_w64(valid && (sl != s0 || ql != s0)) == 0 ? s0 : 0;
 }
 
 template <int R>
@@ -472,9 +484,12 @@ __device__ static inline bool solve_forward(const oco_sys& d, bool valid, const 
     constexpr int R = STAGE::rows;
     constexpr bool EM = STAGE::emergent;
     const int lane = lane_id();
+    /* alpha staged at j + as: runs on whole lines (line_shift) */
+    int as = 0;
     if constexpr (!EM) {
         wave_sync();   /* the previous system's flushes have read the buffers */
         S->base[2][lane] = valid ? d.A : nullptr;
+        as = line_shift<R>(d.A, valid);
     }
 
     /* ---- forward elimination (source/auxiliary.f90:25-41) ----
@@ -498,7 +513,14 @@ __device__ static inline bool solve_forward(const oco_sys& d, bool valid, const 
     bet = bv;
     u = dv / bet;
     scr_put<alpha_reload<STAGE>()>(scr, 0, 0.0, u, pB.alpha1);
-    if constexpr (!EM) stage_put<R>(S->ub, 0, pB.alpha);
+    if constexpr (!EM) {
+        stage_put<R>(S->ub, as, pB.alpha);
+        if (as == R - 1) {   /* a shifted first run that ends at row 0 */
+            wave_sync();
+            stage_flush<R, STAGE::nt>(S->ub, S->base[2], 0, nz, as);
+            wave_sync();
+        }
+    }
     double aA = pB.alpha;   /* alpha of row j-1 */
     pB = pC;                /* row j = 1 */
     pC = point_sys<GREY, WS>(d, 2, rec, w, M);
@@ -517,10 +539,10 @@ __device__ static inline bool solve_forward(const oco_sys& d, bool valid, const 
         u = (dv - av * u) / bet;
         scr_put<alpha_reload<STAGE>()>(scr, j, gam, u, pB.alpha1);
         if constexpr (!EM) {
-            stage_put<R>(S->ub, j, pB.alpha);
-            if ((j & (R - 1)) == R - 1) {
+            stage_put<R>(S->ub, j + as, pB.alpha);
+            if (((j + as) & (R - 1)) == R - 1) {
                 wave_sync();
-                stage_flush<R, STAGE::nt>(S->ub, S->base[2], j / R, nz);
+                stage_flush<R, STAGE::nt>(S->ub, S->base[2], (j + as) / R, nz, as);
                 wave_sync();
             }
         }
@@ -543,10 +565,10 @@ __device__ static inline bool solve_forward(const oco_sys& d, bool valid, const 
         u = (dv - av * u) / bet;
         scr_put<alpha_reload<STAGE>()>(scr, j, gam, u, pB.alpha1);
         if constexpr (!EM) {
-            stage_put<R>(S->ub, j, pB.alpha);
-            if ((j & (R - 1)) == R - 1) {
+            stage_put<R>(S->ub, j + as, pB.alpha);
+            if (((j + as) & (R - 1)) == R - 1) {
                 wave_sync();
-                stage_flush<R, STAGE::nt>(S->ub, S->base[2], j / R, nz);
+                stage_flush<R, STAGE::nt>(S->ub, S->base[2], (j + as) / R, nz, as);
                 wave_sync();
             }
         }
@@ -564,9 +586,9 @@ __device__ static inline bool solve_forward(const oco_sys& d, bool valid, const 
         }
         scr_put<alpha_reload<STAGE>()>(scr, j, gam, u, pC.alpha1);
         if constexpr (!EM) {
-            stage_put<R>(S->ub, j, pC.alpha);
+            stage_put<R>(S->ub, j + as, pC.alpha);
             wave_sync();
-            stage_flush<R, STAGE::nt>(S->ub, S->base[2], j / R, nz);   /* last (possibly partial) block */
+            stage_flush<R, STAGE::nt>(S->ub, S->base[2], (j + as) / R, nz, as);   /* last (possibly partial) block */
         }
     }
     (void)u;
@@ -593,17 +615,19 @@ __device__ static inline void solve_backward(const oco_sys& d, bool valid, const
         S->base[0][lane] = valid ? d.U : nullptr;
         S->base[1][lane] = valid ? d.V : nullptr;
     }
-    /* U runs in whole lines: U(e) is staged at e + us, so each flushed run
-     * of R doubles starts on an R-double boundary of memory, when every valid
-     * lane's U row has the same offset `us` in its line (the wavefront's
-     * systems are one column's consecutive wavelengths, ncol (nz + 6) doubles
-     * apart); otherwise us = 0.  Measured on MI355X, FP64 drop-in, 256 x 256
-     * columns, nz = 512 (U rows of 518 doubles), nw = 128: 74.2 -> 71.1 ms. */
-    int us = 0;
+    /* Runs in whole lines: U(e) is staged at e + us and V(z) (with the
+     * alpha it replaces) at z + vs, so each flushed run of R doubles starts
+     * on an R-double boundary of memory, when every valid lane's row has the
+     * same offset in its line (the wavefront's systems are one column's
+     * consecutive wavelengths, ncol (nz + 6) or ncol nz doubles apart);
+     * otherwise 0.  Measured on MI355X, FP64 drop-in, 256 x 256 columns,
+     * nw = 128: nz = 512 (U rows of 518 doubles) 74.2 -> 71.1 ms; nz = 500
+     * and 504, whose V and alpha rows are not whole lines either, 74.3 ->
+     * 69.0 and 74.0 -> 70.5 ms (nz = 496: 68.2-70.4 ms). */
+    int us = 0, vs = 0;
     if constexpr (!EM) {
-        const int sl = (int)(((unsigned long)d.U >> 3) & (R - 1));
-        const int s0 = __builtin_amdgcn_readfirstlane(sl);
-        us = __builtin_amdgcn_ballot_w64(valid && sl != s0) == 0 ? s0 : 0;
+        us = line_shift<R>(d.U, valid);
+        vs = line_shift<R>(d.V, valid, alpha_reload<STAGE>() == OCO_A1_STAGED ? d.A : nullptr);
     }
 
     /* ---- back-substitution (:43-45) fused with update_ghosts (:49-62),
@@ -617,15 +641,15 @@ __device__ static inline void solve_backward(const oco_sys& d, bool valid, const
      * replaces it; apf holds the next block's alpha, in flight */
     double apf[ST ? R : 1];
     if constexpr (ST) {
-        const int bt = (nz - 1) / R;
-        alpha_fetch<R>(apf, S->base[2], bt, nz);
+        const int bt = (nz - 1 + vs) / R;
+        alpha_fetch<R>(apf, S->base[2], bt, nz, vs);
         alpha_park<R>(S->vb, apf);
-        if (bt >= 1) alpha_fetch<R>(apf, S->base[2], bt - 1, nz);
+        if (bt >= 1) alpha_fetch<R>(apf, S->base[2], bt - 1, nz, vs);
         wave_sync();
     }
     /* 1/alpha of V row e (a1p: the register pipeline's value) */
     auto a1v = [&](int e, double a1p) -> double {
-        if constexpr (ST) return 1.0 / S->vb[lane * (R + 1) + (e & (R - 1))];
+        if constexpr (ST) return 1.0 / S->vb[lane * (R + 1) + ((e + vs) & (R - 1))];
         else return a1p;
     };
     /* V block b has been flushed: park block b - 1's alpha, fetch b - 2's */
@@ -634,7 +658,7 @@ __device__ static inline void solve_backward(const oco_sys& d, bool valid, const
             if (b >= 1) {
                 alpha_park<R>(S->vb, apf);
                 wave_sync();
-                if (b >= 2) alpha_fetch<R>(apf, S->base[2], b - 2, nz);
+                if (b >= 2) alpha_fetch<R>(apf, S->base[2], b - 2, nz, vs);
             }
         }
     };
@@ -676,12 +700,12 @@ __device__ static inline void solve_backward(const oco_sys& d, bool valid, const
         if constexpr (EM) {
             if (valid) *gptr(d.V + 1) = vtop;
         } else {
-            stage_put<R>(S->vb, nz - 1, vtop);
-            if (((nz - 1) & (R - 1)) == 0) {
+            stage_put<R>(S->vb, nz - 1 + vs, vtop);
+            if (((nz - 1 + vs) & (R - 1)) == 0) {
                 wave_sync();
-                stage_flush<R, STAGE::nt>(S->vb, S->base[1], (nz - 1) / R, nz);
+                stage_flush<R, STAGE::nt>(S->vb, S->base[1], (nz - 1 + vs) / R, nz, vs);
                 wave_sync();
-                vflushed((nz - 1) / R);
+                vflushed((nz - 1 + vs) / R);
             }
         }
     }
@@ -715,18 +739,14 @@ __device__ static inline void solve_backward(const oco_sys& d, bool valid, const
                     if constexpr (!EM) {
                         stage_put<R>(S->ub, e + us, Uj);
                         double df = c60 * (((45.0 * (w4 - w2)) - (9.0 * (w5 - w1))) + (w6 - w0));
-                        stage_put<R>(S->vb, e, ((-1.0 * a1v(e, a1q3)) * df) * dz1);
-                        if (us != 0 && ((e + us) & (R - 1)) == 0) {
+                        stage_put<R>(S->vb, e + vs, ((-1.0 * a1v(e, a1q3)) * df) * dz1);
+                        const bool ue = ((e + us) & (R - 1)) == 0, ve = ((e + vs) & (R - 1)) == 0;
+                        if (ue || ve) {
                             wave_sync();
-                            stage_flush<R, STAGE::nt>(S->ub, S->base[0], (e + us) / R, nz + 6, us);
+                            if (ue) stage_flush<R, STAGE::nt>(S->ub, S->base[0], (e + us) / R, nz + 6, us);
+                            if (ve) stage_flush<R, STAGE::nt>(S->vb, S->base[1], (e + vs) / R, nz, vs);
                             wave_sync();
-                        }
-                        if ((e & (R - 1)) == 0) {
-                            wave_sync();
-                            if (us == 0) stage_flush<R, STAGE::nt>(S->ub, S->base[0], e / R, nz + 6);
-                            stage_flush<R, STAGE::nt>(S->vb, S->base[1], e / R, nz);
-                            wave_sync();
-                            vflushed(e / R);
+                            if (ve) vflushed((e + vs) / R);
                         }
                     }
                 }
@@ -761,14 +781,21 @@ __device__ static inline void solve_backward(const oco_sys& d, bool valid, const
             if constexpr (EM) {
                 if (valid) *gptr(d.V) = v;
             } else {
-                stage_put<R>(S->vb, 2 - k, v);
+                const int z = 2 - k;
+                stage_put<R>(S->vb, z + vs, v);
+                if (z + vs >= R && ((z + vs) & (R - 1)) == 0) {   /* a shifted block that ends here */
+                    wave_sync();
+                    stage_flush<R, STAGE::nt>(S->vb, S->base[1], (z + vs) / R, nz, vs);
+                    wave_sync();
+                    vflushed((z + vs) / R);
+                }
             }
         }
     }
     if constexpr (!EM) {
         wave_sync();
         stage_flush<R, STAGE::nt>(S->ub, S->base[0], 0, nz + 6, us);
-        stage_flush<R, STAGE::nt>(S->vb, S->base[1], 0, nz);
+        stage_flush<R, STAGE::nt>(S->vb, S->base[1], 0, nz, vs);
     }
 }
 
