@@ -157,13 +157,14 @@ def test_c4_slice_fp64(ctx, where):
 
 
 @pytest.mark.parametrize("nz", [37, 101])
-def test_u_runs_shifted_to_lines(ctx, nz):
-    """The drop-in stages U shifted by the offset of its rows in a 128-byte
-    line when a wavefront's 64 systems share that offset: with 4 x 4 columns
-    and nw = 64, a column's rows are 16 (nz + 6) doubles apart, so every
-    column's wavefront shifts, and with nz + 6 odd the 16 columns take all 16
-    shifts (including 14 and 15, whose first block ends in the bottom
-    ghosts).  Bit-exact against the oracle."""
+def test_runs_shifted_to_lines(ctx, nz):
+    """The drop-in stages U, V and alpha shifted by the offset of their rows
+    in a 128-byte line when a wavefront's 64 systems share that offset: with
+    4 x 4 columns and nw = 64, a column's rows are 16 (nz + 6) and 16 nz
+    doubles apart, so every column's wavefront shifts, and with nz odd the 16
+    columns take all 16 shifts of each array (including 14 and 15, whose
+    first block ends in the bottom ghosts or rows, and 15 for alpha, whose
+    first block is row 0 alone).  Bit-exact against the oracle."""
     from ocotillo_amd.workloads import log_waves
     from oracle.oracle import Oracle
     print("case u shifts nz", nz, flush=True)
