@@ -179,6 +179,28 @@ def test_runs_shifted_to_lines(ctx, nz):
     assert np.array_equal(A, pA) and np.array_equal(V, pV) and np.array_equal(U, pU)
 
 
+@pytest.mark.parametrize("seed", range(6))
+def test_random_shapes(ctx, seed):
+    """Seeded block shapes (nx, ny 1-9, nz 4-140, nw 1-150, so tiles of 64
+    wavelengths span columns and rows start anywhere in a line): bit-exact
+    against the oracle."""
+    from ocotillo_amd.workloads import log_waves
+    from oracle.oracle import Oracle
+    rng = np.random.default_rng(900 + seed)
+    nx, ny, nz, nw = (int(rng.integers(1, 10)), int(rng.integers(1, 10)), int(rng.integers(4, 141)),
+                      int(rng.integers(1, 151)))
+    print("case random", seed, (nx, ny, nz, nw), flush=True)
+    rho, T, dz = c2_block(nx, ny, nz=nz, seed=int(rng.integers(1 << 30)))
+    waves = log_waves(nw)
+    ctx.set_wavelengths(waves)
+    ctx.set_physics(2e4, False, 0.4)
+    U, V, A, d = ctx.solve_block(rho, T, dz)
+    pU, pV, pA, pd = Oracle(waves, portable=True).solve_block(rho, T, dz)
+    assert d.status == pd.status
+    if d.status == 0:
+        assert np.array_equal(A, pA) and np.array_equal(V, pV) and np.array_equal(U, pU)
+
+
 @pytest.mark.parametrize("nz", [2, 3])
 def test_tiny_columns(ctx, nz):
     """Columns of 2 and 3 cells (the FP64 drop-in's one-thread-per-system
