@@ -1085,7 +1085,7 @@ struct oco_batch {
     int cls_k[OCO_BATCH_CLASSES] = {0};
     /* per precision mode: autotuned kernel variant and grid, captured graph
      * and the context's fully_ionized_T it was captured with (FP64 variants
-     * 0..2: lane per system with 1, 2, 4 back-substitution rows per chunk;
+     * 0..2: lane per system with 2, 4, 8 back-substitution rows per chunk;
      * mixed: the classes' indices into BATCH_WAVE_K as base-3 digits, class 0
      * the most significant) */
     int nblocks[2] = {0, 0}, variant[2] = {1, 0}, tuned[2] = {0, 0};
